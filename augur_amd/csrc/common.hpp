@@ -1,0 +1,94 @@
+// Shared helpers for the augur HIP library: status codes, the thread-local
+// last-error string, launch checking and small device utilities.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "../../include/augur.h"
+
+namespace augur {
+
+void set_error(const char* fmt, ...);
+
+#define AUGUR_REQUIRE(cond, code, ...)        \
+  do {                                        \
+    if (!(cond)) {                            \
+      ::augur::set_error(__VA_ARGS__);        \
+      return (code);                          \
+    }                                         \
+  } while (0)
+
+#define AUGUR_HIP(call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::augur::set_error("%s failed: %s", #call, hipGetErrorString(e_));       \
+      return AUGUR_EHIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+#define AUGUR_LAUNCHED(name)                                                   \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) {                                                    \
+      ::augur::set_error("launch of %s failed: %s", name, hipGetErrorString(e_)); \
+      return AUGUR_EHIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+inline int ilog2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) ++l;
+  return l;
+}
+
+inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
+
+// Carves aligned sub-buffers out of a caller-provided workspace.
+struct Carver {
+  char* base;
+  size_t cap, used = 0;
+  Carver(void* b, size_t c) : base(static_cast<char*>(b)), cap(c) {}
+  template <class T>
+  T* take(size_t count) {
+    size_t off = align_up(used);
+    used = off + count * sizeof(T);
+    return (base && used <= cap) ? reinterpret_cast<T*>(base + off) : nullptr;
+  }
+  bool ok() const { return base != nullptr && used <= cap; }
+};
+
+// Grid-stride bound for 1D launches: enough blocks to fill 256 CUs several
+// times over without exceeding the work.
+inline unsigned blocks_for(int64_t n, int threads, int64_t cap = 65536) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+}  // namespace augur
+
+// ---------------------------------------------------------------- device side
+namespace augur_dev {
+
+__device__ __forceinline__ int wrap(int i, int G) {  // G is a power of two
+  return i & (G - 1);
+}
+
+// CIC base coordinate u = (p + off)/h - 0.5 with the reference's rounding
+// sequence: rounded add, IEEE divide, rounded subtract, no contraction.
+__device__ __forceinline__ float cic_u(float p, float off, float h) {
+  float t = __fadd_rn(p, off);
+  t = __fdiv_rn(t, h);
+  return __fsub_rn(t, 0.5f);
+}
+
+}  // namespace augur_dev

@@ -1,0 +1,480 @@
+// LDS-staged radix-4/2 Stockham FFT for gfx950, batched over lines of a 2D or
+// 3D grid (x fastest), with the periodic Green's-function Poisson solve,
+// spectral gradient, open-boundary softened kernel and FD gradient.
+//
+// One workgroup transforms a tile of lines held in LDS:
+//   rows pass (x axis):  L = 4096/N contiguous lines, loaded row by row;
+//   cols pass (y or z):  W adjacent columns (consecutive x) of N elements,
+//                        loaded so that each LDS row is one coalesced segment.
+// The in-LDS transform is Stockham autosort (no bit reversal) with twiddles
+// from a per-workgroup table computed in FP64 and rounded to FP32.
+#include "internal.hpp"
+
+using namespace augur;
+
+namespace augur {
+
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_TILE = 4096;  // complex elements per workgroup
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+template <int N>
+__device__ __forceinline__ void make_twiddles(float2* tw) {
+  // tw[t] = exp(-2 pi i t / N)
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    double s, c;
+    sincospi(2.0 * (double)t / (double)N, &s, &c);
+    tw[t] = make_float2((float)c, (float)-s);
+  }
+}
+
+// Transforms `nl` lines of length N stored at a + line*ls (and b as scratch).
+// Returns the buffer holding the result.  Caller must __syncthreads() before.
+template <int N, bool INV>
+__device__ float2* lds_fft(float2* a, float2* b, const float2* tw, int nl, int ls) {
+  float2* src = a;
+  float2* dst = b;
+  int Ns = 1;
+  constexpr int LOGN = (N <= 1) ? 0 : 31 - __builtin_clz(N);
+#pragma unroll 1
+  for (int st = 0; st < LOGN / 2; ++st) {
+    constexpr int Q = N / 4;
+    const int nb = nl * Q;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+      const int line = t / Q, j = t - line * Q;
+      const float2* x = src + line * ls;
+      float2* y = dst + line * ls;
+      float2 v0 = x[j], v1 = x[j + Q], v2 = x[j + 2 * Q], v3 = x[j + 3 * Q];
+      const int k = j & (Ns - 1);
+      if (k) {
+        const int ts = k * (N / (Ns * 4));
+        float2 w1 = tw[ts], w2 = tw[2 * ts], w3 = tw[3 * ts];
+        if (INV) {
+          w1.y = -w1.y;
+          w2.y = -w2.y;
+          w3.y = -w3.y;
+        }
+        v1 = cmul(v1, w1);
+        v2 = cmul(v2, w2);
+        v3 = cmul(v3, w3);
+      }
+      const float2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3);
+      float2 t3 = csub(v1, v3);
+      // forward: t3 * (-i); inverse: t3 * (+i)
+      t3 = INV ? make_float2(-t3.y, t3.x) : make_float2(t3.y, -t3.x);
+      const int d = (j - k) * 4 + k;
+      y[d] = cadd(t0, t2);
+      y[d + Ns] = cadd(t1, t3);
+      y[d + 2 * Ns] = csub(t0, t2);
+      y[d + 3 * Ns] = csub(t1, t3);
+    }
+    __syncthreads();
+    float2* tmp = src;
+    src = dst;
+    dst = tmp;
+    Ns *= 4;
+  }
+  if (LOGN & 1) {
+    constexpr int Q = N / 2;
+    const int nb = nl * Q;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+      const int line = t / Q, j = t - line * Q;
+      const float2* x = src + line * ls;
+      float2* y = dst + line * ls;
+      float2 v0 = x[j], v1 = x[j + Q];
+      const int k = j & (Ns - 1);
+      if (k) {
+        float2 w = tw[k * (N / (Ns * 2))];
+        if (INV) w.y = -w.y;
+        v1 = cmul(v1, w);
+      }
+      const int d = (j - k) * 2 + k;
+      y[d] = cadd(v0, v1);
+      y[d + Ns] = csub(v0, v1);
+    }
+    __syncthreads();
+    float2* tmp = src;
+    src = dst;
+    dst = tmp;
+  }
+  return src;
+}
+
+// ---------------------------------------------------------------- rows R2C
+// Real rows of the FFT shape; line l = (b, outer...) with `rows_per_batch`
+// rows per batch entry.  A row exists in the input when its outer coordinates
+// are inside in_shape; columns beyond in_nx read as zero.
+struct RowMap {
+  int64_t rows_per_batch;  // prod(shape[:-1])
+  int64_t n_rows;          // batch * rows_per_batch
+  int ny, nz;              // FFT outer extents (nz = 1 for 2D)
+  int in_ny, in_nz, in_nx; // input extents
+};
+
+template <int N>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows_r2c(const float* __restrict__ in, float2* __restrict__ out,
+                                                              RowMap m) {
+  constexpr int L = FFT_TILE / N;
+  constexpr int LS = N + 1;
+  __shared__ float2 bufa[L * LS], bufb[L * LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const int nxc = N / 2 + 1;
+  for (int e = threadIdx.x; e < L * N; e += FFT_THREADS) {
+    const int l = e / N, x = e - l * N;
+    const int64_t row = row0 + l;
+    float v = 0.0f;
+    if (row < m.n_rows && x < m.in_nx) {
+      const int64_t b = row / m.rows_per_batch;
+      const int64_t r = row - b * m.rows_per_batch;
+      const int y = (int)(r % m.ny), z = (int)(r / m.ny);
+      if (y < m.in_ny && z < m.in_nz) v = in[((b * m.in_nz + z) * m.in_ny + y) * (int64_t)m.in_nx + x];
+    }
+    bufa[l * LS + x] = make_float2(v, 0.0f);
+  }
+  __syncthreads();
+  float2* r = lds_fft<N, false>(bufa, bufb, tw, L, LS);
+  for (int e = threadIdx.x; e < L * nxc; e += FFT_THREADS) {
+    const int l = e / nxc, k = e - l * nxc;
+    const int64_t row = row0 + l;
+    if (row < m.n_rows) out[row * nxc + k] = r[l * LS + k];
+  }
+}
+
+// ---------------------------------------------------------------- rows C2R
+template <int N>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows_c2r(const float2* __restrict__ in, float* __restrict__ out,
+                                                              RowMap m, float scale) {
+  // m.in_* here describe the cropped OUTPUT extents.
+  constexpr int L = FFT_TILE / N;
+  constexpr int LS = N + 1;
+  __shared__ float2 bufa[L * LS], bufb[L * LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const int nxc = N / 2 + 1;
+  for (int e = threadIdx.x; e < L * N; e += FFT_THREADS) {
+    const int l = e / N, k = e - l * N;
+    const int64_t row = row0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < m.n_rows) {
+      if (k < nxc) {
+        v = in[row * nxc + k];
+      } else {
+        v = in[row * nxc + (N - k)];
+        v.y = -v.y;
+      }
+      // C2R ignores the imaginary parts of the DC and Nyquist bins
+      if (k == 0 || k == N / 2) v.y = 0.0f;
+    }
+    bufa[l * LS + k] = v;
+  }
+  __syncthreads();
+  float2* r = lds_fft<N, true>(bufa, bufb, tw, L, LS);
+  for (int e = threadIdx.x; e < L * N; e += FFT_THREADS) {
+    const int l = e / N, x = e - l * N;
+    const int64_t row = row0 + l;
+    if (row >= m.n_rows || x >= m.in_nx) continue;
+    const int64_t b = row / m.rows_per_batch;
+    const int64_t rr = row - b * m.rows_per_batch;
+    const int y = (int)(rr % m.ny), z = (int)(rr / m.ny);
+    if (y < m.in_ny && z < m.in_nz) out[((b * m.in_nz + z) * m.in_ny + y) * (int64_t)m.in_nx + x] = r[l * LS + x].x * scale;
+  }
+}
+
+// ------------------------------------------------------ strided columns C2C
+// Lines along a strided axis: for outer index o in [0, n_outer) the line base
+// is (o / q) * s1 + (o % q) * s2, element stride es; columns c in [0, ncols)
+// are adjacent (stride 1).  In-place.
+struct ColMap {
+  int64_t n_outer, q, s1, s2, es;
+  int ncols;
+};
+
+template <int N, bool INV>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(float2* __restrict__ data, ColMap m) {
+  constexpr int W0 = FFT_TILE / N;
+  constexpr int W = W0 > 64 ? 64 : W0;
+  constexpr int LS = N + 1;
+  __shared__ float2 bufa[W * LS], bufb[W * LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  const int nblk = (m.ncols + W - 1) / W;
+  const int64_t o = blockIdx.x / nblk;
+  const int c0 = (int)(blockIdx.x - o * nblk) * W;
+  const int64_t base = (o / m.q) * m.s1 + (o % m.q) * m.s2 + c0;
+  for (int e = threadIdx.x; e < W * N; e += FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    float2 v = make_float2(0.f, 0.f);
+    if (c0 + c < m.ncols) v = data[base + k * m.es + c];
+    bufa[c * LS + k] = v;
+  }
+  __syncthreads();
+  float2* r = lds_fft<N, INV>(bufa, bufb, tw, W, LS);
+  for (int e = threadIdx.x; e < W * N; e += FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    if (c0 + c < m.ncols) data[base + k * m.es + c] = r[c * LS + k];
+  }
+}
+
+template <bool INV>
+int launch_cols(int n, float2* data, const ColMap& m, hipStream_t s) {
+  TimedScope ts_(INV ? "fft_cols_inv" : "fft_cols_fwd", s);
+  const int W0 = FFT_TILE / n;
+  const int W = W0 > 64 ? 64 : W0;
+  const int64_t blocks = m.n_outer * ((m.ncols + W - 1) / W);
+  AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "fft: too many lines");
+  dim3 g((unsigned)blocks), b(FFT_THREADS);
+  switch (n) {
+#define CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_fft_cols<NN, INV>), g, b, 0, s, data, m); break;
+    CASE(4) CASE(8) CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512)
+#undef CASE
+    default: set_error("fft: unsupported length %d", n); return AUGUR_EUNSUPPORTED;
+  }
+  AUGUR_LAUNCHED("fft_cols");
+  return AUGUR_OK;
+}
+
+int launch_rows_r2c(int n, const float* in, float2* out, const RowMap& m, hipStream_t s) {
+  TimedScope ts_("fft_rows_r2c", s);
+  const int L = FFT_TILE / n;
+  dim3 g((unsigned)((m.n_rows + L - 1) / L)), b(FFT_THREADS);
+  switch (n) {
+#define CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_fft_rows_r2c<NN>), g, b, 0, s, in, out, m); break;
+    CASE(4) CASE(8) CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512)
+#undef CASE
+    default: set_error("fft: unsupported length %d", n); return AUGUR_EUNSUPPORTED;
+  }
+  AUGUR_LAUNCHED("fft_rows_r2c");
+  return AUGUR_OK;
+}
+
+int launch_rows_c2r(int n, const float2* in, float* out, const RowMap& m, float scale, hipStream_t s) {
+  TimedScope ts_("fft_rows_c2r", s);
+  const int L = FFT_TILE / n;
+  dim3 g((unsigned)((m.n_rows + L - 1) / L)), b(FFT_THREADS);
+  switch (n) {
+#define CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_fft_rows_c2r<NN>), g, b, 0, s, in, out, m, scale); break;
+    CASE(4) CASE(8) CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512)
+#undef CASE
+    default: set_error("fft: unsupported length %d", n); return AUGUR_EUNSUPPORTED;
+  }
+  AUGUR_LAUNCHED("fft_rows_c2r");
+  return AUGUR_OK;
+}
+
+static int check_shape(int ndim, const int64_t* shape) {
+  AUGUR_REQUIRE(ndim == 2 || ndim == 3, AUGUR_EINVAL, "fft: ndim must be 2 or 3");
+  AUGUR_REQUIRE(shape != nullptr, AUGUR_EINVAL, "fft: null shape");
+  for (int d = 0; d < ndim; ++d)
+    AUGUR_REQUIRE(is_pow2(shape[d]) && shape[d] >= 4 && shape[d] <= 512, AUGUR_EUNSUPPORTED,
+                  "fft: extents must be powers of two in [4, 512] (got %lld)", (long long)shape[d]);
+  return AUGUR_OK;
+}
+
+// Forward/inverse C2C over the strided axes of a (batch, [nz,] ny, nxc) spectrum.
+template <bool INV>
+int spectrum_cols(float2* spec, int ndim, const int64_t* shape, int64_t batch, hipStream_t s) {
+  const int nxc = (int)(shape[ndim - 1] / 2 + 1);
+  const int64_t ny = shape[ndim - 2];
+  const int64_t nz = ndim == 3 ? shape[0] : 1;
+  // y axis: outer = (b, z), base = o * ny * nxc, element stride nxc
+  ColMap my{batch * nz, 1, ny * nxc, 0, nxc, nxc};
+  // z axis: outer = (b, y), base = b * nz*ny*nxc + y * nxc, element stride ny*nxc
+  ColMap mz{batch * ny, ny, nz * ny * nxc, nxc, ny * nxc, nxc};
+  int rc;
+  if (!INV) {
+    rc = launch_cols<INV>((int)ny, spec, my, s);
+    if (rc) return rc;
+    if (ndim == 3) rc = launch_cols<INV>((int)nz, spec, mz, s);
+  } else {
+    if (ndim == 3) {
+      rc = launch_cols<INV>((int)nz, spec, mz, s);
+      if (rc) return rc;
+    }
+    rc = launch_cols<INV>((int)ny, spec, my, s);
+  }
+  return rc;
+}
+
+// ----------------------------------------------------------- Poisson solve
+__global__ void k_poisson(const float2* __restrict__ rho, int64_t B, int G, float kf, float green, float scale,
+                          float2* __restrict__ acc, float2* __restrict__ phi) {
+  const int nxc = G / 2 + 1;
+  const int64_t per = (int64_t)G * G * nxc;
+  const int64_t tot = B * per;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / per;
+    const int64_t e = i - b * per;
+    const int kx = (int)(e % nxc);
+    const int ky = (int)((e / nxc) % G);
+    const int kz = (int)(e / ((int64_t)nxc * G));
+    const int ny = ky <= G / 2 ? ky : ky - G;
+    const int nz = kz <= G / 2 ? kz : kz - G;
+    const float fx = kf * kx, fy = kf * ny, fz = kf * nz;
+    const float k2 = fx * fx + fy * fy + fz * fz;
+    float2 p = make_float2(0.f, 0.f);
+    if (e != 0) {
+      const float g = green * scale / k2;  // -4 pi G / k^2 times the output scale
+      const float2 r = rho[i];
+      p = make_float2(r.x * g, r.y * g);
+    }
+    if (phi) phi[i] = p;
+    // a_d = -i k_d phi = (k_d * phi.im, -k_d * phi.re); Nyquist of axis d zeroed
+    const float ax = (kx == G / 2) ? 0.f : fx;
+    const float ay = (ky == G / 2) ? 0.f : fy;
+    const float az = (kz == G / 2) ? 0.f : fz;
+    float2* out = acc + b * 3 * per + e;
+    out[0] = make_float2(ax * p.y, -ax * p.x);
+    out[per] = make_float2(ay * p.y, -ay * p.x);
+    out[2 * per] = make_float2(az * p.y, -az * p.x);
+  }
+}
+
+__global__ void k_complex_mul(const float2* __restrict__ a, const float2* __restrict__ b, float2* __restrict__ out,
+                              int64_t n, int64_t batch, float scale) {
+  const int64_t tot = n * batch;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+    float2 p = cmul(a[i], b[i % n]);
+    out[i] = make_float2(p.x * scale, p.y * scale);
+  }
+}
+
+__global__ void k_open_kernel_2d(int G, float h, float eps2, float* out) {
+  const int n = 2 * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) {
+    const int iy = i / n, ix = i % n;
+    // reference: d = min(idx, 2G - idx) * h in the tensor dtype (FP32)
+    const float dy = __fmul_rn((float)min(iy, n - iy), h), dx = __fmul_rn((float)min(ix, n - ix), h);
+    const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dy, dy), __fmul_rn(dx, dx)), eps2);
+    out[i] = __fdiv_rn(-1.0f, __fsqrt_rn(r2));
+  }
+}
+
+__global__ void k_neg_grad_fd(const float* __restrict__ phi, int64_t B, int G, float twoh, float* __restrict__ out) {
+  const int64_t per = (int64_t)G * G;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < B * per; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / per;
+    const int e = (int)(i - b * per);
+    const int y = e / G, x = e % G;
+    const float* p = phi + b * per;
+    const float xp = x + 1 < G ? p[e + 1] : 0.f, xm = x > 0 ? p[e - 1] : 0.f;
+    const float yp = y + 1 < G ? p[e + G] : 0.f, ym = y > 0 ? p[e - G] : 0.f;
+    // reference: -(p[i+1] - p[i-1]) / (2h), IEEE divide
+    out[b * 2 * per + e] = -__fdiv_rn(__fsub_rn(xp, xm), twoh);
+    out[b * 2 * per + per + e] = -__fdiv_rn(__fsub_rn(yp, ym), twoh);
+  }
+}
+
+}  // namespace augur
+
+extern "C" {
+
+size_t augur_fft_workspace_bytes(int ndim, const int64_t* shape, int64_t batch) {
+  (void)ndim;
+  (void)shape;
+  (void)batch;
+  return 0;  // the LDS kernels need no global scratch
+}
+
+int augur_fft_r2c(const float* in, const int64_t* in_shape, float* out, int ndim, const int64_t* shape,
+                  int64_t batch, void* ws, size_t ws_bytes, void* stream) {
+  (void)ws;
+  (void)ws_bytes;
+  int rc = check_shape(ndim, shape);
+  if (rc) return rc;
+  AUGUR_REQUIRE(batch >= 1 && in && out, AUGUR_EINVAL, "fft_r2c: bad arguments");
+  const int64_t* is = in_shape ? in_shape : shape;
+  for (int d = 0; d < ndim; ++d)
+    AUGUR_REQUIRE(is[d] >= 1 && is[d] <= shape[d], AUGUR_EINVAL, "fft_r2c: in_shape must be within shape");
+  hipStream_t s = as_stream(stream);
+  RowMap m;
+  m.ny = (int)shape[ndim - 2];
+  m.nz = ndim == 3 ? (int)shape[0] : 1;
+  m.rows_per_batch = (int64_t)m.ny * m.nz;
+  m.n_rows = batch * m.rows_per_batch;
+  m.in_nx = (int)is[ndim - 1];
+  m.in_ny = (int)is[ndim - 2];
+  m.in_nz = ndim == 3 ? (int)is[0] : 1;
+  rc = launch_rows_r2c((int)shape[ndim - 1], in, (float2*)out, m, s);
+  if (rc) return rc;
+  return spectrum_cols<false>((float2*)out, ndim, shape, batch, s);
+}
+
+int augur_fft_c2r(float* in, float* out, const int64_t* out_shape, int ndim, const int64_t* shape, int64_t batch,
+                  float scale, void* ws, size_t ws_bytes, void* stream) {
+  (void)ws;
+  (void)ws_bytes;
+  int rc = check_shape(ndim, shape);
+  if (rc) return rc;
+  AUGUR_REQUIRE(batch >= 1 && in && out, AUGUR_EINVAL, "fft_c2r: bad arguments");
+  const int64_t* os = out_shape ? out_shape : shape;
+  for (int d = 0; d < ndim; ++d)
+    AUGUR_REQUIRE(os[d] >= 1 && os[d] <= shape[d], AUGUR_EINVAL, "fft_c2r: out_shape must be within shape");
+  hipStream_t s = as_stream(stream);
+  rc = spectrum_cols<true>((float2*)in, ndim, shape, batch, s);
+  if (rc) return rc;
+  RowMap m;
+  m.ny = (int)shape[ndim - 2];
+  m.nz = ndim == 3 ? (int)shape[0] : 1;
+  m.rows_per_batch = (int64_t)m.ny * m.nz;
+  m.n_rows = batch * m.rows_per_batch;
+  m.in_nx = (int)os[ndim - 1];
+  m.in_ny = (int)os[ndim - 2];
+  m.in_nz = ndim == 3 ? (int)os[0] : 1;
+  return launch_rows_c2r((int)shape[ndim - 1], (const float2*)in, out, m, scale, s);
+}
+
+int augur_complex_mul(const float* a, const float* b, float* out, int64_t n, int64_t batch, float scale, void* stream) {
+  AUGUR_REQUIRE(n >= 0 && batch >= 1, AUGUR_EINVAL, "complex_mul: bad sizes");
+  if (n == 0) return AUGUR_OK;
+  AUGUR_REQUIRE(a && b && out, AUGUR_EINVAL, "complex_mul: null pointer");
+  hipLaunchKernelGGL(k_complex_mul, dim3(blocks_for(n * batch, 256)), dim3(256), 0, as_stream(stream),
+                     (const float2*)a, (const float2*)b, (float2*)out, n, batch, scale);
+  AUGUR_LAUNCHED("complex_mul");
+  return AUGUR_OK;
+}
+
+int augur_poisson_green_grad(const float* rho_hat, int64_t B, int64_t G, float L, float Gconst, float scale,
+                             float* acc_hat, float* phi_hat, void* stream) {
+  AUGUR_REQUIRE(B >= 1 && is_pow2(G) && G >= 4 && G <= 512 && L > 0.0f, AUGUR_EINVAL,
+                "poisson: need B >= 1, power-of-two G in [4, 512], L > 0");
+  AUGUR_REQUIRE(rho_hat && acc_hat, AUGUR_EINVAL, "poisson: null pointer");
+  const float kf = (float)(2.0 * 3.14159265358979323846 / (double)L);
+  const float green = (float)(-4.0 * 3.14159265358979323846 * (double)Gconst);
+  const int64_t tot = B * G * G * (G / 2 + 1);
+  TimedScope ts_("poisson", as_stream(stream));
+  hipLaunchKernelGGL(k_poisson, dim3(blocks_for(tot, 256)), dim3(256), 0, as_stream(stream), (const float2*)rho_hat, B,
+                     (int)G, kf, green, scale, (float2*)acc_hat, (float2*)phi_hat);
+  AUGUR_LAUNCHED("poisson");
+  return AUGUR_OK;
+}
+
+int augur_open_kernel_2d(int64_t G, float h, float eps, float* out, void* stream) {
+  AUGUR_REQUIRE(G >= 1 && G <= 4096 && out, AUGUR_EINVAL, "open_kernel_2d: bad arguments");
+    // eps^2 as the reference forms it: a Python float squared, then cast to FP32
+  const float eps2 = (float)((double)eps * (double)eps);
+  hipLaunchKernelGGL(k_open_kernel_2d, dim3(blocks_for(4 * G * G, 256)), dim3(256), 0, as_stream(stream), (int)G, h,
+                     eps2, out);
+  AUGUR_LAUNCHED("open_kernel_2d");
+  return AUGUR_OK;
+}
+
+int augur_neg_grad_fd(const float* phi, int64_t B, int64_t G, float h, float* out, void* stream) {
+  AUGUR_REQUIRE(B >= 1 && G >= 1 && h > 0.0f && phi && out, AUGUR_EINVAL, "neg_grad_fd: bad arguments");
+  hipLaunchKernelGGL(k_neg_grad_fd, dim3(blocks_for(B * G * G, 256)), dim3(256), 0, as_stream(stream), phi, B, (int)G,
+                     (float)(2.0 * (double)h), out);
+  AUGUR_LAUNCHED("neg_grad_fd");
+  return AUGUR_OK;
+}
+
+}  // extern "C"

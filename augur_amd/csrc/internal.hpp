@@ -1,0 +1,28 @@
+// Host-side helpers shared between the library's translation units.
+#pragma once
+#include "common.hpp"
+
+namespace augur {
+
+int fill_u64(void* p, int64_t n, unsigned long long v, hipStream_t s);
+int fill_u32(void* p, int64_t n, uint32_t v, hipStream_t s);
+size_t scan_workspace_bytes(int64_t n);
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s);
+size_t radix_workspace_bytes(int64_t n);
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* ws, size_t ws_bytes,
+                     uint32_t** keys_out, uint32_t** vals_out, hipStream_t s);
+
+// Per-launch event timing (see timing.hip).  A TimedScope around a launch
+// records a start/stop event pair on the stream when timing is enabled and
+// does nothing otherwise.
+int timing_start(const char* name, hipStream_t s, int* slot);
+int timing_stop(int slot, hipStream_t s);
+
+struct TimedScope {
+  int slot;
+  hipStream_t s;
+  TimedScope(const char* name, hipStream_t st) : slot(-1), s(st) { timing_start(name, s, &slot); }
+  ~TimedScope() { timing_stop(slot, s); }
+};
+
+}  // namespace augur

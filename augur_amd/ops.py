@@ -1,0 +1,254 @@
+"""Typed wrappers over the C ABI: allocate outputs and workspaces with torch on
+the tensors' device and launch on the current stream.  Every function needs
+its tensors on a ROCm GPU and raises otherwise (there is no CPU path)."""
+import ctypes
+
+import torch
+
+from . import _lib
+
+F32, F64, I64, I32 = torch.float32, torch.float64, torch.int64, torch.int32
+
+
+def _dev(*ts):
+    dev = None
+    for t in ts:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise RuntimeError("augur_amd runs on a ROCm GPU only: got a tensor on " + str(t.device))
+        dev = t.device if dev is None else dev
+    if dev is None:
+        raise RuntimeError("augur_amd: no device tensor given")
+    return dev
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _c(t, dtype):
+    return None if t is None else t.contiguous().to(dtype)
+
+
+def _stream(dev):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _ws(nbytes, dev):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+
+
+def _shape_arr(shape):
+    return (ctypes.c_int64 * len(shape))(*[int(s) for s in shape])
+
+
+# ------------------------------------------------------------------ keys / sort
+def cell_keys(pos, mask, off, cell, C, counts=False):
+    """pos (B,N,D) -> int64 keys (B*N,), optional counts (B*C^D+1,)."""
+    pos = _c(pos, F32)
+    dev = _dev(pos)
+    B, N, D = pos.shape
+    mask = _c(mask, F32) if mask is not None else None
+    keys = torch.empty(B * N, dtype=I64, device=dev)
+    cnt = torch.empty(B * C ** D + 1, dtype=I64, device=dev) if counts else None
+    _lib.call("augur_cell_keys", _p(pos), _p(mask), B, N, D, float(off), float(cell), int(C), _p(keys), _p(cnt),
+              _stream(dev))
+    return (keys, cnt) if counts else keys
+
+
+def cic_base(pos, off, h):
+    """pos (..., D) -> (i0 int32 (..., D), w float32 (..., D))."""
+    pos = _c(pos, F32)
+    dev = _dev(pos)
+    D = pos.shape[-1]
+    n = pos.numel() // D
+    i0 = torch.empty(pos.shape, dtype=I32, device=dev)
+    w = torch.empty(pos.shape, dtype=F32, device=dev)
+    _lib.call("augur_cic_base", _p(pos), n, D, float(off), float(h), _p(i0), _p(w), _stream(dev))
+    return i0, w
+
+
+def sort_by_cell(keys, nkeys, starts=False):
+    """Stable argsort by key -> int64 perm (and starts (nkeys+1,))."""
+    keys = _c(keys, I64)
+    dev = _dev(keys)
+    n = keys.numel()
+    perm = torch.empty(n, dtype=I64, device=dev)
+    st = torch.empty(nkeys + 1, dtype=I64, device=dev) if starts else None
+    wsb = _lib.size("augur_sort_workspace_bytes", n, nkeys)
+    ws = _ws(wsb, dev)
+    _lib.call("augur_sort_by_cell", _p(keys), n, int(nkeys), _p(perm), _p(st), _p(ws), wsb, _stream(dev))
+    return (perm, st) if starts else perm
+
+
+# ------------------------------------------------------------- deposit / gather
+def cic_deposit(pos, mass, G, h, off, periodic, vel=None, mask=None, scale=None, mass_value=1.0):
+    """pos (B,N,D), mass (B,N) or None -> (B, C, G, .., G) float32, C = 1 or 1+D."""
+    pos = _c(pos, F32)
+    dev = _dev(pos)
+    B, N, D = pos.shape
+    mass = _c(mass, F32)
+    vel = _c(vel, F32)
+    mask = _c(mask, F32)
+    C = 1 if vel is None else 1 + D
+    if scale is None:
+        scale = 1.0 / h ** D
+    out = torch.empty((B, C) + (G,) * D, dtype=F32, device=dev)
+    wsb = _lib.size("augur_cic_deposit_workspace_bytes", B, D, G, C)
+    ws = _ws(wsb, dev)
+    _lib.call("augur_cic_deposit", _p(pos), _p(vel), _p(mass), float(mass_value), _p(mask), B, N, D, int(G),
+              float(off), float(h), int(bool(periodic)), float(scale), _p(out), _p(ws), wsb, _stream(dev))
+    return out
+
+
+def cic_gather(field, pos, h, off, periodic):
+    """field (B, C, G..G), pos (B, N, D) -> (B, N, C)."""
+    field = _c(field, F32)
+    pos = _c(pos, F32)
+    dev = _dev(field, pos)
+    B, N, D = pos.shape
+    C, G = field.shape[1], field.shape[-1]
+    out = torch.empty(B, N, C, dtype=F32, device=dev)
+    _lib.call("augur_cic_gather", _p(field), _p(pos), B, N, D, int(G), int(C), float(off), float(h),
+              int(bool(periodic)), _p(out), _stream(dev))
+    return out
+
+
+# ------------------------------------------------------------------------ FFT
+def rfftn(x, s=None):
+    """Real FFT over the last 2 or 3 axes of x (batch dims in front), zero
+    extended to s (the reference's rfft2(x, s=...)).  complex64 output."""
+    x = _c(x, F32)
+    dev = _dev(x)
+    nd = 3 if (s is not None and len(s) == 3) else (2 if s is not None else None)
+    if nd is None:
+        raise ValueError("rfftn: pass s (the FFT shape over the last 2 or 3 axes)")
+    in_shape = x.shape[-nd:]
+    batch = x.numel() // max(1, int(torch.tensor(in_shape).prod()))
+    out = torch.empty(x.shape[:-nd] + tuple(s[:-1]) + (s[-1] // 2 + 1,), dtype=torch.complex64, device=dev)
+    _lib.call("augur_fft_r2c", _p(x), _shape_arr(in_shape), _p(out), nd, _shape_arr(s), batch, None, 0, _stream(dev))
+    return out
+
+
+def irfftn(X, s, out_shape=None, scale=None):
+    """Inverse of rfftn (torch 'backward' normalisation unless scale given),
+    cropped to out_shape.  X is copied (the kernels consume their input)."""
+    Xc = X.contiguous().to(torch.complex64).clone()
+    dev = _dev(Xc)
+    nd = len(s)
+    batch = Xc.numel() // (int(torch.tensor(s[:-1]).prod()) * (s[-1] // 2 + 1))
+    os_ = tuple(out_shape) if out_shape is not None else tuple(s)
+    if scale is None:
+        scale = 1.0 / float(torch.tensor(s).prod())
+    out = torch.empty(X.shape[:-nd] + os_, dtype=F32, device=dev)
+    _lib.call("augur_fft_c2r", _p(Xc), _p(out), _shape_arr(os_), nd, _shape_arr(s), batch, float(scale), None, 0,
+              _stream(dev))
+    return out
+
+
+def complex_mul(a, b, scale=1.0):
+    a = a.contiguous().to(torch.complex64)
+    b = b.contiguous().to(torch.complex64)
+    dev = _dev(a, b)
+    n = b.numel()
+    out = torch.empty_like(a)
+    _lib.call("augur_complex_mul", _p(a), _p(b), _p(out), n, a.numel() // n, float(scale), _stream(dev))
+    return out
+
+
+def poisson_green_grad(rho_hat, G, L, Gconst=1.0, scale=1.0, want_phi=False):
+    rho_hat = rho_hat.contiguous().to(torch.complex64)
+    dev = _dev(rho_hat)
+    B = rho_hat.numel() // (G * G * (G // 2 + 1))
+    acc = torch.empty((B, 3, G, G, G // 2 + 1), dtype=torch.complex64, device=dev)
+    phi = torch.empty((B, G, G, G // 2 + 1), dtype=torch.complex64, device=dev) if want_phi else None
+    _lib.call("augur_poisson_green_grad", _p(rho_hat), B, G, float(L), float(Gconst), float(scale), _p(acc), _p(phi),
+              _stream(dev))
+    return (acc, phi) if want_phi else acc
+
+
+def open_kernel_2d(G, h, eps, device):
+    out = torch.empty(2 * G, 2 * G, dtype=F32, device=device)
+    _lib.call("augur_open_kernel_2d", int(G), float(h), float(eps), _p(out), _stream(torch.device(device)))
+    return out
+
+
+def neg_grad_fd(phi, h):
+    """phi (B, 1, G, G) -> (B, 2, G, G), zero-padded central differences."""
+    phi = _c(phi, F32)
+    dev = _dev(phi)
+    B, G = phi.shape[0], phi.shape[-1]
+    out = torch.empty(B, 2, G, G, dtype=F32, device=dev)
+    _lib.call("augur_neg_grad_fd", _p(phi), B, G, float(h), _p(out), _stream(dev))
+    return out
+
+
+# ------------------------------------------------------------------- oracles
+def _check_targets(tg, n):
+    """Target indices index the source arrays inside the kernels: validate on the host."""
+    if tg is not None and tg.numel() and (int(tg.min()) < 0 or int(tg.max()) >= n):
+        raise ValueError(f"target indices must lie in [0, {n})")
+
+
+def direct_potential_f64(pos, eps, mass=None, targets=None):
+    """FP64 softened potential sums phi_t = sum_{j != t} m_j / sqrt(r^2 + eps^2)."""
+    return direct_sum_f64(pos, eps, mass, targets, potential=True)
+
+
+def direct_sum_f64(pos, eps, mass=None, targets=None, potential=False):
+    """FP64 softened direct sum.  pos (n, D) float32 or float64."""
+    dev = _dev(pos)
+    n, D = pos.shape
+    p64 = _c(pos, F64) if pos.dtype == F64 else None
+    p32 = _c(pos, F32) if pos.dtype != F64 else None
+    m = _c(mass, F64) if mass is not None else None
+    tg = _c(targets, I64) if targets is not None else None
+    _check_targets(tg, n)
+    t = n if tg is None else tg.numel()
+    out = torch.empty((t,) if potential else (t, D), dtype=F64, device=dev)
+    wsb = _lib.size("augur_direct_sum_workspace_bytes", n, t, D)
+    ws = _ws(wsb, dev)
+    name = "augur_direct_potential_f64" if potential else "augur_direct_sum_f64"
+    _lib.call(name, _p(p64), _p(p32), _p(m), n, D, _p(tg), t, float(eps), _p(out), _p(ws), wsb, _stream(dev))
+    return out
+
+
+def ewald_f64(pos, mass, targets, L=1.0, eps=0.0, alpha_l=8.0, kmax=None):
+    """FP64 Ewald (periodic) accelerations at the targets.  pos (n, 3) float32."""
+    import math
+    pos = _c(pos, F32)
+    dev = _dev(pos)
+    n = pos.shape[0]
+    if kmax is None:
+        kmax = int(math.ceil(2 * alpha_l * math.sqrt(18.5) / (2 * math.pi))) + 1
+    m = None
+    mval = 1.0
+    if torch.is_tensor(mass):
+        m = _c(mass, F64)
+    elif mass is not None:
+        mval = float(mass)
+    tg = _c(targets, I64)
+    _check_targets(tg, n)
+    t = tg.numel()
+    out = torch.empty(t, 3, dtype=F64, device=dev)
+    wsb = _lib.size("augur_ewald_workspace_bytes", n, t, kmax)
+    ws = _ws(wsb, dev)
+    _lib.call("augur_ewald_f64", _p(pos), _p(m), mval, n, _p(tg), t, float(L), float(eps), float(alpha_l), int(kmax),
+              _p(out), _p(ws), wsb, _stream(dev))
+    return out
+
+
+def energy_momentum_f64(vel, mass=None, mass_value=1.0, phi=None):
+    """-> float64 tensor [px, py, pz, kinetic, 0.5*sum m phi, sum m]."""
+    vel = _c(vel, F32)
+    dev = _dev(vel)
+    n = vel.shape[0]
+    m = _c(mass, F32)
+    ph = _c(phi, F32)
+    out = torch.empty(6, dtype=F64, device=dev)
+    wsb = _lib.size("augur_reduce_workspace_bytes", n)
+    ws = _ws(wsb, dev)
+    _lib.call("augur_reduce_energy_momentum_f64", _p(vel), _p(m), float(mass_value), _p(ph), n, _p(out), _p(ws), wsb,
+              _stream(dev))
+    return out

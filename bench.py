@@ -1,0 +1,163 @@
+"""Benchmark: plain periodic PM on one MI355X at N = 2^24 particles, 256^3 grid.
+
+One step = one KDK leapfrog step of the whole scene (deposit -> FFT Poisson +
+spectral gradient -> fused CIC gather + kick + drift), inputs resident in HBM.
+Prints one JSON line (rank 0) with value = particle-steps/s over exactly
+--steps timed steps, the roofline of the dominant kernel (HIP events on the
+launch stream inside the timed region), per-stage HBM rates, the force
+rel-RMS against the FP64 Ewald oracle, and the CPU baseline (the PyTorch-CPU
+FP32 restatement of the same step timed on this host).
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "particle-steps/s at N=2^24: surrogate vs plain PM at stated force rel-RMS vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(name, N, G):
+    """Compulsory HBM bytes per launch of each kernel."""
+    G3 = G ** 3
+    spec = 8 * G * G * (G // 2 + 1)
+    return {
+        "gather_kdk": 48 * N + 12 * G3,          # r/w pos+vel, read 3 force grids
+        "dep_fixed": 12 * N,                     # read pos
+        "fft_rows_r2c": 4 * G3 + spec,
+        "fft_cols_fwd": 2 * spec,
+        "poisson": spec + 3 * spec,
+        "fft_cols_inv": 2 * 3 * spec,
+        "fft_rows_c2r": 3 * (spec + 4 * G3),
+    }.get(name)
+
+
+STAGES = {  # stage -> (kernels, algorithmic bytes per step)
+    "deposit": (("fill_u64", "dep_absmax", "dep_fixed", "dep_convert"), lambda N, G: 12 * N + 4 * G ** 3),
+    "fft_poisson_grad": (("fft_rows_r2c", "fft_cols_fwd", "poisson", "fft_cols_inv", "fft_rows_c2r"),
+                         lambda N, G: 4 * G ** 3 + 8 * G * G * (G // 2 + 1) + 3 * (8 * G * G * (G // 2 + 1) + 4 * G ** 3)),
+    "gather_kdk": (("gather_kdk",), lambda N, G: 48 * N + 12 * G ** 3),
+}
+
+
+def cpu_baseline(pos, vel, G, dt, steps=1):
+    """Time the PyTorch-CPU FP32 port of one KDK step on the full workload."""
+    import torch
+    from oracle import torch_cpu as tc
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    n = pos.shape[0]
+    p = torch.from_numpy(pos)
+    v = torch.from_numpy(vel)
+    m = torch.full((n,), 1.0 / n)
+    a = tc.pm_acc(p, m, G)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p, v, a = tc.kdk_step(p, v, a, m, G, dt)
+    sec = (time.perf_counter() - t0) / steps
+    return {"value": n / sec, "unit": "particle-steps/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"{steps} KDK step(s) of the same N={n}, {G}^3 workload with the PyTorch-CPU FP32 "
+                      f"restatement (oracle/torch_cpu.py), {sec:.2f} s/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--targets", type=int, default=16384)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from augur_amd import _lib, ops, scenes
+    from augur_amd.pm import PeriodicPM
+
+    dev = torch.device("cuda:0")
+    N, G = a.n, a.grid
+    t0 = time.perf_counter()
+    pos_np, vel_np = scenes.cfg2(n=N)
+    print(f"[bench] scene N={N} G={G} built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    st = pm.load(torch.from_numpy(pos_np).to(dev), torch.from_numpy(vel_np).to(dev), mass_value=1.0 / N)
+    torch.cuda.synchronize()
+
+    # force error of the plain PM path vs the FP64 periodic (Ewald) oracle,
+    # on seeded targets of the initial scene
+    rel = None
+    if not a.no_oracle:
+        tg = torch.from_numpy(scenes.targets(N, a.targets)).to(dev)
+        pos0 = torch.from_numpy(pos_np).to(dev)
+        a_pm = pm.accel(pos0, mass_value=1.0 / N)[tg].double()
+        a_ex = ops.ewald_f64(pos0, 1.0 / N, tg, L=1.0, eps=1.0 / 256)
+        rel = float((((a_pm - a_ex) ** 2).sum(-1).mean() / (a_ex ** 2).sum(-1).mean()).sqrt())
+        del pos0, a_pm, a_ex
+        print(f"[bench] PM force rel-RMS vs FP64 Ewald on {a.targets} targets: {rel:.4e}", file=sys.stderr,
+              flush=True)
+
+    pm.kdk(st, a.dt, a.warmup) if a.warmup > 0 else None
+    torch.cuda.synchronize()
+    _lib.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pm.kdk(st, a.dt, a.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing = _lib.timing_collect()
+    _lib.timing_enable(False)
+    sec = t1 - t0
+    value = N * a.steps / sec
+    print(f"[bench] {a.steps} steps in {sec * 1e3:.2f} ms -> {value:.4e} particle-steps/s", file=sys.stderr,
+          flush=True)
+
+    # dominant kernel and per-stage rates
+    dom = max(timing.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcnt) = dom
+    avg_s = dms / dcnt / 1e3
+    ab = algorithmic_bytes(dname, N, G)
+    achieved = ab / avg_s / 1e9 if ab else None
+    roofline = {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "avg_launch_us": avg_s * 1e6, "algorithmic_bytes": ab}
+    stages = {}
+    for sname, (kern, fb) in STAGES.items():
+        ms = sum(timing.get(k, (0.0, 0))[0] for k in kern) / a.steps
+        b = fb(N, G)
+        stages[sname] = {"ms_per_step": ms, "algorithmic_bytes": b,
+                         "GB_s": b / (ms / 1e3) / 1e9 if ms > 0 else None,
+                         "frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS if ms > 0 else None}
+    kernels = {k: {"ms_total": v[0], "launches": v[1], "us_avg": v[0] / v[1] * 1e3} for k, v in timing.items()}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(pos_np, vel_np, G, a.dt)
+        print(f"[bench] CPU baseline {cpu['value']:.3e} p-steps/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": sec * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain periodic PM, N={N} (clumpy: 50% uniform + 64 Gaussian clumps sigma 0.02), "
+                               f"{G}^3 grid, KDK dt={a.dt}, Maxwellian sigma_v=0.01",
+                   "N": N, "grid": G, "dt": a.dt, "path": "plain_pm"},
+        "force_rel_rms_vs_fp64": rel, "oracle": "FP64 Ewald (periodic, eps=1/256)",
+        "surrogate": None,
+        "roofline": roofline, "stages": stages, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,218 @@
+/*
+ * augur.h -- C ABI of libaugur_hip.so, the MI355X (gfx950) kernels behind the
+ * particle-mesh (PM) gravity step and its scatter-field CNN surrogate.
+ *
+ * Conventions (every entry point):
+ *   - Plain pointers to device memory, sizes in elements, an explicit HIP
+ *     stream passed as void* (hipStream_t).  All work is stream-ordered; no
+ *     call synchronises the host, allocates device memory or frees it.
+ *   - Scratch memory is caller-provided: augur_*_workspace_bytes() gives the
+ *     size, the caller passes (ws, ws_bytes).  Calls are re-entrant and safe
+ *     under stream capture (no hipMemset nodes; buffers are cleared by
+ *     kernels).
+ *   - Return 0 (AUGUR_OK) or a negative status; augur_last_error() returns a
+ *     thread-local description of the last failure.
+ *   - Positions are FP32 (B, N, D) row-major, D in {2, 3}, component 0 = x.
+ *     Grids are (B, C, G^D) with x fastest: cell ((b*G + iz)*G + iy)*G + ix.
+ *   - "off" is added to positions before dividing by the cell size h:
+ *     u = (pos + off)/h - 0.5 (the reference's (pos + extent/2)/h - 0.5 with
+ *     off = extent/2; off = 0 for a periodic box [0, G*h)).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * Faraday-dot-py/Augur).
+ */
+#ifndef AUGUR_H
+#define AUGUR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AUGUR_OK 0
+#define AUGUR_EINVAL (-1)       /* bad argument or shape */
+#define AUGUR_EUNSUPPORTED (-2) /* valid but not supported (e.g. non power-of-two FFT) */
+#define AUGUR_EHIP (-3)         /* HIP runtime error */
+#define AUGUR_EWORKSPACE (-4)   /* workspace missing or too small */
+
+const char* augur_last_error(void);
+int augur_abi_version(void);
+
+/* ---------------------------------------------------------------- a1 keys */
+
+/* Pair-term cell keys: c = clamp(floor((p + off)/cell), 0, C-1) per axis,
+ * key = b*C^D + lin(c) (x fastest); masked (mask == 0) -> B*C^D.  mask may be
+ * NULL (all valid).  counts (B*C^D + 1 entries, may be NULL) receives the
+ * occupancy.  Replaces ScatterField._cell_keys, scripts/scatter_field.py:302-311. */
+int augur_cell_keys(const float* pos, const float* mask, int64_t B, int64_t N, int D, float off, float cell,
+                    int64_t C, int64_t* keys, int64_t* counts, void* stream);
+
+/* CIC base corner i0 = floor(u) and weight w = u - i0 per axis, FP32 with the
+ * reference's rounding order.  i0, w: (n, D).
+ * Replaces ScatterField._corners, scripts/scatter_field.py:179-182. */
+int augur_cic_base(const float* pos, int64_t n, int D, float off, float h, int32_t* i0, float* w, void* stream);
+
+/* -------------------------------------------------------------- a3 sort */
+
+/* Stable sort by cell: perm orders indices by (key, index), bit-exact with
+ * torch.argsort(key, stable=True); starts (nkeys + 1 entries, may be NULL)
+ * receives cumsum(counts) - counts with starts[nkeys] = n.  Keys in
+ * [0, nkeys), nkeys < 2^31 (out-of-range keys are clamped).  Replaces scripts/scatter_field.py:331-333. */
+size_t augur_sort_workspace_bytes(int64_t n, int64_t nkeys);
+int augur_sort_by_cell(const int64_t* keys, int64_t n, int64_t nkeys, int64_t* perm, int64_t* starts, void* ws,
+                       size_t ws_bytes, void* stream);
+
+/* ----------------------------------------------------------- a2 deposit */
+
+/* Deterministic CIC deposit.  Channels: [m] (vel == NULL) or
+ * [m, m*v_0, .., m*v_{D-1}], each times `scale` (the reference's
+ * in_scale/h^D).  mass: (B, N) or NULL with mass_value for all; mask (B, N)
+ * or NULL.  periodic != 0 wraps corners mod G, else drops corners outside
+ * [0, G).  Output (B, C, G^D).  Every contribution is accumulated as an exact
+ * fixed-point integer, so the grid is bitwise independent of particle order,
+ * thread count and launch shape.
+ * Replaces ScatterField.scatter (scripts/scatter_field.py:237-250) and
+ * cic_density (model/tiered/energy.py:9-34). */
+size_t augur_cic_deposit_workspace_bytes(int64_t B, int D, int64_t G, int C);
+int augur_cic_deposit(const float* pos, const float* vel, const float* mass, float mass_value, const float* mask,
+                      int64_t B, int64_t N, int D, int64_t G, float off, float h, int periodic, float scale, float* out,
+                      void* ws, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------ a6 gather */
+
+/* CIC gather of a (B, C, G^D) field at (B, N, D) positions -> (B, N, C).
+ * Replaces ScatterField.gather, scripts/scatter_field.py:252-261. */
+int augur_cic_gather(const float* field, const float* pos, int64_t B, int64_t N, int D, int64_t G, int C, float off,
+                     float h, int periodic, float* out, void* stream);
+
+/* ---------------------------------------------------------------- a4 FFT */
+
+/* Batched real-to-complex FFT over the last ndim (2 or 3) axes; shape =
+ * [ny, nx] or [nz, ny, nx], powers of two in [4, 512].  The input is
+ * (batch, in_shape) and is zero-extended to `shape` (in_shape may be NULL =
+ * shape), which gives the reference's rfft2(x, s=(2G, 2G)) padding.  Output
+ * (batch, .., nx/2 + 1) complex64 interleaved, unnormalised (torch "backward").
+ * Replaces torch.fft.rfft2 at scripts/scatter_field.py:273,284 and
+ * scripts/pm_baseline_force_error.py:74, and rfftn for the 3D solve. */
+size_t augur_fft_workspace_bytes(int ndim, const int64_t* shape, int64_t batch);
+int augur_fft_r2c(const float* in, const int64_t* in_shape, float* out, int ndim, const int64_t* shape,
+                  int64_t batch, void* ws, size_t ws_bytes, void* stream);
+
+/* Inverse of augur_fft_r2c times `scale` (pass 1/prod(shape) for torch's
+ * irfftn).  The real output is cropped to out_shape (NULL = shape), which
+ * gives the reference's irfft2(...)[..., :G, :G].  The complex input is
+ * consumed (used as scratch).
+ * Replaces torch.fft.irfft2 at scripts/scatter_field.py:285 and
+ * scripts/pm_baseline_force_error.py:74. */
+int augur_fft_c2r(float* in, float* out, const int64_t* out_shape, int ndim, const int64_t* shape, int64_t batch,
+                  float scale, void* ws, size_t ws_bytes, void* stream);
+
+/* Pointwise complex product out = a * b (b broadcast over `batch`), n complex
+ * elements per batch entry, times `scale`.  The spectral product of
+ * scripts/pm_baseline_force_error.py:74 and scripts/scatter_field.py:285. */
+int augur_complex_mul(const float* a, const float* b, float* out, int64_t n, int64_t batch, float scale, void* stream);
+
+/* ------------------------------------------- a4 + a5 Poisson and gradient */
+
+/* Periodic Poisson solve with spectral gradient on a (B, G, G, G/2+1) rfftn
+ * spectrum of rho: phi_hat = -4 pi Gconst rho_hat / |k|^2 (phi_hat(0) = 0),
+ * acc_hat_d = -i k_d phi_hat with the Nyquist plane of axis d zeroed,
+ * k = 2 pi n / L.  acc_hat: (B, 3, G, G, G/2+1) [a_x, a_y, a_z]; phi_hat may be
+ * NULL.  Outputs are multiplied by `scale` (fold the inverse-FFT 1/G^3 here).
+ * New for the 3D periodic solve; replaces the kernel convolution + neg_grad of
+ * scripts/pm_baseline_force_error.py:71-75. */
+int augur_poisson_green_grad(const float* rho_hat, int64_t B, int64_t G, float L, float Gconst, float scale,
+                             float* acc_hat, float* phi_hat, void* stream);
+
+/* Softened open-boundary kernel K = -1/sqrt(dx^2 + dy^2 + eps^2) at
+ * min-image distances d = min(i, 2G - i)*h on the (2G)^2 domain.
+ * Replaces scripts/pm_baseline_force_error.py:71-73. */
+int augur_open_kernel_2d(int64_t G, float h, float eps, float* out, void* stream);
+
+/* Zero-padded central-difference -grad of (B, G, G) -> (B, 2, G, G)
+ * [-dphi/dx, -dphi/dy].  Replaces ScatterField.neg_grad,
+ * scripts/scatter_field.py:551-555. */
+int augur_neg_grad_fd(const float* phi, int64_t B, int64_t G, float h, float* out, void* stream);
+
+/* ------------------------------------------------------- a9 / a10 oracle */
+
+/* FP64 softened direct sum, G = 1: for each target t,
+ * a_t = sum_{j != idx_t} m_j (x_j - x_t) (|x_j - x_t|^2 + eps^2)^-1.5.
+ * Sources: FP64 (pos64) or FP32 (pos32, promoted exactly); exactly one is
+ * non-NULL, shape (n, D).  mass (n) or NULL (all 1).  targets: (t) indices
+ * into the sources, or NULL for all.  out: (t, D) FP64.  Sources are split
+ * into slices whose partial sums are added in a fixed order (run-to-run
+ * identical).  Replaces exact_acc, scripts/scatter_scaling_ops.py:56-71. */
+size_t augur_direct_sum_workspace_bytes(int64_t n, int64_t t, int D);
+int augur_direct_sum_f64(const double* pos64, const float* pos32, const double* mass, int64_t n, int D,
+                         const int64_t* targets, int64_t t, double eps, double* out, void* ws, size_t ws_bytes,
+                         void* stream);
+
+/* FP64 softened potential sums phi_t = sum_{j != idx_t} m_j (|x_j - x_t|^2 +
+ * eps^2)^-1/2, out (t) FP64, same arguments and workspace as above.  The
+ * pair-potential part of exact_energy (scripts/scatter_scaling_ops.py:90-103)
+ * and gravity_sim.energy (scripts/gravity_sim.py:14-18) is -0.5 sum m_t phi_t. */
+int augur_direct_potential_f64(const double* pos64, const float* pos32, const double* mass, int64_t n, int D,
+                               const int64_t* targets, int64_t t, double eps, double* out, void* ws, size_t ws_bytes,
+                               void* stream);
+
+/* FP64 Ewald sum for the periodic cube [0, L)^3 with the mean density removed
+ * (the problem the periodic PM solve approximates), Plummer softening eps on
+ * the nearest image only.  Real space: minimum image only, erfc split with
+ * alpha = alpha_l / L (the dropped images are below erfc(alpha_l / 2));
+ * reciprocal: |n_k|_inf <= kmax.  Sources FP32 (n, 3), mass (n) FP64 or NULL
+ * (= mass_value).  out: (t, 3) FP64.  New (the reference is isolated). */
+size_t augur_ewald_workspace_bytes(int64_t n, int64_t t, int kmax);
+int augur_ewald_f64(const float* pos32, const double* mass, double mass_value, int64_t n, const int64_t* targets,
+                    int64_t t, double L, double eps, double alpha_l, int kmax, double* out, void* ws, size_t ws_bytes,
+                    void* stream);
+
+/* Deterministic FP64 reductions over (n, 3) FP32 velocities and optional
+ * (n) FP32 potentials: out[0..2] = momentum sum m v, out[3] = kinetic
+ * energy 0.5 sum m v^2, out[4] = 0.5 sum m phi (0 if phi == NULL),
+ * out[5] = sum m.  Replaces the FP64 momentum / energy sums of
+ * scripts/scatter_bh_1b.py:150-186 and scatter_field.momentum (:768-769). */
+size_t augur_reduce_workspace_bytes(int64_t n);
+int augur_reduce_energy_momentum_f64(const float* vel, const float* mass, float mass_value, const float* phi,
+                                     int64_t n, double* out, void* ws, size_t ws_bytes, void* stream);
+
+/* ----------------------------------------------- periodic PM stepper (a7) */
+
+/* The plain PM step on a periodic cube [0, L)^3, G^3 cells (G a power of two
+ * in [8, 512]), particles held sorted by CIC base cell in caller-owned state:
+ *   pos, vel, acc: (n, 3) FP32; ids: (n) int32 original index; mass (n) or
+ *   NULL (= mass_value).
+ * augur_pm_prepare(): stable sort of the state by cell (physical reorder of
+ *   pos, vel, acc, mass, ids) and the fixed-point scale of the deposit.
+ * augur_pm_accel(): acc = PM force at pos (deposit -> rfftn -> Green's
+ *   function -> spectral gradient -> irfftn -> CIC gather).
+ * augur_pm_kdk(): nsteps of KDK leapfrog with the fused gather + kick + drift:
+ *   v += dt/2 a; x += dt v (wrapped); a = PM(x); v += dt/2 a.
+ * Together these replace ScatterField._step (verlet; scripts/scatter_field.py:
+ * 677-688), pm_acc (scripts/pm_baseline_force_error.py:68-76) and
+ * gravity_sim.rollout's KDK (scripts/gravity_sim.py:31-44) for 3D periodic PM. */
+size_t augur_pm_workspace_bytes(int64_t n, int64_t G);
+int augur_pm_prepare(float* pos, float* vel, float* acc, float* mass, int32_t* ids, int64_t n, int64_t G, float L,
+                     void* ws, size_t ws_bytes, void* stream);
+int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_value, int64_t n, int64_t G, float L,
+                   float Gconst, void* ws, size_t ws_bytes, void* stream);
+int augur_pm_kdk(float* pos, float* vel, float* acc, const float* mass, float mass_value, int64_t n, int64_t G,
+                 float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------ measurement */
+
+/* Per-launch timing: when enabled, each kernel launch of the library records
+ * a HIP event pair on its own stream; augur_timing_collect() waits for them
+ * and returns, per kernel name, the summed milliseconds and launch count
+ * (names '\n'-separated in `names`).  Disabled by default; no event work is
+ * done while disabled.  Replaces the CUDA-event component timers of
+ * scripts/scatter_field_2d_cost_blowup_bench.py:64-80. */
+int augur_timing_enable(int on);
+int augur_timing_collect(char* names, size_t names_cap, double* ms, int64_t* counts, int max_entries, int* n_entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AUGUR_H */

@@ -1,0 +1,54 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the header
+declares, validates arguments on the host, and the Python layer refuses CPU
+tensors instead of falling back."""
+import ctypes
+
+import pytest
+import torch
+
+from augur_amd import _lib, ops
+
+
+def test_exports_every_header_symbol(lib):
+    names = _lib.header_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+    assert lib.augur_abi_version() == 1
+
+
+def test_workspace_queries_are_host_only(lib):
+    assert _lib.size("augur_sort_workspace_bytes", 1 << 20, 1 << 18) >= 8 << 20
+    assert _lib.size("augur_cic_deposit_workspace_bytes", 2, 3, 64, 4) >= 2 * 4 * 64 ** 3 * 8
+    assert _lib.size("augur_pm_workspace_bytes", 1 << 16, 64) >= 64 ** 3 * 8
+    assert _lib.size("augur_direct_sum_workspace_bytes", 4096, 512, 3) > 0
+    assert _lib.size("augur_ewald_workspace_bytes", 4096, 64, 8) > 0
+
+
+@pytest.mark.parametrize("name,args,needle", [
+    ("augur_cic_deposit", (None, None, None, 1.0, None, 1, 10, 4, 8, 0.0, 1.0, 1, 1.0, None, None, 0, None), "D must"),
+    ("augur_cic_deposit", (None, None, None, 1.0, None, 1, 10, 3, 12, 0.0, 1.0, 1, 1.0, None, None, 0, None),
+     "power-of-two"),
+    ("augur_cell_keys", (None, None, 1, 4, 5, 0.0, 1.0, 4, None, None, None), "D must"),
+    ("augur_pm_kdk", (None, None, None, None, 1.0, 100, 48, 1.0, 1.0, 0.1, 1, None, 0, None), "power of two"),
+    ("augur_poisson_green_grad", (None, 1, 1024, 1.0, 1.0, 1.0, None, None, None), "power-of-two"),
+    ("augur_direct_sum_f64", (None, None, None, 10, 3, None, 4, 0.1, None, None, 0, None), "exactly one"),
+])
+def test_host_validation_errors(lib, name, args, needle):
+    with pytest.raises(_lib.AugurError, match=needle):
+        _lib.call(name, *args)
+
+
+def test_fft_rejects_non_power_of_two(lib):
+    shape = (ctypes.c_int64 * 2)(12, 16)
+    with pytest.raises(_lib.AugurError, match="powers of two"):
+        _lib.call("augur_fft_r2c", None, None, None, 2, shape, 1, None, 0, None)
+
+
+def test_ops_refuse_cpu_tensors():
+    pos = torch.rand(1, 8, 3)
+    with pytest.raises(RuntimeError, match="ROCm GPU only"):
+        ops.cic_deposit(pos, None, 8, 0.125, 0.0, True)
+    with pytest.raises(RuntimeError, match="ROCm GPU only"):
+        ops.direct_sum_f64(pos[0], 0.1)

@@ -1,0 +1,325 @@
+"""GPU parity of the HIP kernels (through the C ABI) against the CPU oracle on
+the same seeded inputs.  Integer/index work is compared bit-exactly; FP32
+arithmetic within the stated tolerances."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import direct as odirect
+from oracle import pm as opm
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    return torch.device("cuda:0")
+
+
+def T(x, dev, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    return t if dtype is None else t.to(dtype)
+
+
+def adversarial_positions(rng, B, N, D, extent, h):
+    """Mix of random positions, exact cell walls/centres (FP32) and values one
+    ulp either side of a wall, some outside the grid."""
+    p = rng.uniform(-extent / 2 * 1.1, extent / 2 * 1.1, (B, N, D)).astype(F32)
+    k = rng.integers(-2, int(extent / h) + 2, (B, N, D))
+    wall = (-extent / 2 + k * h).astype(F32)
+    sel = rng.integers(0, 5, (B, N, D))
+    p = np.where(sel == 1, wall, p)
+    p = np.where(sel == 2, np.nextafter(wall, F32(np.inf)), p)
+    p = np.where(sel == 3, np.nextafter(wall, F32(-np.inf)), p)
+    p = np.where(sel == 4, (wall + F32(h / 2)).astype(F32), p)
+    return p.astype(F32)
+
+
+# ----------------------------------------------------------------- a1 / a3
+@pytest.mark.parametrize("D,extent,C", [(2, 64.0, 32), (3, 1.0, 16), (2, 10.0, 7), (3, 3.0, 10)])
+def test_cell_keys_bit_exact(dev, D, extent, C):
+    from augur_amd import ops
+    rng = np.random.default_rng(10 + D)
+    B, N = 3, 5000
+    cell = extent / C
+    pos = adversarial_positions(rng, B, N, D, extent, cell)
+    mask = (rng.uniform(size=(B, N)) > 0.1).astype(F32)
+    keys, counts = ops.cell_keys(T(pos, dev), T(mask, dev), extent / 2, cell, C, counts=True)
+    rk, rc = opm.cell_keys(pos, mask, extent, cell, C)
+    assert np.array_equal(keys.cpu().numpy(), rk)
+    assert np.array_equal(counts.cpu().numpy(), rc)
+
+
+@pytest.mark.parametrize("D,extent,G", [(2, 64.0, 128), (3, 1.0, 64), (2, 7.0, 20)])
+def test_cic_base_bit_exact(dev, D, extent, G):
+    from augur_amd import ops
+    rng = np.random.default_rng(20 + D)
+    h = extent / G
+    pos = adversarial_positions(rng, 2, 4000, D, extent, h)
+    i0, w = ops.cic_base(T(pos, dev), extent / 2, h)
+    ri0, rw = opm.cic_base(pos, extent / 2, h)
+    assert np.array_equal(i0.cpu().numpy().astype(np.int64), ri0)
+    assert np.array_equal(w.cpu().numpy().view(np.uint32), rw.view(np.uint32))
+
+
+@pytest.mark.parametrize("n,nkeys", [(0, 5), (1, 1), (1000, 3), (4096, 4096), (4097, 100), (300000, 70000),
+                                     (200000, (1 << 24) + 1)])
+def test_sort_by_cell_bit_exact(dev, n, nkeys):
+    from augur_amd import ops
+    rng = np.random.default_rng(n + nkeys)
+    keys = rng.integers(0, nkeys, n).astype(np.int64)
+    if n > 10:
+        keys[: n // 3] = keys[0]        # long run of ties
+    perm, starts = ops.sort_by_cell(T(keys, dev), nkeys, starts=True)
+    assert np.array_equal(perm.cpu().numpy(), opm.stable_sort_by_cell(keys))
+    counts = np.bincount(keys, minlength=nkeys)
+    exp = np.concatenate([[0], np.cumsum(counts)])
+    assert np.array_equal(starts.cpu().numpy(), exp)
+
+
+def test_sort_matches_reference_cell_key_pipeline(dev):
+    # _cell_keys -> argsort(stable) -> counts/starts (scatter_field.py:331-333)
+    from augur_amd import ops
+    rng = np.random.default_rng(5)
+    B, N, C, extent = 2, 20000, 32, 64.0
+    pos = rng.normal(0, 8, (B, N, 2)).astype(F32)
+    mask = (rng.uniform(size=(B, N)) > 0.05).astype(F32)
+    keys, counts = ops.cell_keys(T(pos, dev), T(mask, dev), extent / 2, extent / C, C, counts=True)
+    perm, starts = ops.sort_by_cell(keys, B * C * C + 1, starts=True)
+    rk, rc = opm.cell_keys(pos, mask, extent, extent / C, C)
+    assert np.array_equal(perm.cpu().numpy(), np.argsort(rk, kind="stable"))
+    assert np.array_equal(starts.cpu().numpy()[:-1], np.cumsum(rc) - rc)
+
+
+# --------------------------------------------------------------- a2 deposit
+@pytest.mark.parametrize("D,periodic,G,vel", [(2, False, 64, True), (3, True, 32, False), (3, True, 16, True),
+                                              (2, True, 32, False), (3, False, 16, False)])
+def test_deposit_matches_oracle(dev, D, periodic, G, vel):
+    from augur_amd import ops
+    rng = np.random.default_rng(30 + G)
+    B, N = 2, 20000
+    extent = 1.0 if periodic else 16.0
+    h = extent / G
+    off = 0.0 if periodic else extent / 2
+    lo, hi = (0.0, extent) if periodic else (-extent / 2 * 1.05, extent / 2 * 1.05)
+    pos = rng.uniform(lo, hi, (B, N, D)).astype(F32)
+    pos[:, : N // 4] = (pos[:, : N // 4] * F32(0.05) + F32(lo + (hi - lo) * 0.4)).astype(F32)  # dense clump
+    mass = rng.uniform(0.5, 2.0, (B, N)).astype(F32)
+    mask = (rng.uniform(size=(B, N)) > 0.1).astype(F32)
+    v = rng.normal(size=(B, N, D)).astype(F32) if vel else None
+    out = ops.cic_deposit(T(pos, dev), T(mass, dev), G, h, off, periodic, vel=None if v is None else T(v, dev),
+                          mask=T(mask, dev), scale=1.0 / h ** D).cpu().numpy()
+    ref = opm.cic_deposit(pos, mass, mask, G, h, off, periodic, vel=v)
+    for c in range(out.shape[1]):
+        err = np.abs(out[:, c] - ref[:, c]).max() / np.abs(ref[:, c]).max()
+        assert err < 2e-6, (c, err)
+    # fixed-point accumulation: bitwise independent of particle order
+    p = rng.permutation(N)
+    out2 = ops.cic_deposit(T(pos[:, p], dev), T(mass[:, p], dev), G, h, off, periodic,
+                           vel=None if v is None else T(v[:, p], dev), mask=T(mask[:, p], dev),
+                           scale=1.0 / h ** D).cpu().numpy()
+    assert np.array_equal(out.view(np.uint32), out2.view(np.uint32))
+
+
+def test_deposit_mass_conservation_periodic(dev):
+    from augur_amd import ops
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 1, (1, 100000, 3)).astype(F32)
+    out = ops.cic_deposit(T(pos, dev), None, 32, 1 / 32, 0.0, True, mass_value=1e-5)
+    assert abs(float(out.double().sum()) / 32 ** 3 - 1.0) < 1e-6
+
+
+# ----------------------------------------------------------------- a6 gather
+@pytest.mark.parametrize("D,periodic,G,C", [(2, False, 64, 3), (3, True, 32, 3), (3, False, 16, 1), (2, True, 16, 2)])
+def test_gather_matches_oracle(dev, D, periodic, G, C):
+    from augur_amd import ops
+    rng = np.random.default_rng(40 + G)
+    B, N = 2, 10000
+    extent = 1.0 if periodic else 16.0
+    h = extent / G
+    off = 0.0 if periodic else extent / 2
+    lo, hi = (0.0, extent) if periodic else (-extent / 2 * 1.05, extent / 2 * 1.05)
+    pos = rng.uniform(lo, hi, (B, N, D)).astype(F32)
+    field = rng.normal(size=(B, C) + (G,) * D).astype(F32)
+    out = ops.cic_gather(T(field, dev), T(pos, dev), h, off, periodic).cpu().numpy()
+    ref = opm.cic_gather(field, pos, h, off, periodic)
+    assert np.abs(out - ref).max() < 1e-5
+
+
+# -------------------------------------------------------------------- a4 FFT
+@pytest.mark.parametrize("s,ins,B", [((8, 16), None, 3), ((64, 64), (40, 33), 2), ((256, 256), (128, 128), 1),
+                                     ((8, 8, 8), None, 2), ((16, 32, 64), (16, 20, 64), 1), ((64, 64, 64), None, 1)])
+def test_rfftn_matches_numpy(dev, s, ins, B):
+    from augur_amd import ops
+    rng = np.random.default_rng(sum(s))
+    ins = ins or s
+    x = rng.normal(size=(B,) + tuple(ins)).astype(F32)
+    X = ops.rfftn(T(x, dev), s=s).cpu().numpy()
+    ref = np.fft.rfftn(x.astype(np.float64), s=s, axes=tuple(range(1, len(s) + 1)))
+    assert np.abs(X - ref).max() < 3e-6 * np.abs(ref).max()
+    y = ops.irfftn(T(X, dev), s=s, out_shape=ins).cpu().numpy()
+    assert np.abs(y - x).max() < 3e-6 * np.abs(x).max()
+
+
+def test_irfftn_matches_numpy_on_hermitian_input(dev):
+    from augur_amd import ops
+    rng = np.random.default_rng(7)
+    s = (32, 16, 64)
+    X = np.fft.rfftn(rng.normal(size=s))
+    y = ops.irfftn(T(X.astype(np.complex64), dev), s=s).cpu().numpy()
+    assert np.abs(y - np.fft.irfftn(X, s=s)).max() < 3e-6 * np.abs(np.fft.irfftn(X, s=s)).max()
+
+
+# --------------------------------------------------------- a4/a5 3D Poisson
+@pytest.mark.parametrize("G,L", [(16, 1.0), (32, 2.0)])
+def test_poisson_green_grad_matches_oracle(dev, G, L):
+    from augur_amd import ops
+    rng = np.random.default_rng(G)
+    rho = rng.uniform(0, 3, (G, G, G)).astype(F32)
+    spec = ops.rfftn(T(rho[None], dev), s=(G, G, G))
+    acc = ops.poisson_green_grad(spec, G, L, 1.3, scale=1.0 / G ** 3)
+    a = ops.irfftn(acc, s=(G, G, G), scale=1.0).cpu().numpy()[0]
+    ref, _ = opm.poisson_periodic_accel_grids(rho.astype(np.float64), L, 1.3)
+    assert np.abs(a - ref).max() < 2e-5 * np.abs(ref).max()
+
+
+def test_poisson_single_mode_closed_form(dev):
+    from augur_amd import ops
+    G, L = 32, 1.0
+    z, y, x = np.meshgrid(*[np.arange(G) / G] * 3, indexing="ij")
+    k = 2 * np.pi * np.array([2, 1, 3])
+    ph = k[0] * x + k[1] * y + k[2] * z
+    rho = (1.0 + 0.5 * np.cos(ph)).astype(F32)
+    spec = ops.rfftn(T(rho[None], dev), s=(G, G, G))
+    a = ops.irfftn(ops.poisson_green_grad(spec, G, L, 1.0, scale=1.0 / G ** 3), s=(G, G, G), scale=1.0)
+    a = a.cpu().numpy()[0]
+    for d in range(3):
+        exact = -4 * np.pi * 0.5 * k[d] * np.sin(ph) / (k ** 2).sum()
+        assert np.abs(a[d] - exact).max() < 1e-5 * np.abs(exact).max()
+
+
+# ---------------------------------------------------- whole 2D reference pm_acc
+def plummer2d(n, rng):
+    u = rng.uniform(0.0, 0.97, 4 * n)
+    R = 4.0 * np.sqrt(u / (1 - u))
+    th = rng.uniform(0, 2 * np.pi, 4 * n)
+    p = np.stack([R * np.cos(th), R * np.sin(th)], 1)
+    return p[(np.abs(p) < 28).all(1)][:n].astype(F32)
+
+
+def test_pm_acc_2d_reference_mode_parity(dev):
+    # the reference configuration of scripts/pm_baseline_force_error.py:111-112:
+    # G=128, extent 64 (h=0.5), EPS=0.5; FP32 vs the FP64 oracle <= 1e-5 rel-RMS
+    from augur_amd import scatter_field as sf
+    from augur_amd.pm_baseline_force_error import pm_acc
+    rng = np.random.default_rng(4738)
+    B, n = 2, 1000
+    pos = np.stack([plummer2d(n, rng) for _ in range(B)])
+    m = sf.ScatterField(grid=128, extent=64.0, nonet=True, potential=True, verlet=True, pm=True, momfix=False)
+    mass = np.ones((B, n), F32)
+    mask = np.ones((B, n), F32)
+    mask[1, :50] = 0
+    a = pm_acc(m, T(pos, dev), T(np.zeros_like(pos), dev), T(mass, dev), T(mask, dev)).cpu().numpy()
+    ref = opm.pm_acc_2d_open(pos, mass, mask, 128, 64.0)
+    rel = np.sqrt(((a - ref) ** 2).sum() / (ref ** 2).sum())
+    assert rel < 1e-5, rel
+    assert np.abs(a[1, :50]).max() == 0.0
+
+
+# ------------------------------------------------------- 3D periodic PM path
+def test_pm_accel_3d_matches_oracle(dev):
+    from augur_amd.pm import PeriodicPM
+    rng = np.random.default_rng(11)
+    n, G = 8192, 32
+    pos = np.mod(np.concatenate([rng.uniform(0, 1, (n // 2, 3)), 0.5 + 0.05 * rng.normal(size=(n // 2, 3))]), 1.0)
+    pos = pos.astype(F32)
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    a = pm.accel(T(pos, dev)).cpu().numpy()
+    ref = opm.pm_acc_3d_periodic(pos, 1.0 / n, G)
+    rel = np.sqrt(((a - ref) ** 2).sum() / (ref ** 2).sum())
+    assert rel < 1e-5, rel
+    # sorted state gives the same forces (in the original order)
+    st = pm.load(T(pos, dev), mass_value=1.0 / n)
+    _, _, acc = pm.unload(st)
+    assert np.abs(acc.cpu().numpy() - a).max() <= 1e-6 * np.abs(a).max()
+    # momentum: sum m a = 0 for CIC with an odd spectral kernel
+    assert np.abs(a.sum(0)).max() < 1e-5 * np.abs(a).sum(0).max()
+
+
+def test_pm_kdk_matches_oracle(dev):
+    from augur_amd.pm import PeriodicPM
+    rng = np.random.default_rng(12)
+    n, G, dt, steps = 4096, 32, 2e-3, 10
+    pos = np.mod(0.5 + 0.08 * rng.normal(size=(n, 3)), 1.0).astype(F32)
+    vel = (0.05 * rng.normal(size=(n, 3))).astype(F32)
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    p, v, a = (x.cpu().numpy() for x in pm.rollout(T(pos, dev), T(vel, dev), dt=dt, steps=steps))
+    rp, rv = opm.kdk_pm_rollout(pos, vel, 1.0 / n, G, dt, steps)
+    dp = np.abs(((p - rp) + 0.5) % 1.0 - 0.5).max()
+    assert dp < 2e-5, dp
+    assert np.abs(v - rv).max() < 2e-4 * np.abs(rv).max()
+
+
+# --------------------------------------------------------------- a9 oracle
+def test_direct_sum_known_answer_and_oracle(dev):
+    from augur_amd import ops
+    a = ops.direct_sum_f64(torch.tensor([[0.0, 0.0], [3.0, 0.0]], dtype=torch.float64, device=dev), 0.5).cpu()
+    assert abs(float(a[0, 0]) - 0.10663718) < 1e-7 and abs(float(a[1, 0]) + 0.10663718) < 1e-7
+    rng = np.random.default_rng(13)
+    for D in (2, 3):
+        pos = rng.normal(size=(3000, D)).astype(F32)
+        m = rng.uniform(0.5, 1.5, 3000)
+        tg = np.sort(rng.choice(3000, 300, replace=False))
+        g = ops.direct_sum_f64(T(pos, dev), 0.05, T(m, dev), T(tg, dev)).cpu().numpy()
+        ref = odirect.direct_acc(pos.astype(np.float64), 0.05, m, tg)
+        assert np.abs(g - ref).max() < 1e-11 * np.abs(ref).max()
+        phi = ops.direct_potential_f64(T(pos.astype(np.float64), dev), 0.05, T(m, dev)).cpu().numpy()
+        d = pos.astype(np.float64)[None] - pos.astype(np.float64)[:, None]
+        inv = 1 / np.sqrt((d ** 2).sum(-1) + 0.05 ** 2)
+        np.fill_diagonal(inv, 0)
+        assert np.abs(phi - inv @ m).max() < 1e-11 * np.abs(phi).max()
+
+
+def test_direct_sum_matches_reference_rollout_fixture(dev):
+    # KDK with the GPU FP64 direct sum reproduces web/scatter/tests/truth_ref.json
+    import json
+    import os
+    from augur_amd.scatter_scaling_ops import exact_rollout
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    init = json.load(open(os.path.join(g, "truth_init.json")))
+    ref = json.load(open(os.path.join(g, "truth_ref.json")))
+    n = init["n"]
+    pos = torch.tensor(init["pos"], dtype=torch.float64, device=dev).reshape(n, 2)
+    vel = torch.tensor(init["vel"], dtype=torch.float64, device=dev).reshape(n, 2)
+    ps, _ = exact_rollout(pos, vel, 100, init["dt"], 4, 0.5)
+    for t in (10, 100):
+        assert np.abs(ps[t].cpu().numpy().reshape(-1) - np.array(ref["pos"][str(t)])).max() < 1e-6
+
+
+def test_ewald_matches_oracle(dev):
+    from augur_amd import ops
+    rng = np.random.default_rng(14)
+    pos = np.mod(rng.uniform(0, 1, (600, 3)), 1).astype(F32)
+    m = rng.uniform(0.5, 1.5, 600)
+    tg = np.arange(0, 600, 7)
+    g = ops.ewald_f64(T(pos, dev), T(m, dev), T(tg, dev), eps=0.01).cpu().numpy()
+    ref = odirect.ewald_acc(pos.astype(np.float64), m, tg, eps=0.01)
+    assert np.abs(g - ref).max() < 1e-7 * np.abs(ref).max()
+
+
+def test_energy_momentum_reduction(dev):
+    from augur_amd import ops
+    rng = np.random.default_rng(15)
+    v = rng.normal(size=(100001, 3)).astype(F32)
+    m = rng.uniform(0.1, 1, 100001).astype(F32)
+    phi = rng.normal(size=100001).astype(F32)
+    out = ops.energy_momentum_f64(T(v, dev), T(m, dev), phi=T(phi, dev)).cpu().numpy()
+    vd, md = v.astype(np.float64), m.astype(np.float64)
+    np.testing.assert_allclose(out[:3], (md[:, None] * vd).sum(0), rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(out[3], 0.5 * (md * (vd ** 2).sum(1)).sum(), rtol=1e-12)
+    np.testing.assert_allclose(out[4], 0.5 * (md * phi).sum(), rtol=1e-10)
+    np.testing.assert_allclose(out[5], md.sum(), rtol=1e-12)
