@@ -48,6 +48,13 @@ SIGNATURES = {
     "augur_pm_accel": (c_int, [P, P, P, c_float, c_int64, c_int64, c_float, c_float, P, c_size_t, P]),
     "augur_pm_kdk": (c_int, [P, P, P, P, c_float, c_int64, c_int64, c_float, c_float, c_float, c_int, P, c_size_t,
                              P]),
+    "augur_conv3_bf16_fwd": (c_int, [P, c_int, P, P, c_int64, c_int64, c_int, c_int, P, P, P, P, P]),
+    "augur_conv3_wgrad_workspace_bytes": (c_size_t, [c_int, c_int64, c_int64, c_int]),
+    "augur_conv3_bf16_wgrad": (c_int, [P, c_int, P, c_int64, c_int64, P, P, c_int, P, c_size_t, P]),
+    "augur_conv3_flip_weights": (c_int, [P, c_int, c_int, c_int, P, P]),
+    "augur_gelu_bwd": (c_int, [P, P, c_int64, P, P]),
+    "augur_pack_cl_bf16": (c_int, [P, c_int, c_int64, c_int64, c_int, c_float, P, P]),
+    "augur_adam_update": (c_int, [P, P, P, P, c_int64, c_float, c_float, c_float, c_float, c_int, c_float, P, P]),
     "augur_timing_enable": (c_int, [c_int]),
     "augur_timing_collect": (c_int, [P, c_size_t, P, P, c_int, P]),
 }

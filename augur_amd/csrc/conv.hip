@@ -1,0 +1,456 @@
+// BF16 MFMA 3x3x3 periodic convolution (forward, data-gradient, weight-
+// gradient) for the scatter-field surrogate on gfx950, FP32 accumulation.
+//
+// Activations are channels-last BF16: x[b][z][y][x][c].  Weights are stored
+// tap-major [27][32][CIN] (tap = (dz*3 + dy)*3 + dx, offsets d - 1).
+// One workgroup (4 waves) computes a 32 x 4 x 2 output tile for all 32 output
+// channels: the tile's input with a one-voxel periodic halo and the whole
+// weight tensor are staged in LDS, and each wave runs
+// v_mfma_f32_32x32x16_bf16 over 27 taps x CIN/16 K-steps for two 32-voxel
+// rows, sharing each weight fragment between its two rows.
+#include "internal.hpp"
+
+using namespace augur;
+
+namespace augur {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+
+constexpr int CT_X = 32, CT_Y = 4, CT_Z = 2;
+constexpr int CONV_THREADS = 256;
+constexpr int HX = CT_X + 2, HY = CT_Y + 2, HZ = CT_Z + 2;
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// round-to-nearest-even FP32 -> BF16 (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+}
+
+enum ConvMode : int {
+  CONV_OUT_F32_CF = 0,   // y[b][co][vox] = conv + bias, co < cout (FP32, channels-first)
+  CONV_GELU = 1,         // z = conv + bias -> z_out; h_out = gelu(z)            (BF16 channels-last)
+  CONV_RES_GELU = 2,     // z -> z_out; h_out = x_center + gelu(z)               (BF16, CIN = 32)
+  CONV_OUT_F32_CL = 3,   // y[vox][co] = conv (+ addend[vox][co])                (FP32 channels-last, no bias)
+};
+
+template <int CIN, int MODE>
+__global__ __launch_bounds__(CONV_THREADS) void k_conv3_fwd(const uint16_t* __restrict__ x,
+                                                            const uint16_t* __restrict__ w,
+                                                            const float* __restrict__ bias, int G, int cout,
+                                                            uint16_t* __restrict__ z_out, uint16_t* __restrict__ h_out,
+                                                            float* __restrict__ y_out,
+                                                            const float* __restrict__ addend) {
+  constexpr int CP = CIN + 8;  // padded channel stride (conflict-free 16-byte LDS reads)
+  __shared__ __attribute__((aligned(16))) uint16_t wl[27 * 32 * CP];
+  __shared__ __attribute__((aligned(16))) uint16_t xl[HZ * HY * HX * CP];
+  const int tid = threadIdx.x;
+  const int ntx = G / CT_X, nty = G / CT_Y, ntz = G / CT_Z;
+  int t = blockIdx.x;
+  const int bx = t % ntx;
+  t /= ntx;
+  const int by = t % nty;
+  t /= nty;
+  const int bz = t % ntz;
+  const int b = t / ntz;
+  const int x0 = bx * CT_X, y0 = by * CT_Y, z0 = bz * CT_Z;
+  const int64_t G3 = (int64_t)G * G * G;
+  const uint16_t* xb = x + (int64_t)b * G3 * CIN;
+
+  // weights -> LDS (16-byte chunks)
+  constexpr int WCH = 27 * 32 * (CIN / 8);
+  for (int c = tid; c < WCH; c += CONV_THREADS) {
+    const int row = c / (CIN / 8), ch = c % (CIN / 8);
+    *reinterpret_cast<uint4*>(&wl[row * CP + ch * 8]) = *reinterpret_cast<const uint4*>(&w[(int64_t)row * CIN + ch * 8]);
+  }
+  // input tile with periodic halo -> LDS
+  constexpr int XCH = HZ * HY * HX * (CIN / 8);
+  for (int c = tid; c < XCH; c += CONV_THREADS) {
+    const int ch = c % (CIN / 8);
+    int v = c / (CIN / 8);
+    const int tx = v % HX;
+    v /= HX;
+    const int ty = v % HY;
+    const int tz = v / HY;
+    const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1), gz = (z0 + tz - 1) & (G - 1);
+    const int64_t gv = ((int64_t)gz * G + gy) * G + gx;
+    *reinterpret_cast<uint4*>(&xl[((tz * HY + ty) * HX + tx) * CP + ch * 8]) =
+        *reinterpret_cast<const uint4*>(&xb[gv * CIN + ch * 8]);
+  }
+  __syncthreads();
+
+  const int lane = tid & 63, wv = tid >> 6;
+  const int i = lane & 31, kh = lane >> 5;
+  int rz[2], ry[2];
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int r = wv * 2 + rr;
+    rz[rr] = r / CT_Y;
+    ry[rr] = r % CT_Y;
+  }
+  f32x16 acc[2];
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[rr][q] = 0.0f;
+
+#pragma unroll 1
+  for (int tap = 0; tap < 27; ++tap) {
+    const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+#pragma unroll
+    for (int hh = 0; hh < CIN / 16; ++hh) {
+      const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&wl[(tap * 32 + i) * CP + hh * 16 + kh * 8]);
+#pragma unroll
+      for (int rr = 0; rr < 2; ++rr) {
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(
+            &xl[(((rz[rr] + dz) * HY + (ry[rr] + dy)) * HX + (i + dx)) * CP + hh * 16 + kh * 8]);
+        acc[rr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[rr], 0, 0, 0);
+      }
+    }
+  }
+
+  // epilogue: acc[rr][q] holds (voxel = 8*(q/4) + 4*kh + q%4, co = i) of row rr
+  const int co = i;
+  const float bco = (MODE == CONV_OUT_F32_CL || co >= cout) ? 0.0f : bias[co];
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int gz = z0 + rz[rr], gy = y0 + ry[rr];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int vx = 8 * (q / 4) + 4 * kh + (q % 4);
+      const int64_t gv = ((int64_t)gz * G + gy) * G + x0 + vx;
+      const float val = acc[rr][q] + bco;
+      if (MODE == CONV_OUT_F32_CF) {
+        if (co < cout) y_out[((int64_t)b * cout + co) * G3 + gv] = val;
+      } else if (MODE == CONV_OUT_F32_CL) {
+        if (co < cout) {
+          const int64_t o = ((int64_t)b * G3 + gv) * cout + co;
+          y_out[o] = addend ? val + addend[o] : val;
+        }
+      } else {
+        const int64_t o = ((int64_t)b * G3 + gv) * 32 + co;
+        if (z_out) z_out[o] = f2bf(val);
+        float hv = gelu(val);
+        if (MODE == CONV_RES_GELU)
+          hv += bf2f(xl[(((rz[rr] + 1) * HY + (ry[rr] + 1)) * HX + (vx + 1)) * CP + co]);
+        h_out[o] = f2bf(hv);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ weight grad
+// dW[tap][co][ci] = sum_v dz[v][co] * x[v + o_tap][ci]; db[co] = sum_v dz[v][co].
+// Persistent workgroups walk tiles in a fixed order and write one partial per
+// workgroup; k_sum_partials adds them in workgroup order (deterministic).
+// In LDS the tile is stored channel-major so that eight consecutive voxels of
+// one channel form a 16-byte MFMA fragment: A = dz^T (co x voxels),
+// B = shifted x (voxels x ci).  Wave w accumulates taps w, w+4, ...
+constexpr int DW_TAPS_PER_WAVE = 7;  // ceil(27 / 4)
+constexpr int XS = HX + 6;           // x stride of the channel-major input tile (padded)
+constexpr int DS = CT_X + 8;         // x stride of the channel-major dz tile (padded)
+
+template <int CIN>
+__global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __restrict__ x,
+                                                              const uint16_t* __restrict__ dz, int B, int G,
+                                                              int64_t ntiles, float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) uint16_t xl[CIN * HZ * HY * XS];
+  __shared__ __attribute__((aligned(16))) uint16_t dl[32 * CT_Z * CT_Y * DS];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i = lane & 31, kh = lane >> 5;
+  const int ntx = G / CT_X, nty = G / CT_Y, ntz = G / CT_Z;
+  const int64_t G3 = (int64_t)G * G * G;
+  f32x16 acc[DW_TAPS_PER_WAVE][CIN / 32 > 0 ? CIN / 32 : 1];
+#pragma unroll
+  for (int k = 0; k < DW_TAPS_PER_WAVE; ++k)
+#pragma unroll
+    for (int n = 0; n < (CIN / 32 > 0 ? CIN / 32 : 1); ++n)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[k][n][q] = 0.0f;
+  float dbias = 0.0f;  // lanes 0..31 of wave 0 own co = lane
+
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int64_t t = tile;
+    const int bx = (int)(t % ntx);
+    t /= ntx;
+    const int by = (int)(t % nty);
+    t /= nty;
+    const int bz = (int)(t % ntz);
+    const int b = (int)(t / ntz);
+    const int x0 = bx * CT_X, y0 = by * CT_Y, z0 = bz * CT_Z;
+    __syncthreads();
+    // x tile (with halo) -> channel-major LDS
+    for (int c = tid; c < HZ * HY * HX * (CIN / 8); c += CONV_THREADS) {
+      const int ch = c % (CIN / 8);
+      int v = c / (CIN / 8);
+      const int tx = v % HX;
+      v /= HX;
+      const int ty = v % HY;
+      const int tz = v / HY;
+      const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1), gz = (z0 + tz - 1) & (G - 1);
+      const uint4 raw = *reinterpret_cast<const uint4*>(&x[((int64_t)b * G3 + ((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&raw);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) xl[(((ch * 8 + k) * HZ + tz) * HY + ty) * XS + tx] = e[k];
+    }
+    // dz tile -> channel-major LDS
+    for (int c = tid; c < CT_Z * CT_Y * CT_X * 4; c += CONV_THREADS) {
+      const int ch = c % 4;
+      int v = c / 4;
+      const int tx = v % CT_X;
+      v /= CT_X;
+      const int ty = v % CT_Y;
+      const int tz = v / CT_Y;
+      const int64_t gv = ((int64_t)(z0 + tz) * G + (y0 + ty)) * G + x0 + tx;
+      const uint4 raw = *reinterpret_cast<const uint4*>(&dz[((int64_t)b * G3 + gv) * 32 + ch * 8]);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&raw);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) dl[(((ch * 8 + k) * CT_Z + tz) * CT_Y + ty) * DS + tx] = e[k];
+    }
+    __syncthreads();
+    if (wv == 0 && lane < 32) {
+      for (int r = 0; r < CT_Z * CT_Y; ++r)
+        for (int vx = 0; vx < CT_X; ++vx) dbias += bf2f(dl[((lane * CT_Z + r / CT_Y) * CT_Y + r % CT_Y) * DS + vx]);
+    }
+    // K loop: 8 rows x 32 voxels = 16 steps of 16 voxels (two per row)
+#pragma unroll
+    for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+      const int tap = wv + 4 * k;
+      if (tap >= 27) break;
+      const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
+#pragma unroll 1
+      for (int r = 0; r < CT_Z * CT_Y; ++r) {
+        const int rz = r / CT_Y, ry = r % CT_Y;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          const int vb = half * 16 + kh * 8;  // first of this lane's 8 voxels
+          bf16x8 av;
+          const uint16_t* ap = &dl[((i * CT_Z + rz) * CT_Y + ry) * DS + vb];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) av[e] = __builtin_bit_cast(__bf16, ap[e]);
+#pragma unroll
+          for (int n = 0; n < (CIN / 32 > 0 ? CIN / 32 : 1); ++n) {
+            const int ci = n * 32 + i;
+            bf16x8 bv;
+            if (ci < CIN) {
+              const uint16_t* bp = &xl[((ci * HZ + rz + dzo) * HY + ry + dyo) * XS + vb + dxo];
+#pragma unroll
+              for (int e = 0; e < 8; ++e) bv[e] = __builtin_bit_cast(__bf16, bp[e]);
+            } else {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) bv[e] = (__bf16)0.0f;
+            }
+            acc[k][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[k][n], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  // partial layout: [wg][27][32 co][CIN] then [wg] bias block of 32
+  float* pw = part + (int64_t)blockIdx.x * (27 * 32 * CIN + 32);
+#pragma unroll
+  for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+    const int tap = wv + 4 * k;
+    if (tap >= 27) break;
+#pragma unroll
+    for (int n = 0; n < (CIN / 32 > 0 ? CIN / 32 : 1); ++n) {
+      const int ci = n * 32 + i;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int co = 8 * (q / 4) + 4 * kh + (q % 4);
+        if (ci < CIN) pw[(tap * 32 + co) * CIN + ci] = acc[k][n][q];
+      }
+    }
+  }
+  if (wv == 0 && lane < 32) pw[27 * 32 * CIN + lane] = dbias;
+}
+
+// out_w[j] (j < mw) and out_b[j - mw] = sum over partials in workgroup order
+__global__ void k_sum_partials(const float* __restrict__ part, int nparts, int64_t m, int64_t mw,
+                               float* __restrict__ out_w, float* __restrict__ out_b) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * m + j];
+    if (j < mw)
+      out_w[j] = s;
+    else
+      out_b[j - mw] = s;
+  }
+}
+
+// dz = dh * gelu'(z)  (BF16 out), dh FP32 channels-last, z BF16
+__global__ void k_gelu_bwd(const float* __restrict__ dh, const uint16_t* __restrict__ z, uint16_t* __restrict__ dz,
+                           int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dz[i] = f2bf(dh[i] * gelu_grad(bf2f(z[i])));
+}
+
+// Data-gradient weights: W'[t'][a][c] = W[26 - t'][c][a] for c < cout_fwd
+// (else 0), a < 32 rows (rows a >= cin_fwd are 0); W is [27][32][cin_fwd],
+// W' is [27][32][cinp].
+__global__ void k_flip_weights(const uint16_t* __restrict__ w, int cin_fwd, int cout_fwd, int cinp,
+                               uint16_t* __restrict__ wt) {
+  const int n = 27 * 32 * cinp;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int c = j % cinp, a = (j / cinp) % 32, tp = j / (32 * cinp);
+    wt[j] = (c < cout_fwd && a < cin_fwd) ? w[((26 - tp) * 32 + c) * cin_fwd + a] : (uint16_t)0;
+  }
+}
+
+// FP32 channels-first [B][C][G^3] * scale -> BF16 channels-last [B][G^3][CP] (zero padded)
+__global__ void k_pack_cl(const float* __restrict__ in, int C, int64_t B, int64_t G3, int CP, float scale,
+                          uint16_t* __restrict__ out) {
+  const int64_t n = B * G3 * CP;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(j % CP);
+    const int64_t v = (j / CP) % G3, b = j / (CP * G3);
+    out[j] = c < C ? f2bf(in[(b * C + c) * G3 + v] * scale) : (uint16_t)0;
+  }
+}
+
+// Adam (decoupled bias correction), FP32 master weights, optional BF16 copy.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int64_t n, float lr, float b1, float b2, float eps, float c1, float c2, float gscale,
+                       uint16_t* __restrict__ p_bf16) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const float gj = g[j] * gscale;
+    const float mj = b1 * m[j] + (1.0f - b1) * gj;
+    const float vj = b2 * v[j] + (1.0f - b2) * gj * gj;
+    m[j] = mj;
+    v[j] = vj;
+    const float pj = p[j] - lr * (mj / c1) / (sqrtf(vj / c2) + eps);
+    p[j] = pj;
+    if (p_bf16) p_bf16[j] = f2bf(pj);
+  }
+}
+
+int conv_tiles_ok(int64_t G) { return is_pow2(G) && G >= 32 && G <= 1024; }
+
+template <int CIN, int MODE>
+int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t B, int64_t G, int cout, uint16_t* z,
+               uint16_t* h, float* y, const float* addend, hipStream_t s, const char* name) {
+  const int64_t blocks = B * (G / CT_X) * (G / CT_Y) * (G / CT_Z);
+  AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
+  TimedScope ts_(name, s);
+  hipLaunchKernelGGL((k_conv3_fwd<CIN, MODE>), dim3((unsigned)blocks), dim3(CONV_THREADS), 0, s, x, w, bias, (int)G,
+                     cout, z, h, y, addend);
+  AUGUR_LAUNCHED(name);
+  return AUGUR_OK;
+}
+
+}  // namespace augur
+
+extern "C" {
+
+int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
+                         int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
+                         void* stream) {
+  AUGUR_REQUIRE(conv_tiles_ok(G), AUGUR_EUNSUPPORTED, "conv: G must be a power of two in [32, 1024]");
+  AUGUR_REQUIRE(B >= 1 && x && w, AUGUR_EINVAL, "conv: bad arguments");
+  AUGUR_REQUIRE(cin == 16 || cin == 32, AUGUR_EUNSUPPORTED, "conv: cin must be 16 or 32 (pad channels)");
+  AUGUR_REQUIRE(cout >= 1 && cout <= 32, AUGUR_EINVAL, "conv: cout must be 1..32");
+  hipStream_t s = as_stream(stream);
+  switch (mode) {
+    case CONV_OUT_F32_CF:
+      AUGUR_REQUIRE(y_out && bias, AUGUR_EINVAL, "conv: mode 0 needs y_out and bias");
+      return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr, s, "conv_fwd_out")
+                       : launch_fwd<32, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr, s, "conv_fwd_out");
+    case CONV_GELU:
+      AUGUR_REQUIRE(h_out && bias && cout == 32, AUGUR_EINVAL, "conv: mode 1 needs h_out, bias, cout 32");
+      return cin == 16 ? launch_fwd<16, CONV_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, s, "conv_fwd_gelu")
+                       : launch_fwd<32, CONV_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, s, "conv_fwd_gelu");
+    case CONV_RES_GELU:
+      AUGUR_REQUIRE(h_out && bias && cout == 32 && cin == 32, AUGUR_EINVAL, "conv: mode 2 needs cin = cout = 32");
+      return launch_fwd<32, CONV_RES_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, s, "conv_fwd_res");
+    case CONV_OUT_F32_CL:
+      AUGUR_REQUIRE(y_out, AUGUR_EINVAL, "conv: mode 3 needs y_out");
+      return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend, s, "conv_bwd_data")
+                       : launch_fwd<32, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend, s, "conv_bwd_data");
+    default:
+      set_error("conv: unknown mode %d", mode);
+      return AUGUR_EINVAL;
+  }
+}
+
+size_t augur_conv3_wgrad_workspace_bytes(int cin, int64_t B, int64_t G, int nwg) {
+  (void)B;
+  (void)G;
+  return align_up((size_t)nwg * (27 * 32 * cin + 32) * sizeof(float)) + 256;
+}
+
+int augur_conv3_bf16_wgrad(const uint16_t* x, int cin, const uint16_t* dz, int64_t B, int64_t G, float* dw, float* db,
+                           int nwg, void* ws, size_t ws_bytes, void* stream) {
+  AUGUR_REQUIRE(conv_tiles_ok(G), AUGUR_EUNSUPPORTED, "conv: G must be a power of two in [32, 1024]");
+  AUGUR_REQUIRE(cin == 16 || cin == 32, AUGUR_EUNSUPPORTED, "conv wgrad: cin must be 16 or 32");
+  AUGUR_REQUIRE(B >= 1 && x && dz && dw && db && nwg >= 1 && nwg <= 65536, AUGUR_EINVAL, "conv wgrad: bad arguments");
+  const int64_t m = 27 * 32 * cin + 32;
+  AUGUR_REQUIRE(ws && ws_bytes >= (size_t)nwg * m * sizeof(float), AUGUR_EWORKSPACE, "conv wgrad: workspace too small");
+  hipStream_t s = as_stream(stream);
+  const int64_t ntiles = B * (G / CT_X) * (G / CT_Y) * (G / CT_Z);
+  {
+    TimedScope ts_("conv_wgrad", s);
+    if (cin == 16)
+      hipLaunchKernelGGL((k_conv3_wgrad<16>), dim3(nwg), dim3(CONV_THREADS), 0, s, x, dz, (int)B, (int)G, ntiles,
+                         (float*)ws);
+    else
+      hipLaunchKernelGGL((k_conv3_wgrad<32>), dim3(nwg), dim3(CONV_THREADS), 0, s, x, dz, (int)B, (int)G, ntiles,
+                         (float*)ws);
+    AUGUR_LAUNCHED("conv_wgrad");
+  }
+  hipLaunchKernelGGL(k_sum_partials, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const float*)ws, nwg, m,
+                     (int64_t)27 * 32 * cin, dw, db);
+  AUGUR_LAUNCHED("sum_partials");
+  return AUGUR_OK;
+}
+
+int augur_conv3_flip_weights(const uint16_t* w, int cin_fwd, int cout_fwd, int cinp, uint16_t* wt, void* stream) {
+  AUGUR_REQUIRE(w && wt && cin_fwd >= 1 && cin_fwd <= 32 && cout_fwd >= 1 && cout_fwd <= cinp &&
+                    (cinp == 16 || cinp == 32),
+                AUGUR_EINVAL, "conv flip: bad arguments");
+  hipLaunchKernelGGL(k_flip_weights, dim3(blocks_for(27 * 32 * cinp, 256)), dim3(256), 0, as_stream(stream), w,
+                     cin_fwd, cout_fwd, cinp, wt);
+  AUGUR_LAUNCHED("flip_weights");
+  return AUGUR_OK;
+}
+
+int augur_gelu_bwd(const float* dh, const uint16_t* z, int64_t n, uint16_t* dz, void* stream) {
+  AUGUR_REQUIRE(n >= 0 && (n == 0 || (dh && z && dz)), AUGUR_EINVAL, "gelu_bwd: bad arguments");
+  if (n == 0) return AUGUR_OK;
+  hipStream_t s = as_stream(stream);
+  TimedScope ts_("gelu_bwd", s);
+  hipLaunchKernelGGL(k_gelu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, s, dh, z, dz, n);
+  AUGUR_LAUNCHED("gelu_bwd");
+  return AUGUR_OK;
+}
+
+int augur_pack_cl_bf16(const float* in, int C, int64_t B, int64_t G, int CP, float scale, uint16_t* out,
+                       void* stream) {
+  AUGUR_REQUIRE(in && out && C >= 1 && C <= CP && B >= 1 && G >= 1, AUGUR_EINVAL, "pack_cl: bad arguments");
+  const int64_t G3 = G * G * G;
+  hipLaunchKernelGGL(k_pack_cl, dim3(blocks_for(B * G3 * CP, 256)), dim3(256), 0, as_stream(stream), in, C, B, G3, CP,
+                     scale, out);
+  AUGUR_LAUNCHED("pack_cl");
+  return AUGUR_OK;
+}
+
+int augur_adam_update(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                      float eps, int step, float gscale, uint16_t* p_bf16, void* stream) {
+  AUGUR_REQUIRE(p && g && m && v && n >= 0 && step >= 1, AUGUR_EINVAL, "adam: bad arguments");
+  if (n == 0) return AUGUR_OK;
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p, g, m, v, n, lr, b1, b2, eps,
+                     c1, c2, gscale, p_bf16);
+  AUGUR_LAUNCHED("adam");
+  return AUGUR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,216 @@
+"""The scatter-field CNN surrogate on the BF16 MFMA convolution kernels.
+
+``LocalNet3D`` is the reference LocalNet topology (scripts/scatter_field.py:
+15-26) in 3D with periodic padding: h = gelu(conv_in(x)); for each middle
+layer h = h + gelu(conv(h)); out = conv_out(h).  With ``layers=5`` it has six
+3^3 convolutions of width 32.  Weights are FP32 masters [27][32][cin]
+(tap-major) with BF16 copies used by the kernels; activations are BF16
+channels-last with FP32 accumulation.
+
+Training (``train_step``) runs the forward with saved pre-activations, the
+backward through the project's own kernels (data gradient = forward
+convolution with flipped weights, weight gradient = MFMA reduction over
+voxels, GELU derivative) and Adam on the FP32 masters.
+"""
+import math
+
+import torch
+
+from . import _lib, ops
+from .ops import _dev, _p, _stream, _ws
+
+BF16 = torch.bfloat16
+
+MODE_OUT_CF, MODE_GELU, MODE_RES_GELU, MODE_OUT_CL = 0, 1, 2, 3
+
+
+def pad16(c):
+    return 16 if c <= 16 else 32
+
+
+def to_taps(w_torch):
+    """torch conv3d weight (co, ci, 3, 3, 3) -> (27, co, ci) tap-major."""
+    co, ci = w_torch.shape[:2]
+    return w_torch.permute(2, 3, 4, 0, 1).reshape(27, co, ci)
+
+
+def from_taps(w_taps):
+    """(27, co, ci) -> torch conv3d layout (co, ci, 3, 3, 3)."""
+    co, ci = w_taps.shape[1:]
+    return w_taps.reshape(3, 3, 3, co, ci).permute(3, 4, 0, 1, 2)
+
+
+def kernel_weights(w_taps, cinp):
+    """(27, cout, cin) FP32 -> BF16 [27][32][cinp] zero padded (kernel layout)."""
+    out = torch.zeros(27, 32, cinp, dtype=BF16, device=w_taps.device)
+    out[:, : w_taps.shape[1], : w_taps.shape[2]] = w_taps.to(BF16)
+    return out
+
+
+def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None):
+    """x: BF16 channels-last (B, G, G, G, cin); w: BF16 (27, 32, cin); bias FP32 (32,)."""
+    dev = _dev(x)
+    B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
+    y = h = None
+    if mode == MODE_OUT_CF:
+        y = torch.empty(B, cout, G, G, G, dtype=torch.float32, device=dev)
+    elif mode == MODE_OUT_CL:
+        y = torch.empty(B, G, G, G, cout, dtype=torch.float32, device=dev)
+    else:
+        h = torch.empty(B, G, G, G, 32, dtype=BF16, device=dev)
+    _lib.call("augur_conv3_bf16_fwd", _p(x), cin, _p(w), _p(bias), B, G, mode, cout, _p(z_out), _p(h), _p(y),
+              _p(addend), _stream(dev))
+    return h if h is not None else y
+
+
+def conv3_wgrad(x, dz, nwg=None):
+    """-> (dw (27, 32, cin) FP32, db (32,) FP32)."""
+    dev = _dev(x, dz)
+    B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
+    if nwg is None:
+        ntiles = B * (G // 32) * (G // 4) * (G // 2)
+        nwg = max(1, min(1024, ntiles))
+    dw = torch.empty(27, 32, cin, dtype=torch.float32, device=dev)
+    db = torch.empty(32, dtype=torch.float32, device=dev)
+    wsb = _lib.size("augur_conv3_wgrad_workspace_bytes", cin, B, G, nwg)
+    ws = _ws(wsb, dev)
+    _lib.call("augur_conv3_bf16_wgrad", _p(x), cin, _p(dz), B, G, _p(dw), _p(db), nwg, _p(ws), wsb, _stream(dev))
+    return dw, db
+
+
+def flip_weights(w, cin_fwd, cout_fwd, cinp):
+    dev = _dev(w)
+    wt = torch.empty(27, 32, cinp, dtype=BF16, device=dev)
+    _lib.call("augur_conv3_flip_weights", _p(w), cin_fwd, cout_fwd, cinp, _p(wt), _stream(dev))
+    return wt
+
+
+def gelu_bwd(dh, z):
+    dz = torch.empty(z.shape, dtype=BF16, device=z.device)
+    _lib.call("augur_gelu_bwd", _p(dh), _p(z), z.numel(), _p(dz), _stream(_dev(dh, z)))
+    return dz
+
+
+def pack_cl(x, cp, scale=1.0):
+    """FP32 (B, C, G, G, G) -> BF16 channels-last (B, G, G, G, cp)."""
+    dev = _dev(x)
+    x = x.contiguous().float()
+    B, C, G = x.shape[0], x.shape[1], x.shape[-1]
+    out = torch.empty(B, G, G, G, cp, dtype=BF16, device=dev)
+    _lib.call("augur_pack_cl_bf16", _p(x), C, B, G, cp, float(scale), _p(out), _stream(dev))
+    return out
+
+
+def adam_update(p, g, m, v, lr, step, b1=0.9, b2=0.999, eps=1e-8, gscale=1.0, p_bf16=None):
+    _lib.call("augur_adam_update", _p(p), _p(g), _p(m), _p(v), p.numel(), float(lr), float(b1), float(b2),
+              float(eps), int(step), float(gscale), _p(p_bf16), _stream(_dev(p)))
+
+
+class LocalNet3D:
+    """3D LocalNet: conv_in (cin -> 32) + GELU, (layers - 1) residual GELU
+    convs (32 -> 32), conv_out (32 -> cout, zero-initialised as in the
+    reference, scripts/scatter_field.py:135-136)."""
+
+    def __init__(self, cin=4, cout=3, width=32, layers=5, device="cuda", seed=4738, init_std=None):
+        assert width == 32, "the MFMA kernels are built for width 32"
+        self.cin, self.cout, self.layers = cin, cout, layers
+        self.cinp = pad16(cin)
+        self.device = torch.device(device)
+        g = torch.Generator().manual_seed(seed)
+        shapes = [(cin, 32)] + [(32, 32)] * (layers - 1) + [(32, cout)]
+        self.w, self.b = [], []
+        for k, (ci, co) in enumerate(shapes):
+            bound = 1.0 / math.sqrt(ci * 27)   # torch's default conv init scale
+            std = init_std if init_std is not None else bound
+            w = (torch.rand(27, co, ci, generator=g) * 2 - 1) * std
+            b = (torch.rand(co, generator=g) * 2 - 1) * std
+            if k == len(shapes) - 1 and init_std is None:
+                w.zero_()
+                b.zero_()
+            self.w.append(w.to(self.device))
+            self.b.append(torch.zeros(32, device=self.device).index_copy_(0, torch.arange(co, device=self.device),
+                                                                          b.to(self.device)))
+        self.adam = None
+        self.step_count = 0
+        self.sync_bf16()
+
+    # ----------------------------------------------------------- weights
+    def sync_bf16(self):
+        self.wk = [kernel_weights(w, pad16(w.shape[2])) for w in self.w]
+
+    def parameters(self):
+        return self.w + self.b
+
+    def state_dict(self):
+        return {"w": [w.cpu() for w in self.w], "b": [b.cpu() for b in self.b], "cin": self.cin,
+                "cout": self.cout, "layers": self.layers, "step": self.step_count}
+
+    def load_state_dict(self, sd):
+        self.w = [w.to(self.device) for w in sd["w"]]
+        self.b = [b.to(self.device) for b in sd["b"]]
+        self.step_count = sd.get("step", 0)
+        self.sync_bf16()
+
+    def torch_weights(self):
+        """[(weight (co, ci, 3, 3, 3), bias (co,))] for a torch reference."""
+        return [(from_taps(w), b[: w.shape[1]]) for w, b in zip(self.w, self.b)]
+
+    # ----------------------------------------------------------- forward
+    def __call__(self, x):
+        return self.forward(x)
+
+    def forward(self, x, save=False):
+        """x: FP32 (B, cin, G, G, G) -> FP32 (B, cout, G, G, G)."""
+        xc = pack_cl(x, self.cinp)
+        zs, hs = [], [xc]
+        need_z = save
+        z = torch.empty(xc.shape[:-1] + (32,), dtype=BF16, device=xc.device) if need_z else None
+        h = conv3_fwd(xc, self.wk[0], self.b[0], MODE_GELU, 32, z_out=z)
+        zs.append(z)
+        hs.append(h)
+        for k in range(1, self.layers):
+            z = torch.empty_like(h) if need_z else None
+            h = conv3_fwd(h, self.wk[k], self.b[k], MODE_RES_GELU, 32, z_out=z)
+            zs.append(z)
+            hs.append(h)
+        y = conv3_fwd(h, self.wk[-1], self.b[-1], MODE_OUT_CF, self.cout)
+        if save:
+            self._saved = (zs, hs)
+        return y
+
+    def backward(self, dy):
+        """dy: FP32 (B, cout, G, G, G) gradient of the loss w.r.t. the output.
+        Returns the FP32 gradients [(dw (27, co, ci), db (co,))] per layer."""
+        zs, hs = self._saved
+        grads = [None] * (self.layers + 1)
+        dyc = pack_cl(dy, 32)
+        # output layer
+        dw, db = conv3_wgrad(hs[-1], dyc)
+        grads[-1] = (dw[:, : self.cout, :], db[: self.cout])
+        wt = flip_weights(self.wk[-1], 32, self.cout, 32)
+        dh = conv3_fwd(dyc, wt, None, MODE_OUT_CL, 32)            # dL/dh_L, FP32 channels-last
+        for k in range(self.layers - 1, 0, -1):
+            dz = gelu_bwd(dh, zs[k])
+            dw, db = conv3_wgrad(hs[k], dz)
+            grads[k] = (dw, db)
+            wt = flip_weights(self.wk[k], 32, 32, 32)
+            dh = conv3_fwd(dz, wt, None, MODE_OUT_CL, 32, addend=dh)  # residual path adds dh
+        dz = gelu_bwd(dh, zs[0])
+        dw, db = conv3_wgrad(hs[0], dz)
+        grads[0] = (dw[:, :, : self.cin], db)
+        self._saved = None
+        return grads
+
+    def adam_step(self, grads, lr=3e-4, gscale=1.0):
+        """Adam on the FP32 masters (scripts/train_scatter_field.py uses torch.optim.Adam)."""
+        if self.adam is None:
+            self.adam = [((torch.zeros_like(w), torch.zeros_like(w)), (torch.zeros_like(b), torch.zeros_like(b)))
+                         for w, b in zip(self.w, self.b)]
+        self.step_count += 1
+        for k, ((dw, db), w, b) in enumerate(zip(grads, self.w, self.b)):
+            gb = torch.zeros_like(b)
+            gb[: db.shape[0]] = db
+            (mw, vw), (mb, vb) = self.adam[k]
+            adam_update(w, dw.contiguous(), mw, vw, lr, self.step_count, gscale=gscale)
+            adam_update(b, gb, mb, vb, lr, self.step_count, gscale=gscale)
+        self.sync_bf16()

@@ -201,6 +201,52 @@ int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_v
 int augur_pm_kdk(float* pos, float* vel, float* acc, const float* mass, float mass_value, int64_t n, int64_t G,
                  float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream);
 
+/* ------------------------------------------ a11 / a12 surrogate network */
+
+/* BF16 MFMA 3x3x3 periodic convolution, FP32 accumulation, on a G^3 grid
+ * (G a power of two in [32, 1024]).  x: BF16 channels-last [B][G^3][cin],
+ * cin in {16, 32} (zero-pad narrower inputs); w: BF16 [27][32][cin] with
+ * tap = (dz*3 + dy)*3 + dx for offsets d - 1 and rows co >= cout zero;
+ * bias: FP32 [32].  Modes (epilogue):
+ *   0: y_out FP32 channels-first [B][cout][G^3] = conv + bias;
+ *   1: z_out BF16 [B][G^3][32] = conv + bias (may be NULL), h_out = gelu(z);
+ *   2: as 1 with a residual: h_out = x + gelu(z) (cin = cout = 32);
+ *   3: y_out FP32 channels-last [B][G^3][cout] = conv (+ addend, same layout),
+ *      no bias (the data gradient, with augur_conv3_flip_weights()).
+ * gelu is the erf form.  Replaces the nn.Conv2d + F.gelu layers of LocalNet
+ * (scripts/scatter_field.py:15-26), here in 3D with periodic padding. */
+int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
+                         int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
+                         void* stream);
+
+/* Weight gradient dw[27][32][cin] = sum_v dz[v][co] x[v + o_tap][ci] and
+ * db[32] = sum_v dz[v][co] (FP32), dz BF16 channels-last [B][G^3][32].
+ * nwg persistent workgroups each write a partial that is summed in a fixed
+ * order (run-to-run identical).  Replaces autograd's conv weight gradient
+ * (scripts/train_scatter_field.py:225 loss.backward()). */
+size_t augur_conv3_wgrad_workspace_bytes(int cin, int64_t B, int64_t G, int nwg);
+int augur_conv3_bf16_wgrad(const uint16_t* x, int cin, const uint16_t* dz, int64_t B, int64_t G, float* dw, float* db,
+                           int nwg, void* ws, size_t ws_bytes, void* stream);
+
+/* Data-gradient weights W'[t'][a][c] = W[26 - t'][c][a] (c < cout_fwd,
+ * a < cin_fwd, else 0), W: [27][32][cin_fwd] -> W': [27][32][cinp]; the data
+ * gradient of a layer is then augur_conv3_bf16_fwd(dz, cinp, W', mode 3). */
+int augur_conv3_flip_weights(const uint16_t* w, int cin_fwd, int cout_fwd, int cinp, uint16_t* wt, void* stream);
+
+/* dz = dh * gelu'(z): dh FP32, z BF16, dz BF16, n elements. */
+int augur_gelu_bwd(const float* dh, const uint16_t* z, int64_t n, uint16_t* dz, void* stream);
+
+/* FP32 channels-first [B][C][G^3] * scale -> BF16 channels-last [B][G^3][CP],
+ * channels C..CP-1 zero. */
+int augur_pack_cl_bf16(const float* in, int C, int64_t B, int64_t G, int CP, float scale, uint16_t* out,
+                       void* stream);
+
+/* Adam step on FP32 master weights (bias-corrected, step >= 1), gradient
+ * times gscale; p_bf16 (may be NULL) receives the BF16 copy.  Replaces
+ * torch.optim.Adam in scripts/train_scatter_field.py:158-255. */
+int augur_adam_update(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                      float eps, int step, float gscale, uint16_t* p_bf16, void* stream);
+
 /* ------------------------------------------------------------ measurement */
 
 /* Per-launch timing: when enabled, each kernel launch of the library records

@@ -1,0 +1,133 @@
+"""GPU parity of the BF16 MFMA 3^3 convolution kernels against a float64
+torch reference of the same op (periodic conv3d), on BF16-representable
+inputs, plus the full LocalNet3D forward/backward against autograd."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    return torch.device("cuda:0")
+
+
+def ref_conv(x_cf, w_t, b=None):
+    """float64 periodic conv3d on the CPU. x (B, C, G, G, G), w (co, ci, 3, 3, 3)."""
+    xp = F.pad(x_cf.double().cpu(), (1, 1, 1, 1, 1, 1), mode="circular")
+    return F.conv3d(xp, w_t.double().cpu(), None if b is None else b.double().cpu())
+
+
+def bf16_round(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+@pytest.mark.parametrize("cin", [4, 16, 32])
+def test_conv_forward_modes(dev, cin):
+    from augur_amd import surrogate as S
+    g = torch.Generator().manual_seed(cin)
+    B, G = 2, 32
+    cinp = S.pad16(cin)
+    x = bf16_round(torch.randn(B, cin, G, G, G, generator=g))
+    w = bf16_round(torch.randn(27, 32, cin, generator=g) * 0.1)
+    b = torch.randn(32, generator=g) * 0.1
+    xc = S.pack_cl(x.to(dev), cinp)
+    wk = S.kernel_weights(w.to(dev), cinp)
+    ref = ref_conv(x, S.from_taps(w), b)                     # (B, 32, G, G, G) float64
+    # mode 0: FP32 channels-first, only the first 7 channels
+    y = S.conv3_fwd(xc, wk, b.to(dev), S.MODE_OUT_CF, 7).cpu().double()
+    assert (y - ref[:, :7]).abs().max() < 1e-4 * ref.abs().max()
+    # mode 1: z and gelu(z) in BF16
+    z = torch.empty(B, G, G, G, 32, dtype=torch.bfloat16, device=dev)
+    h = S.conv3_fwd(xc, wk, b.to(dev), S.MODE_GELU, 32, z_out=z)
+    refz = ref.permute(0, 2, 3, 4, 1)
+    assert (z.cpu().double() - refz).abs().max() < 1e-2 * refz.abs().max()
+    assert (h.cpu().double() - F.gelu(refz)).abs().max() < 1e-2 * refz.abs().max()
+    if cin == 32:
+        h2 = S.conv3_fwd(xc, wk, b.to(dev), S.MODE_RES_GELU, 32)
+        exp = x.double().permute(0, 2, 3, 4, 1) + F.gelu(refz)
+        assert (h2.cpu().double() - exp).abs().max() < 1.5e-2 * exp.abs().max()
+
+
+def test_conv_data_gradient_is_transposed_conv(dev):
+    from augur_amd import surrogate as S
+    g = torch.Generator().manual_seed(7)
+    B, G = 1, 32
+    dz = bf16_round(torch.randn(B, 32, G, G, G, generator=g))
+    w = bf16_round(torch.randn(27, 3, 32, generator=g) * 0.1)     # an output layer: 32 -> 3
+    x = torch.randn(B, 32, G, G, G, dtype=torch.float64, requires_grad=True)
+    y = ref_conv(x, S.from_taps(w))
+    y.backward(dz[:, :3].double())
+    wk = S.kernel_weights(w.to(dev), 32)
+    wt = S.flip_weights(wk, 32, 3, 32)
+    dzc = S.pack_cl(dz[:, :3].to(dev), 32)
+    dx = S.conv3_fwd(dzc, wt, None, S.MODE_OUT_CL, 32).cpu().double().permute(0, 4, 1, 2, 3)
+    assert (dx - x.grad).abs().max() < 1e-4 * x.grad.abs().max()
+
+
+@pytest.mark.parametrize("cin", [16, 32])
+def test_conv_weight_gradient(dev, cin):
+    from augur_amd import surrogate as S
+    g = torch.Generator().manual_seed(11 + cin)
+    B, G = 2, 32
+    x = bf16_round(torch.randn(B, cin, G, G, G, generator=g))
+    dz = bf16_round(torch.randn(B, 32, G, G, G, generator=g))
+    wt = torch.zeros(32, cin, 3, 3, 3, dtype=torch.float64, requires_grad=True)
+    bt = torch.zeros(32, dtype=torch.float64, requires_grad=True)
+    ref_conv(x, wt, bt).backward(dz.double())
+    xc = S.pack_cl(x.to(dev), cin)
+    dzc = S.pack_cl(dz.to(dev), 32)
+    for nwg in (1, 37, 256):
+        dw, db = S.conv3_wgrad(xc, dzc, nwg=nwg)
+        ref = S.to_taps(wt.grad)
+        assert (dw.cpu().double() - ref).abs().max() < 1e-4 * ref.abs().max()
+        assert (db.cpu().double() - bt.grad).abs().max() < 1e-4 * bt.grad.abs().max()
+
+
+def test_localnet3d_forward_backward_vs_autograd(dev):
+    from augur_amd import surrogate as S
+    from oracle.torch_cpu import localnet3d
+    B, G = 1, 32
+    net = S.LocalNet3D(cin=4, cout=3, layers=5, device=dev, init_std=0.05)
+    g = torch.Generator().manual_seed(3)
+    x = torch.randn(B, 4, G, G, G, generator=g)
+    y = net.forward(x.to(dev), save=True)
+    tw = [(w.detach().cpu().double().requires_grad_(True), b.detach().cpu().double().requires_grad_(True))
+          for w, b in net.torch_weights()]
+    # the reference sees the same BF16-rounded weights and input
+    with torch.no_grad():
+        for (w, b) in tw:
+            w.copy_(bf16_round(w.float()).double())
+    yr = localnet3d(bf16_round(x).double(), [w for w, _ in tw], [b for _, b in tw])
+    rel = ((y.cpu().double() - yr) ** 2).sum().sqrt() / (yr ** 2).sum().sqrt()
+    assert rel < 3e-2, rel
+    dy = torch.randn(yr.shape, generator=g, dtype=torch.float64)
+    yr.backward(dy)
+    grads = net.backward(dy.float().to(dev))
+    for k, ((dw, db), (w, b)) in enumerate(zip(grads, tw)):
+        ref = S.to_taps(w.grad)
+        rel = ((dw.cpu().double() - ref) ** 2).sum().sqrt() / (ref ** 2).sum().sqrt()
+        assert rel < 5e-2, (k, float(rel))
+        relb = ((db.cpu().double() - b.grad) ** 2).sum().sqrt() / (b.grad ** 2).sum().sqrt()
+        assert relb < 5e-2, (k, float(relb))
+
+
+def test_adam_matches_torch(dev):
+    from augur_amd import surrogate as S
+    g = torch.Generator().manual_seed(5)
+    p0 = torch.randn(1000, generator=g)
+    p = p0.clone().to(dev)
+    m = torch.zeros_like(p)
+    v = torch.zeros_like(p)
+    tp = p0.clone().double().requires_grad_(True)
+    opt = torch.optim.Adam([tp], lr=3e-4)
+    for step in range(1, 4):
+        gr = torch.randn(1000, generator=g)
+        S.adam_update(p, gr.to(dev), m, v, 3e-4, step)
+        tp.grad = gr.double()
+        opt.step()
+    assert (p.cpu().double() - tp.detach()).abs().max() < 1e-6
