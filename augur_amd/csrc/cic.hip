@@ -78,7 +78,9 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_u32(uint32_t v, uint32_
   return pre + x - v;
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scan_block(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* sums) {
+__global__ __launch_bounds__(SC_THREADS) void k_scan_block(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* sums,
+                                                           const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   const int64_t base = (int64_t)blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
   uint32_t v[SC_ITEMS];
   uint32_t local = 0;
@@ -97,7 +99,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_block(const uint32_t* in, u
   if (threadIdx.x == 0 && sums) sums[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scan_add(uint32_t* out, int64_t n, const uint32_t* offs) {
+__global__ __launch_bounds__(SC_THREADS) void k_scan_add(uint32_t* out, int64_t n, const uint32_t* offs,
+                                                         const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   const int64_t base = (int64_t)blockIdx.x * SC_TILE;
   const uint32_t o = offs[blockIdx.x];
   for (int j = threadIdx.x; j < SC_TILE; j += SC_THREADS)
@@ -114,22 +118,22 @@ size_t scan_workspace_bytes(int64_t n) {
 }
 
 // out may alias in.
-int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s) {
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s, const uint32_t* gate) {
   if (n <= 0) return AUGUR_OK;
   int64_t nb = (n + SC_TILE - 1) / SC_TILE;
   if (nb == 1) {
-    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SC_THREADS), 0, s, in, out, n, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SC_THREADS), 0, s, in, out, n, (uint32_t*)nullptr, gate);
     AUGUR_LAUNCHED("scan_block");
     return AUGUR_OK;
   }
   uint32_t* sums = (uint32_t*)ws;
   uint32_t* offs = (uint32_t*)((char*)ws + align_up(nb * sizeof(uint32_t)));
   void* rest = (char*)offs + align_up(nb * sizeof(uint32_t));
-  hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, in, out, n, sums);
+  hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, in, out, n, sums, gate);
   AUGUR_LAUNCHED("scan_block");
-  int rc = exclusive_scan_u32(sums, offs, nb, rest, s);
+  int rc = exclusive_scan_u32(sums, offs, nb, rest, s, gate);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, out, n, offs);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, out, n, offs, gate);
   AUGUR_LAUNCHED("scan_add");
   return AUGUR_OK;
 }
@@ -144,7 +148,8 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, h
 constexpr int RS_THREADS = 256, RS_TILE = 4096, RS_WAVE_RUN = 1024;
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, int64_t n, int shift, uint32_t* hist,
-                                                        int nblocks) {
+                                                        int nblocks, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -159,7 +164,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, in
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                                                            uint32_t* vout, int64_t n, int shift, const uint32_t* offs,
-                                                           int nblocks) {
+                                                           int nblocks, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t cnt[RS_THREADS / 64][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < (RS_THREADS / 64) * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
@@ -229,7 +235,7 @@ size_t radix_workspace_bytes(int64_t n) {
 // *keys_out / *vals_out point at the sorted arrays (inside ws or the inputs).
 // keys/vals are clobbered.  If vals == nullptr they are initialised to iota.
 int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* ws, size_t ws_bytes,
-                     uint32_t** keys_out, uint32_t** vals_out, hipStream_t s) {
+                     uint32_t** keys_out, uint32_t** vals_out, hipStream_t s, const uint32_t* gate) {
   const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
   Carver c(ws, ws_bytes);
   uint32_t* k2 = c.take<uint32_t>(n);
@@ -243,12 +249,12 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* 
   }
   uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
   for (int shift = 0; shift < bits; shift += 8) {
-    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ka, n, shift, hist, (int)nb);
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ka, n, shift, hist, (int)nb, gate);
     AUGUR_LAUNCHED("rs_hist");
-    int rc = exclusive_scan_u32(hist, hist, nb * 256, sws, s);
+    int rc = exclusive_scan_u32(hist, hist, nb * 256, sws, s, gate);
     if (rc) return rc;
     hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n, shift, hist,
-                       (int)nb);
+                       (int)nb, gate);
     AUGUR_LAUNCHED("rs_scatter");
     uint32_t* t = ka;
     ka = kb;

@@ -7,10 +7,13 @@ namespace augur {
 int fill_u64(void* p, int64_t n, unsigned long long v, hipStream_t s);
 int fill_u32(void* p, int64_t n, uint32_t v, hipStream_t s);
 size_t scan_workspace_bytes(int64_t n);
-int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s);
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s,
+                       const uint32_t* gate = nullptr);
 size_t radix_workspace_bytes(int64_t n);
 int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* ws, size_t ws_bytes,
-                     uint32_t** keys_out, uint32_t** vals_out, hipStream_t s);
+                     uint32_t** keys_out, uint32_t** vals_out, hipStream_t s, const uint32_t* gate = nullptr);
+
+int fft_r2c_xy(const float* in, float2* out, int64_t G, hipStream_t s);
 
 // Per-launch event timing (see timing.hip).  A TimedScope around a launch
 // records a start/stop event pair on the stream when timing is enabled and

@@ -1,68 +1,472 @@
-// Periodic PM stepper: particles held sorted by CIC base cell; one step is
-// deposit -> rfftn -> Green's function + spectral gradient -> irfftn ->
-// fused CIC gather + kick (+ drift of the next step).
-#include "internal.hpp"
+// Periodic PM stepper on gfx950.
+//
+// State: particles physically sorted by CIC base cell (key = (z*G + y)*G + x
+// of floor((pos/h) - 0.5) wrapped), with cell_start[] (G^3 + 1 offsets) from
+// the last sort.  One step:
+//   1. deposit   k_dep_tiles: one workgroup per 32x32x16 tile of the grid
+//                reads the particles whose base cell (as of the last sort)
+//                lies in the tile's halo-extended region grown by the drift
+//                bound D, and accumulates exact fixed-point CIC contributions
+//                with 64-bit LDS integer atomics; every grid cell is written
+//                once (no global atomics, bitwise order-independent).
+//   2. FFT       x-rows R2C -> y-columns -> z-columns fused with the Green's
+//                function and the inverse z transform (two outputs) ->
+//                inverse y with three outputs -> x-rows C2R of the three
+//                acceleration components written as one float4 grid.
+//   3. gather    k_gather_kdk: CIC gather of the float4 grid fused with the
+//                closing half kick, the next opening half kick and the drift,
+//                and the max-speed reduction that bounds the next D.
+// D (cells) is the cumulative max drift since the last sort; when it would
+// exceed 1 the gated re-sort kernels run (they are always launched and exit
+// at once when not needed, so the step has no host sync and is capturable).
+#include "fft_core.hpp"
 
 using namespace augur;
 using namespace augur_dev;
 
 namespace augur {
 
-struct PmBufs {
-  unsigned long long* fixed;  // G^3 fixed-point deposit accumulator
-  unsigned* maxbits;          // deposit scale bound
-  float* rho;                 // G^3
-  float* spec;                // G*G*(G/2+1) complex
-  float* aspec;               // 3 * G*G*(G/2+1) complex
-  float* force;               // 3 * G^3: [a_x; a_y; a_z]
+// --------------------------------------------------------------- bookkeeping
+struct PmDev {
+  float cum_cells;      // cumulative max drift in cells since the last sort
+  uint32_t vmax_bits;   // max |v| component seen by the last drift (float bits)
+  uint32_t need_sort;   // gate of the re-sort kernels
+  uint32_t maxbits;     // deposit bound: float bits of max |m * scale|
+  uint32_t pad[4];
 };
 
-static bool carve_pm(Carver& c, int64_t G, PmBufs& b) {
-  const int64_t G3 = G * G * G, nc = G * G * (G / 2 + 1);
-  b.fixed = c.take<unsigned long long>(G3);
-  b.maxbits = c.take<unsigned>(4);
-  b.rho = c.take<float>(G3);
-  b.spec = c.take<float>(2 * nc);
-  b.aspec = c.take<float>(6 * nc);
-  b.force = c.take<float>(3 * G3);
-  return c.ok();
+constexpr float DRIFT_MARGIN = 1e-3f;  // cells; covers position rounding
+constexpr int DMAX = 1;                // largest halo the deposit supports
+
+__device__ __forceinline__ int drift_cells(const PmDev* st) {
+  const float c = st->cum_cells;
+  return c <= 0.0f ? 0 : (int)ceilf(c + DRIFT_MARGIN);
 }
 
-__global__ void k_kick_drift(float* __restrict__ pos, float* __restrict__ vel, const float* __restrict__ acc, int64_t n,
-                             float hdt, float dt, float L) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      float v = vel[3 * i + d] + hdt * acc[3 * i + d];
-      float x = pos[3 * i + d] + dt * v;
-      if (x >= L) x -= L;
-      if (x < 0.0f) x += L;
-      vel[3 * i + d] = v;
-      pos[3 * i + d] = x;
-    }
+// cum += dt*vmax/h; vmax = 0; need_sort = (D would exceed DMAX)
+__global__ void k_pm_decide(PmDev* st, float dt_over_h, int force_sort) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float v = __uint_as_float(st->vmax_bits);
+  st->cum_cells += dt_over_h * v;
+  st->vmax_bits = 0u;
+  st->need_sort = (force_sort || st->cum_cells + DRIFT_MARGIN > (float)DMAX) ? 1u : 0u;
+}
+
+__global__ void k_pm_sorted(PmDev* st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (st->need_sort) {
+    st->cum_cells = 0.0f;
+    st->need_sort = 0u;
   }
 }
 
-// a = CIC gather of the three force grids at pos; acc = a; v += dt/2 a; and,
-// unless this is the last step, the next step's v += dt/2 a; x += dt v.
-// With do_kick == 0 only acc is written (force evaluation).
-__global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, float* __restrict__ vel,
-                                                    float* __restrict__ acc, const float* __restrict__ force, int64_t n,
-                                                    int G, float h, float hdt, float dt, float L, int do_kick,
-                                                    int drift) {
-  const int64_t G3 = (int64_t)G * G * G;
+__global__ void k_pm_set_maxbits(PmDev* st, const float* mass, float mass_value, float scale, int64_t n) {
+  // uniform mass: the bound is |mass_value * scale| (same FP32 product as the deposit)
+  if (mass == nullptr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->maxbits = __float_as_uint(fabsf(__fmul_rn(mass_value, scale)));
+    return;
+  }
+  float mx = 0.0f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    mx = fmaxf(mx, fabsf(__fmul_rn(mass[i], scale)));
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&st->maxbits, __float_as_uint(mx));
+}
+
+__device__ __forceinline__ double pm_fixed_scale(uint32_t maxbits, int64_t n) {
+  const float m = __uint_as_float(maxbits);
+  if (!(m > 0.0f)) return 1.0;
+  int e = (int)floor(62.0 - log2((double)m * (double)(n > 0 ? n : 1)));
+  e = e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
+  return ldexp(1.0, e);
+}
+
+// ------------------------------------------------------------- gated sort
+__global__ void k_base_keys(const float* __restrict__ pos, int64_t n, int G, float h, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = wrap((int)floorf(cic_u(pos[3 * i + d], 0.0f, h)), G);
+    keys[i] = ((uint32_t)c[2] * G + c[1]) * G + c[0];
+    vals[i] = (uint32_t)i;
+  }
+}
+
+template <class T, int K>
+__global__ void k_permute(const T* __restrict__ src, T* __restrict__ dst, const uint32_t* __restrict__ perm, int64_t n,
+                          const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = perm[i];
+#pragma unroll
+    for (int k = 0; k < K; ++k) dst[i * K + k] = src[j * K + k];
+  }
+}
+
+template <class T>
+__global__ void k_copy(const T* __restrict__ src, T* __restrict__ dst, int64_t n, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_zero_u32(uint32_t* p, int64_t n, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ cnt,
+                              const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[keys[i]], 1u);
+}
+
+// ---------------------------------------------------------- tile deposit
+constexpr int TD_X = 32, TD_Y = 32, TD_Z = 16;
+constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;  // 16384 int64 = 128 KB of LDS
+constexpr int TD_THREADS = 512;
+constexpr int TD_MAXSEG = (TD_Y + 1 + 2 * DMAX) * (TD_Z + 1 + 2 * DMAX) * 2;
+constexpr int TD_UNROLL = 4;
+
+struct AxisRange {
+  int lo, len;  // wrapped start and length of the extended range (len <= G)
+};
+
+__device__ __forceinline__ AxisRange ext_range(int t0, int T, int D, int G) {
+  AxisRange r;
+  r.len = T + 1 + 2 * D;
+  r.lo = t0 - 1 - D;
+  if (r.len >= G) {
+    r.lo = 0;
+    r.len = G;
+  } else {
+    r.lo &= (G - 1);
+  }
+  return r;
+}
+
+// perm (may be NULL): particle index of sorted slot i; otherwise slot i is particle i.
+__global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(const float* __restrict__ pos,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const float* __restrict__ mass, float mass_value, int64_t n,
+                                                          int G, float h, float scale,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const PmDev* __restrict__ st, float* __restrict__ rho) {
+  __shared__ unsigned long long acc[TD_CELLS];
+  __shared__ uint32_t seg_lo[TD_MAXSEG];
+  __shared__ uint32_t seg_pre[TD_MAXSEG + 1];
+  __shared__ int nseg_sh;
+  const int tid = threadIdx.x;
+  const int ntx = G / TD_X, nty = G / TD_Y;
+  const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
+  const int x0 = bx * TD_X, y0 = by * TD_Y, z0 = bz * TD_Z;
+  // the re-sort gate keeps D <= DMAX; the clamp only protects the LDS segment table
+  const int D = min(drift_cells(st), DMAX);
+  const double S = pm_fixed_scale(st->maxbits, n);
+  for (int i = tid; i < TD_CELLS; i += TD_THREADS) acc[i] = 0ull;
+
+  // segments: one per (z', y') row of the extended region, split at the x wrap
+  const AxisRange rx = ext_range(x0, TD_X, D, G), ry = ext_range(y0, TD_Y, D, G), rz = ext_range(z0, TD_Z, D, G);
+  const int nrow = ry.len * rz.len;
+  const int xsplit = (rx.lo + rx.len > G) ? 2 : 1;
+  for (int r = tid; r < nrow * 2; r += TD_THREADS) {
+    const int part = r & 1, row = r >> 1;
+    const int yy = (ry.lo + row % ry.len) & (G - 1), zz = (rz.lo + row / ry.len) & (G - 1);
+    const uint32_t rbase = ((uint32_t)zz * G + yy) * G;
+    uint32_t lo = 0, hi = 0;
+    if (part < xsplit) {
+      int xa, xb;  // [xa, xb) in row coordinates
+      if (xsplit == 1) {
+        xa = rx.lo;
+        xb = rx.lo + rx.len;
+      } else if (part == 0) {
+        xa = rx.lo;
+        xb = G;
+      } else {
+        xa = 0;
+        xb = rx.lo + rx.len - G;
+      }
+      lo = cell_start[rbase + xa];
+      hi = cell_start[rbase + xb];
+    }
+    seg_lo[r] = lo;
+    seg_pre[r + 1] = hi - lo;
+  }
+  if (tid == 0) {
+    seg_pre[0] = 0;
+    nseg_sh = nrow * 2;
+  }
+  __syncthreads();
+  // inclusive scan of segment lengths (serial over waves: nseg <= TD_MAXSEG)
+  const int nseg = nseg_sh;
+  if (tid < 64) {
+    uint32_t carry = 0;
+    for (int base = 1; base <= nseg; base += 64) {
+      const int j = base + tid;
+      uint32_t v = j <= nseg ? seg_pre[j] : 0u;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (tid >= o) v += y;
+      }
+      if (j <= nseg) seg_pre[j] = v + carry;
+      carry += __shfl(v, 63, 64);
+    }
+  }
+  __syncthreads();
+  const uint32_t total = seg_pre[nseg];
+
+  for (uint32_t w0 = tid; w0 < total; w0 += TD_THREADS * TD_UNROLL) {
+    float px[TD_UNROLL][3], pm[TD_UNROLL];
+    bool ok[TD_UNROLL];
+#pragma unroll
+    for (int u = 0; u < TD_UNROLL; ++u) {
+      const uint32_t w = w0 + u * TD_THREADS;
+      ok[u] = w < total;
+      pm[u] = mass_value;
+      px[u][0] = px[u][1] = px[u][2] = 0.0f;
+      if (ok[u]) {
+        int a = 0, b = nseg;  // seg_pre[a] <= w < seg_pre[b]
+        while (b - a > 1) {
+          const int m = (a + b) >> 1;
+          if (seg_pre[m] <= w) a = m; else b = m;
+        }
+        int64_t slot = (int64_t)seg_lo[a] + (w - seg_pre[a]);
+        const int64_t p = perm ? (int64_t)perm[slot] : slot;
+        px[u][0] = pos[3 * p];
+        px[u][1] = pos[3 * p + 1];
+        px[u][2] = pos[3 * p + 2];
+        if (mass) pm[u] = mass[p];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TD_UNROLL; ++u) {
+      if (!ok[u]) continue;
+      int i0[3];
+      float wv[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float uu = cic_u(px[u][d], 0.0f, h);
+        const float f = floorf(uu);
+        i0[d] = (int)f;
+        wv[d] = __fsub_rn(uu, f);
+      }
+      const float val = __fmul_rn(pm[u], scale);
+      // local tile coordinates of the two corners along each axis (-1 = outside)
+      int lx[2], ly[2], lz[2];
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        const int ax = (i0[0] + o - x0) & (G - 1);
+        const int ay = (i0[1] + o - y0) & (G - 1);
+        const int az = (i0[2] + o - z0) & (G - 1);
+        lx[o] = ax < TD_X ? ax : -1;
+        ly[o] = ay < TD_Y ? ay : -1;
+        lz[o] = az < TD_Z ? az : -1;
+      }
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+        if (lx[ox] < 0 || ly[oy] < 0 || lz[oz] < 0) continue;
+        const float wx = ox ? wv[0] : __fsub_rn(1.0f, wv[0]);
+        const float wy = oy ? wv[1] : __fsub_rn(1.0f, wv[1]);
+        const float wz = oz ? wv[2] : __fsub_rn(1.0f, wv[2]);
+        const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
+        const long long q = (long long)rint((double)__fmul_rn(val, wt) * S);
+        if (q) atomicAdd(&acc[(lz[oz] * TD_Y + ly[oy]) * TD_X + lx[ox]], (unsigned long long)q);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
+    const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
+    const int64_t cell = ((int64_t)(z0 + lz) * G + (y0 + ly)) * G + (x0 + lx);
+    rho[cell] = (float)((double)(long long)acc[i] / S);
+  }
+}
+
+// ------------------------------------------------------ fused FFT passes
+// Spectral layout after the x R2C pass: (z, y|ky, kx) with nxc = G/2 + 1.
+constexpr int PM_FFT_THREADS = 256;
+
+template <int N>
+struct ColTile {
+  static constexpr int W = (4096 / N) > 16 ? 16 : (4096 / N);
+  static constexpr int LS = N + 1;
+};
+
+// z columns at fixed ky: forward z FFT, Green's function g = -4 pi G / k^2 / G^3,
+// outputs A = IFFTz(g rho) and B = IFFTz(-i kz g rho) (Nyquist kz zeroed).
+// In place: A overwrites the input; B goes to outB.
+template <int N>
+__global__ __launch_bounds__(PM_FFT_THREADS) void k_pm_zpass(float2* __restrict__ spec, float2* __restrict__ outB,
+                                                             float kf, float green) {
+  constexpr int W = ColTile<N>::W, LS = ColTile<N>::LS, nxc = N / 2 + 1;
+  __shared__ float2 b0[W * LS], b1[W * LS], b2[W * LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  const int nblk = (nxc + W - 1) / W;
+  const int ky = blockIdx.x / nblk;
+  const int c0 = (blockIdx.x % nblk) * W;
+  const int64_t base = (int64_t)ky * nxc + c0, es = (int64_t)N * nxc;
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    b0[c * LS + k] = (c0 + c < nxc) ? spec[base + k * es + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* r = lds_fft<N, false>(b0, b1, tw, W, LS);  // result in b1 (odd #stages) or b0
+  float2* fa = r;
+  float2* fb = (r == b0) ? b1 : b0;
+  const int ny = ky <= N / 2 ? ky : ky - N;
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e / N, kz = e % N;
+    const int kx = c0 + c;
+    const int nz = kz <= N / 2 ? kz : kz - N;
+    const float fx = kf * kx, fy = kf * ny, fz = kf * nz;
+    const float k2 = fx * fx + fy * fy + fz * fz;
+    const float2 v = fa[c * LS + kz];
+    const float g = (kx == 0 && ky == 0 && kz == 0) ? 0.0f : green / k2;
+    const float2 p = make_float2(v.x * g, v.y * g);
+    fa[c * LS + kz] = p;
+    const float az = (kz == N / 2) ? 0.0f : fz;
+    fb[c * LS + kz] = make_float2(az * p.y, -az * p.x);  // -i kz p
+  }
+  __syncthreads();
+  float2* scratch = b2;
+  float2* ra = lds_fft<N, true>(fa, scratch, tw, W, LS);
+  float2* freeb = (ra == fa) ? scratch : fa;
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    if (c0 + c < nxc) spec[base + k * es + c] = ra[c * LS + k];
+  }
+  __syncthreads();
+  float2* rb = lds_fft<N, true>(fb, freeb, tw, W, LS);
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    if (c0 + c < nxc) outB[base + k * es + c] = rb[c * LS + k];
+  }
+}
+
+// y columns at fixed z: reads A (in0) and B (in2); writes
+//   out0 = IFFTy(A)            (in place of A; the x pass applies -i kx)
+//   out1 = IFFTy(-i ky A)      (Nyquist ky zeroed)
+//   out2 = IFFTy(B)            (in place of B)
+template <int N>
+__global__ __launch_bounds__(PM_FFT_THREADS) void k_pm_yinv(float2* __restrict__ s0, float2* __restrict__ s1,
+                                                            float2* __restrict__ s2, float kf) {
+  constexpr int W = ColTile<N>::W, LS = ColTile<N>::LS, nxc = N / 2 + 1;
+  __shared__ float2 b0[W * LS], b1[W * LS], b2[W * LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  const int nblk = (nxc + W - 1) / W;
+  const int z = blockIdx.x / nblk;
+  const int c0 = (blockIdx.x % nblk) * W;
+  const int64_t base = (int64_t)z * N * nxc + c0, es = nxc;
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    const float2 v = (c0 + c < nxc) ? s0[base + k * es + c] : make_float2(0.f, 0.f);
+    b0[c * LS + k] = v;
+  }
+  __syncthreads();
+  // b1 = -i ky A with ky = kf * signed index, Nyquist zeroed
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e / N, ky = e % N;
+    const float2 v = b0[c * LS + ky];
+    const float f = (ky == N / 2) ? 0.0f : kf * (float)(ky <= N / 2 ? ky : ky - N);
+    b1[c * LS + ky] = make_float2(f * v.y, -f * v.x);
+  }
+  __syncthreads();
+  float2* r0 = lds_fft<N, true>(b0, b2, tw, W, LS);
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    if (c0 + c < nxc) s0[base + k * es + c] = r0[c * LS + k];
+  }
+  __syncthreads();
+  float2* free0 = (r0 == b0) ? b2 : b0;
+  float2* r1 = lds_fft<N, true>(b1, free0, tw, W, LS);
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    if (c0 + c < nxc) s1[base + k * es + c] = r1[c * LS + k];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    b0[c * LS + k] = (c0 + c < nxc) ? s2[base + k * es + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* r2 = lds_fft<N, true>(b0, b2, tw, W, LS);
+  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
+    const int c = e % W, k = e / W;
+    if (c0 + c < nxc) s2[base + k * es + c] = r2[c * LS + k];
+  }
+}
+
+// x rows: three half spectra (A~, -i ky A~, B~) -> real rows; the first gets
+// -i kx (Nyquist kx zeroed); ky's and kx's 2 pi / L factor is `kf`.
+// Output: float4 grid (a_x, a_y, a_z, 0).
+template <int N>
+__global__ __launch_bounds__(PM_FFT_THREADS) void k_pm_xc2r(const float2* __restrict__ s0,
+                                                            const float2* __restrict__ s1,
+                                                            const float2* __restrict__ s2, float kf,
+                                                            float4* __restrict__ force) {
+  constexpr int L = (2048 / N) > 1 ? 2048 / N : 1;
+  constexpr int LS = N + 1, nxc = N / 2 + 1;
+  __shared__ float2 b0[L * LS], b1[L * LS];
+  __shared__ float res[3][L * N];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const float2* src[3] = {s0, s1, s2};
+#pragma unroll 1
+  for (int f = 0; f < 3; ++f) {
+    for (int e = threadIdx.x; e < L * N; e += PM_FFT_THREADS) {
+      const int l = e / N, k = e % N;
+      const int64_t row = row0 + l;
+      const int kk = k < nxc ? k : N - k;
+      float2 v = src[f][row * nxc + kk];
+      if (k >= nxc) v.y = -v.y;
+      if (f == 0) {
+        // -i kx v with signed kx; zero at Nyquist
+        const float fxk = (k == N / 2) ? 0.0f : kf * (float)(k < nxc ? k : k - N);
+        v = make_float2(fxk * v.y, -fxk * v.x);
+      }
+      if (k == 0 || k == N / 2) v.y = 0.0f;
+      b0[l * LS + k] = v;
+    }
+    __syncthreads();
+    float2* r = lds_fft<N, true>(b0, b1, tw, L, LS);
+    for (int e = threadIdx.x; e < L * N; e += PM_FFT_THREADS) res[f][e] = r[(e / N) * LS + (e % N)].x;
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < L * N; e += PM_FFT_THREADS)
+    force[row0 * N + e] = make_float4(res[0][e], res[1][e], res[2][e], 0.0f);
+}
+
+// ------------------------------------------------------------- gather + KDK
+__global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, float* __restrict__ vel,
+                                                    float* __restrict__ acc, const float4* __restrict__ force,
+                                                    const uint32_t* __restrict__ perm, int64_t n, int G, float h,
+                                                    float hdt, float dt, float L, int do_kick, int drift,
+                                                    PmDev* __restrict__ st) {
+  float vmax = 0.0f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pi = perm ? (int64_t)perm[i] : i;
     float p[3];
     int i0[3];
     float w[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      p[d] = pos[3 * i + d];
+      p[d] = pos[3 * pi + d];
       const float u = cic_u(p[d], 0.0f, h);
       const float f = floorf(u);
       i0[d] = (int)f;
       w[d] = __fsub_rn(u, f);
     }
+    const int x0 = wrap(i0[0], G), x1 = wrap(i0[0] + 1, G);
+    const int y0 = wrap(i0[1], G), y1 = wrap(i0[1] + 1, G);
+    const int z0 = wrap(i0[2], G), z1 = wrap(i0[2] + 1, G);
     float a[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int ox = 0; ox < 2; ++ox)
@@ -74,75 +478,234 @@ __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, flo
           const float wy = oy ? w[1] : __fsub_rn(1.0f, w[1]);
           const float wz = oz ? w[2] : __fsub_rn(1.0f, w[2]);
           const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
-          const int64_t cell =
-              ((int64_t)wrap(i0[2] + oz, G) * G + wrap(i0[1] + oy, G)) * G + wrap(i0[0] + ox, G);
-          a[0] += force[cell] * wt;
-          a[1] += force[G3 + cell] * wt;
-          a[2] += force[2 * G3 + cell] * wt;
+          const int64_t cell = ((int64_t)(oz ? z1 : z0) * G + (oy ? y1 : y0)) * G + (ox ? x1 : x0);
+          const float4 f = force[cell];
+          a[0] += f.x * wt;
+          a[1] += f.y * wt;
+          a[2] += f.z * wt;
         }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[3 * i + d] = a[d];
+    for (int d = 0; d < 3; ++d) acc[3 * pi + d] = a[d];
     if (do_kick) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        float v = vel[3 * i + d] + hdt * a[d];
+        float v = vel[3 * pi + d] + hdt * a[d];
         if (drift) {
           v = v + hdt * a[d];
           float x = p[d] + dt * v;
           if (x >= L) x -= L;
           if (x < 0.0f) x += L;
-          pos[3 * i + d] = x;
+          pos[3 * pi + d] = x;
+          vmax = fmaxf(vmax, fabsf(v));
         }
-        vel[3 * i + d] = v;
+        vel[3 * pi + d] = v;
       }
     }
   }
-}
-
-__global__ void k_base_keys(const float* __restrict__ pos, int64_t n, int G, float h, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ vals) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    int c[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) c[d] = wrap((int)floorf(cic_u(pos[3 * i + d], 0.0f, h)), G);
-    keys[i] = ((uint32_t)c[2] * G + c[1]) * G + c[0];
-    vals[i] = (uint32_t)i;
+  if (st && drift) {
+    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&st->vmax_bits, __float_as_uint(vmax));
   }
 }
 
-template <class T, int K>
-__global__ void k_permute(const T* __restrict__ src, T* __restrict__ dst, const uint32_t* __restrict__ perm, int64_t n) {
+__global__ void k_kick_drift(float* __restrict__ pos, float* __restrict__ vel, const float* __restrict__ acc, int64_t n,
+                             float hdt, float dt, float L, PmDev* __restrict__ st) {
+  float vmax = 0.0f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = perm[i];
 #pragma unroll
-    for (int k = 0; k < K; ++k) dst[i * K + k] = src[j * K + k];
+    for (int d = 0; d < 3; ++d) {
+      float v = vel[3 * i + d] + hdt * acc[3 * i + d];
+      float x = pos[3 * i + d] + dt * v;
+      if (x >= L) x -= L;
+      if (x < 0.0f) x += L;
+      vel[3 * i + d] = v;
+      pos[3 * i + d] = x;
+      vmax = fmaxf(vmax, fabsf(v));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&st->vmax_bits, __float_as_uint(vmax));
+}
+
+__global__ void k_pm_init(PmDev* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->cum_cells = 0.0f;
+    st->vmax_bits = 0u;
+    st->need_sort = 0u;
+    st->maxbits = 0u;
   }
 }
 
-// rho = deposit(pos) -> force grids (3, G^3)
-static int pm_force_grids(const float* pos, const float* mass, float mass_value, int64_t n, int64_t G, float L,
-                          float Gconst, PmBufs& b, hipStream_t s) {
-  const float h = (float)((double)L / (double)G);
-  const float scale = (float)(1.0 / ((double)h * (double)h * (double)h));
-  const size_t dep_ws = augur_cic_deposit_workspace_bytes(1, 3, G, 1);
-  // The deposit's own workspace layout is (fixed grid, maxbits): b.fixed and
-  // b.maxbits are carved consecutively with the same alignment.
-  int rc = augur_cic_deposit(pos, nullptr, mass, mass_value, nullptr, 1, n, 3, G, 0.0f, h, 1, scale, b.rho, b.fixed,
-                             dep_ws, s);
-  if (rc) return rc;
-  const int64_t shape[3] = {G, G, G};
-  rc = augur_fft_r2c(b.rho, nullptr, b.spec, 3, shape, 1, nullptr, 0, s);
-  if (rc) return rc;
-  const float inv = (float)(1.0 / ((double)G * (double)G * (double)G));
-  rc = augur_poisson_green_grad(b.spec, 1, G, L, Gconst, inv, b.aspec, nullptr, s);
-  if (rc) return rc;
-  return augur_fft_c2r(b.aspec, b.force, nullptr, 3, shape, 3, 1.0f, nullptr, 0, s);
+// ------------------------------------------------------------- workspace
+struct PmBufs {
+  PmDev* st;
+  uint32_t* cell_start;  // G^3 + 1
+  float* rho;            // G^3
+  float2* s0;            // G*G*nxc complex (spectrum / A / out0)
+  float2* s1;            // out1
+  float2* s2;            // B / out2
+  float4* force;         // G^3 float4
+  uint32_t* keys;        // n
+  uint32_t* vals;        // n
+  void* rws;             // radix sort scratch
+  void* scan_ws;         // scan scratch (G^3 + 1)
+  float* tmp;            // 3n
+  uint32_t* perm;        // n (for unsorted force evaluation)
+};
+
+static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
+  const int64_t G3 = G * G * G, nc = G * G * (G / 2 + 1);
+  b.st = c.take<PmDev>(1);
+  b.cell_start = c.take<uint32_t>(G3 + 1);
+  b.rho = c.take<float>(G3);
+  b.s0 = c.take<float2>(nc);
+  b.s1 = c.take<float2>(nc);
+  b.s2 = c.take<float2>(nc);
+  b.force = c.take<float4>(G3);
+  b.keys = c.take<uint32_t>(n);
+  b.vals = c.take<uint32_t>(n);
+  b.rws = c.take<char>(radix_workspace_bytes(n));
+  b.scan_ws = c.take<char>(scan_workspace_bytes(G3 + 1));
+  b.tmp = c.take<float>(3 * n);
+  b.perm = c.take<uint32_t>(n);
+  return c.ok();
 }
 
 static int check_pm(int64_t n, int64_t G, float L) {
   AUGUR_REQUIRE(n >= 1 && n < ((int64_t)1 << 31), AUGUR_EINVAL, "pm: need 1 <= n < 2^31");
-  AUGUR_REQUIRE(is_pow2(G) && G >= 8 && G <= 512, AUGUR_EUNSUPPORTED, "pm: G must be a power of two in [8, 512]");
+  AUGUR_REQUIRE(is_pow2(G) && G >= 32 && G <= 512, AUGUR_EUNSUPPORTED, "pm: G must be a power of two in [32, 512]");
   AUGUR_REQUIRE(L > 0.0f, AUGUR_EINVAL, "pm: L must be positive");
+  return AUGUR_OK;
+}
+
+// Keys of the current positions, stable radix sort, cell_start; with
+// physical != 0 the state arrays are permuted into sorted order, else the
+// sorted order is left in b.perm.  All kernels are gated by `gate`.
+static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids, int64_t n, int64_t G, float L,
+                   PmBufs& b, const uint32_t* gate, bool physical, hipStream_t s) {
+  TimedScope ts_("pm_sort", s);
+  const float h = (float)((double)L / (double)G);
+  const int64_t G3 = G * G * G;
+  const dim3 g(blocks_for(n, 256)), bl(256);
+  hipLaunchKernelGGL(k_base_keys, g, bl, 0, s, pos, n, (int)G, h, b.keys, b.vals, gate);
+  AUGUR_LAUNCHED("base_keys");
+  uint32_t *ks, *perm;
+  int rc = radix_sort_pairs(b.keys, b.vals, n, 3 * ilog2(G), b.rws, radix_workspace_bytes(n), &ks, &perm, s, gate);
+  if (rc) return rc;
+  // cell_start = exclusive scan of per-cell counts
+  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(G3 + 1, 256)), bl, 0, s, b.cell_start, G3 + 1, gate);
+  AUGUR_LAUNCHED("zero_cells");
+  hipLaunchKernelGGL(k_count_cells, g, bl, 0, s, ks, n, b.cell_start, gate);
+  AUGUR_LAUNCHED("count_cells");
+  rc = exclusive_scan_u32(b.cell_start, b.cell_start, G3 + 1, b.scan_ws, s, gate);
+  if (rc) return rc;
+  if (!physical) {
+    hipLaunchKernelGGL(k_copy<uint32_t>, g, bl, 0, s, perm, b.perm, n, gate);
+    AUGUR_LAUNCHED("copy_perm");
+    return AUGUR_OK;
+  }
+  float* arrs3[3] = {pos, vel, acc};
+  for (float* a : arrs3) {
+    if (!a) continue;
+    hipLaunchKernelGGL((k_permute<float, 3>), g, bl, 0, s, a, b.tmp, perm, n, gate);
+    AUGUR_LAUNCHED("permute3");
+    hipLaunchKernelGGL(k_copy<float>, dim3(blocks_for(3 * n, 256)), bl, 0, s, b.tmp, a, 3 * n, gate);
+    AUGUR_LAUNCHED("copy3");
+  }
+  if (mass) {
+    hipLaunchKernelGGL((k_permute<float, 1>), g, bl, 0, s, mass, b.tmp, perm, n, gate);
+    AUGUR_LAUNCHED("permute1");
+    hipLaunchKernelGGL(k_copy<float>, g, bl, 0, s, b.tmp, mass, n, gate);
+    AUGUR_LAUNCHED("copy1");
+  }
+  if (ids) {
+    hipLaunchKernelGGL((k_permute<int32_t, 1>), g, bl, 0, s, ids, (int32_t*)b.tmp, perm, n, gate);
+    AUGUR_LAUNCHED("permute_ids");
+    hipLaunchKernelGGL(k_copy<int32_t>, g, bl, 0, s, (const int32_t*)b.tmp, ids, n, gate);
+    AUGUR_LAUNCHED("copy_ids");
+  }
+  return AUGUR_OK;
+}
+
+#define PM_SWITCH_G(G, KERNEL, grid, block, s, ...)                                     \
+  switch (G) {                                                                            \
+    case 32: hipLaunchKernelGGL(KERNEL<32>, grid, block, 0, s, __VA_ARGS__); break;       \
+    case 64: hipLaunchKernelGGL(KERNEL<64>, grid, block, 0, s, __VA_ARGS__); break;       \
+    case 128: hipLaunchKernelGGL(KERNEL<128>, grid, block, 0, s, __VA_ARGS__); break;     \
+    case 256: hipLaunchKernelGGL(KERNEL<256>, grid, block, 0, s, __VA_ARGS__); break;     \
+    case 512: hipLaunchKernelGGL(KERNEL<512>, grid, block, 0, s, __VA_ARGS__); break;     \
+    default: set_error("pm: unsupported G %d", (int)(G)); return AUGUR_EUNSUPPORTED;    \
+  }
+
+template <int N>
+static int ncol_blocks() {
+  return (N / 2 + 1 + ColTile<N>::W - 1) / ColTile<N>::W;
+}
+
+static int col_blocks(int G) {
+  switch (G) {
+    case 32: return ncol_blocks<32>();
+    case 64: return ncol_blocks<64>();
+    case 128: return ncol_blocks<128>();
+    case 256: return ncol_blocks<256>();
+    default: return ncol_blocks<512>();
+  }
+}
+
+// deposit of sorted slots (optional perm) -> b.rho
+static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
+                      int64_t G, float L, PmBufs& b, hipStream_t s) {
+  const float h = (float)((double)L / (double)G);
+  const float scale = (float)(1.0 / ((double)h * (double)h * (double)h));
+  {
+    TimedScope ts_("dep_bound", s);
+    if (mass) {
+      hipLaunchKernelGGL(k_pm_set_maxbits, dim3(1), dim3(64), 0, s, b.st, (const float*)nullptr, 0.0f, scale, n);
+      AUGUR_LAUNCHED("dep_bound_reset");
+      hipLaunchKernelGGL(k_pm_set_maxbits, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, b.st, mass, mass_value,
+                         scale, n);
+    } else
+      hipLaunchKernelGGL(k_pm_set_maxbits, dim3(1), dim3(64), 0, s, b.st, (const float*)nullptr, mass_value, scale, n);
+    AUGUR_LAUNCHED("dep_bound");
+  }
+  {
+    TimedScope ts_("dep_tiles", s);
+    const int tiles = (int)((G / TD_X) * (G / TD_Y) * (G / TD_Z));
+    hipLaunchKernelGGL(k_dep_tiles, dim3(tiles), dim3(TD_THREADS), 0, s, pos, perm, mass, mass_value, n, (int)G, h,
+                       scale, b.cell_start, b.st, b.rho);
+    AUGUR_LAUNCHED("dep_tiles");
+  }
+  return AUGUR_OK;
+}
+
+// deposit -> FFT Poisson + spectral gradient -> float4 force grid
+static int pm_force_grid(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
+                         int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
+  int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s);
+  if (rc) return rc;
+  rc = fft_r2c_xy(b.rho, b.s0, G, s);  // x rows R2C + y columns forward
+  if (rc) return rc;
+  const double PI = 3.14159265358979323846;
+  const float kf = (float)(2.0 * PI / (double)L);
+  const float green = (float)(-4.0 * PI * (double)Gconst / ((double)G * G * G));
+  const dim3 cg((unsigned)(G * col_blocks((int)G))), fb(PM_FFT_THREADS);
+  {
+    TimedScope ts_("pm_zpass", s);
+    PM_SWITCH_G(G, k_pm_zpass, cg, fb, s, b.s0, b.s2, kf, green);
+    AUGUR_LAUNCHED("pm_zpass");
+  }
+  {
+    TimedScope ts_("pm_yinv", s);
+    PM_SWITCH_G(G, k_pm_yinv, cg, fb, s, b.s0, b.s1, b.s2, kf);
+    AUGUR_LAUNCHED("pm_yinv");
+  }
+  {
+    const int L_rows = (int)((2048 / G) > 1 ? 2048 / G : 1);
+    TimedScope ts_("pm_xc2r", s);
+    PM_SWITCH_G(G, k_pm_xc2r, dim3((unsigned)(G * G / L_rows)), fb, s, (const float2*)b.s0, (const float2*)b.s1,
+                (const float2*)b.s2, kf, b.force);
+    AUGUR_LAUNCHED("pm_xc2r");
+  }
   return AUGUR_OK;
 }
 
@@ -151,11 +714,10 @@ static int check_pm(int64_t n, int64_t G, float L) {
 extern "C" {
 
 size_t augur_pm_workspace_bytes(int64_t n, int64_t G) {
-  const int64_t G3 = G * G * G, nc = G * G * (G / 2 + 1);
-  size_t grid = align_up(G3 * 8) + align_up(16) + align_up(G3 * 4) + align_up(nc * 8) + align_up(3 * nc * 8) +
-                align_up(3 * G3 * 4);
-  size_t sort = 2 * align_up(n * 4) + radix_workspace_bytes(n) + align_up(n * 3 * 4);
-  return grid + sort + 4096;
+  Carver c(nullptr, 0);
+  PmBufs b;
+  carve_pm(c, n, G, b);
+  return c.used + 4096;
 }
 
 int augur_pm_prepare(float* pos, float* vel, float* acc, float* mass, int32_t* ids, int64_t n, int64_t G, float L,
@@ -166,37 +728,11 @@ int augur_pm_prepare(float* pos, float* vel, float* acc, float* mass, int32_t* i
   hipStream_t s = as_stream(stream);
   Carver c(ws, ws_bytes);
   PmBufs b;
-  carve_pm(c, G, b);
-  uint32_t* keys = c.take<uint32_t>(n);
-  uint32_t* vals = c.take<uint32_t>(n);
-  void* rws = c.take<char>(radix_workspace_bytes(n));
-  float* tmp = c.take<float>(n * 3);
-  AUGUR_REQUIRE(c.ok(), AUGUR_EWORKSPACE, "pm_prepare: workspace too small (%zu < %zu)", ws_bytes, c.used);
-  const float h = (float)((double)L / (double)G);
-  hipLaunchKernelGGL(k_base_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, n, (int)G, h, keys, vals);
-  AUGUR_LAUNCHED("base_keys");
-  uint32_t *ks, *perm;
-  rc = radix_sort_pairs(keys, vals, n, 3 * ilog2(G), rws, radix_workspace_bytes(n), &ks, &perm, s);
-  if (rc) return rc;
-  const dim3 g(blocks_for(n, 256)), bl(256);
-  float* arrs3[3] = {pos, vel, acc};
-  for (float* a : arrs3) {
-    if (!a) continue;
-    hipLaunchKernelGGL((k_permute<float, 3>), g, bl, 0, s, a, tmp, perm, n);
-    AUGUR_LAUNCHED("permute3");
-    AUGUR_HIP(hipMemcpyAsync(a, tmp, n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
-  }
-  if (mass) {
-    hipLaunchKernelGGL((k_permute<float, 1>), g, bl, 0, s, mass, tmp, perm, n);
-    AUGUR_LAUNCHED("permute1");
-    AUGUR_HIP(hipMemcpyAsync(mass, tmp, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-  }
-  if (ids) {
-    hipLaunchKernelGGL((k_permute<int32_t, 1>), g, bl, 0, s, ids, (int32_t*)tmp, perm, n);
-    AUGUR_LAUNCHED("permute_ids");
-    AUGUR_HIP(hipMemcpyAsync(ids, tmp, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-  }
-  return AUGUR_OK;
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_prepare: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
+  hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
+  AUGUR_LAUNCHED("pm_init");
+  return pm_sort(pos, vel, acc, mass, ids, n, G, L, b, nullptr, true, s);
 }
 
 int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_value, int64_t n, int64_t G, float L,
@@ -207,18 +743,41 @@ int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_v
   hipStream_t s = as_stream(stream);
   Carver c(ws, ws_bytes);
   PmBufs b;
-  AUGUR_REQUIRE(carve_pm(c, G, b), AUGUR_EWORKSPACE, "pm_accel: workspace too small (%zu < %zu)", ws_bytes, c.used);
-  rc = pm_force_grids(pos, mass, mass_value, n, G, L, Gconst, b, s);
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_accel: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  // any input order: sort an index permutation (D = 0), deposit through it
+  hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
+  AUGUR_LAUNCHED("pm_init");
+  rc = pm_sort((float*)pos, nullptr, nullptr, nullptr, nullptr, n, G, L, b, nullptr, false, s);
+  if (rc) return rc;
+  rc = pm_force_grid(pos, b.perm, mass, mass_value, n, G, L, Gconst, b, s);
   if (rc) return rc;
   const float h = (float)((double)L / (double)G);
   TimedScope ts_("gather", s);
   hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256)), dim3(256), 0, s, (float*)pos, (float*)nullptr, acc,
-                     b.force, n, (int)G, h, 0.0f, 0.0f, L, 0, 0);
-  AUGUR_LAUNCHED("gather_kdk");
+                     b.force, b.perm, n, (int)G, h, 0.0f, 0.0f, L, 0, 0, (PmDev*)nullptr);
+  AUGUR_LAUNCHED("gather");
   return AUGUR_OK;
 }
 
-int augur_pm_kdk(float* pos, float* vel, float* acc, const float* mass, float mass_value, int64_t n, int64_t G,
+int augur_pm_density(const float* pos, const float* mass, float mass_value, int64_t n, int64_t G, float L, float* rho,
+                     void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_pm(n, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(pos && rho, AUGUR_EINVAL, "pm_density: null pointer");
+  hipStream_t s = as_stream(stream);
+  Carver c(ws, ws_bytes);
+  PmBufs b;
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_density: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
+  hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
+  AUGUR_LAUNCHED("pm_init");
+  rc = pm_sort((float*)pos, nullptr, nullptr, nullptr, nullptr, n, G, L, b, nullptr, false, s);
+  if (rc) return rc;
+  b.rho = rho;
+  return pm_deposit(pos, b.perm, mass, mass_value, n, G, L, b, s);
+}
+
+int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n, int64_t G,
                  float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream) {
   int rc = check_pm(n, G, L);
   if (rc) return rc;
@@ -227,21 +786,28 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, const float* mass, float ma
   hipStream_t s = as_stream(stream);
   Carver c(ws, ws_bytes);
   PmBufs b;
-  AUGUR_REQUIRE(carve_pm(c, G, b), AUGUR_EWORKSPACE, "pm_kdk: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_kdk: workspace too small (%zu < %zu)", ws_bytes, c.used);
   const float h = (float)((double)L / (double)G);
   const float hdt = 0.5f * dt;
   {
     TimedScope ts_("kick_drift", s);
-    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, vel, acc, n, hdt, dt, L);
+    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, vel, acc, n, hdt, dt, L, b.st);
     AUGUR_LAUNCHED("kick_drift");
   }
   for (int st = 0; st < nsteps; ++st) {
-    rc = pm_force_grids(pos, mass, mass_value, n, G, L, Gconst, b, s);
+    hipLaunchKernelGGL(k_pm_decide, dim3(1), dim3(64), 0, s, b.st, dt / h, 0);
+    AUGUR_LAUNCHED("pm_decide");
+    // the state arrays (ids included) are permuted in place when the gate opens
+    rc = pm_sort(pos, vel, acc, mass, ids, n, G, L, b, &b.st->need_sort, true, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st);
+    AUGUR_LAUNCHED("pm_sorted");
+    rc = pm_force_grid(pos, nullptr, mass, mass_value, n, G, L, Gconst, b, s);
     if (rc) return rc;
     const int last = st == nsteps - 1;
     TimedScope ts_("gather_kdk", s);
-    hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, vel, acc, b.force, n, (int)G, h,
-                       hdt, dt, L, 1, last ? 0 : 1);
+    hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, vel, acc, b.force,
+                       (const uint32_t*)nullptr, n, (int)G, h, hdt, dt, L, 1, last ? 0 : 1, b.st);
     AUGUR_LAUNCHED("gather_kdk");
   }
   return AUGUR_OK;

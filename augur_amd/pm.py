@@ -1,19 +1,17 @@
 """Plain periodic particle-mesh gravity on one MI355X.
 
-``PeriodicPM`` keeps its particles sorted by CIC base cell in device memory
-and runs the whole step through the HIP library:
-
-    deposit (deterministic fixed-point CIC) -> rfftn -> Green's function
-    -4 pi G / k^2 -> spectral gradient -> irfftn -> fused CIC gather + kick
-    + drift.
+``PeriodicPM`` runs the whole step through the HIP library: particles held
+sorted by CIC base cell, a tiled deterministic fixed-point deposit, the FFT
+Poisson solve with Green's function -4 pi G / k^2 and spectral gradient, and
+a CIC gather fused with the KDK kick and drift.  The state is re-sorted on the
+device whenever the accumulated drift could move a particle by more than one
+cell since the last sort.
 
 It replaces, for the 3D periodic problem, the reference's pm_acc
 (scripts/pm_baseline_force_error.py:68-76) and its KDK integrators
 (scripts/gravity_sim.py:31-44, ScatterField._step verlet at
 scripts/scatter_field.py:677-688).
 """
-import ctypes
-
 import torch
 
 from . import _lib
@@ -22,38 +20,46 @@ from .ops import _dev, _p, _stream, _ws
 
 class PMState:
     """Particles of one scene in sorted (cell) order: pos, vel, acc (n, 3),
-    ids (n,) original index, mass (n,) or None with mass_value."""
+    ids (n,) original index, mass (n,) or None with mass_value, and the
+    stepper's workspace (cell offsets of the last sort and the drift bound)."""
 
-    def __init__(self, pos, vel, acc, ids, mass, mass_value):
+    def __init__(self, pos, vel, acc, ids, mass, mass_value, ws):
         self.pos, self.vel, self.acc, self.ids, self.mass, self.mass_value = pos, vel, acc, ids, mass, mass_value
+        self.ws = ws
 
     @property
     def n(self):
         return self.pos.shape[0]
 
 
+def _mass_args(mass, mass_value, n, device):
+    if torch.is_tensor(mass):
+        return mass.detach().to(device, torch.float32).reshape(-1).contiguous().clone(), 0.0
+    if mass is not None:
+        return None, float(mass)
+    return None, float(1.0 / n if mass_value is None else mass_value)
+
+
 class PeriodicPM:
-    """Periodic cube [0, L)^3 with G^3 cells (G a power of two, 8..512)."""
+    """Periodic cube [0, L)^3 with G^3 cells (G a power of two, 32..512)."""
 
     def __init__(self, G, L=1.0, Gconst=1.0, device="cuda"):
-        if G & (G - 1) or not 8 <= G <= 512:
-            raise ValueError("G must be a power of two in [8, 512]")
+        if G & (G - 1) or not 32 <= G <= 512:
+            raise ValueError("G must be a power of two in [32, 512]")
         self.G, self.L, self.Gconst = int(G), float(L), float(Gconst)
         self.h = self.L / self.G
         self.device = torch.device(device)
-        self._ws = None
-        self._ws_n = -1
+        self._scratch = None
+        self._scratch_n = -1
 
-    def workspace(self, n):
-        if self._ws is None or self._ws_n < n:
-            nb = _lib.size("augur_pm_workspace_bytes", n, self.G)
-            self._ws = _ws(nb, self.device)
-            self._ws_n = n
-        return self._ws
+    def workspace_bytes(self, n):
+        return _lib.size("augur_pm_workspace_bytes", n, self.G)
 
-    def _wsargs(self, n):
-        ws = self.workspace(n)
-        return _p(ws), ws.numel()
+    def _scratch_ws(self, n):
+        if self._scratch is None or self._scratch_n < n:
+            self._scratch = _ws(self.workspace_bytes(n), self.device)
+            self._scratch_n = n
+        return self._scratch
 
     # ---------------------------------------------------------------- state
     def load(self, pos, vel=None, mass=None, mass_value=None, compute_acc=True):
@@ -66,37 +72,23 @@ class PeriodicPM:
                vel.detach().to(self.device, torch.float32).reshape(-1, 3).contiguous().clone())
         acc = torch.zeros_like(pos)
         ids = torch.arange(n, dtype=torch.int32, device=self.device)
-        m = None
-        if torch.is_tensor(mass):
-            m = mass.detach().to(self.device, torch.float32).reshape(-1).contiguous().clone()
-            mass_value = 0.0
-        elif mass is not None:
-            mass_value = float(mass)
-        elif mass_value is None:
-            mass_value = 1.0 / n
-        st = PMState(pos, vel, acc, ids, m, float(mass_value))
+        m, mv = _mass_args(mass, mass_value, n, self.device)
+        st = PMState(pos, vel, acc, ids, m, mv, _ws(self.workspace_bytes(n), self.device))
         self.sort(st)
         if compute_acc:
-            self.accel_state(st)
+            st.acc = self.accel(st.pos, st.mass if st.mass is not None else None, st.mass_value)
         return st
 
     def sort(self, st):
         """Stable physical reorder of the state by CIC base cell."""
-        wp, wn = self._wsargs(st.n)
         _lib.call("augur_pm_prepare", _p(st.pos), _p(st.vel), _p(st.acc), _p(st.mass), _p(st.ids), st.n, self.G,
-                  self.L, wp, wn, _stream(self.device))
-
-    def accel_state(self, st):
-        wp, wn = self._wsargs(st.n)
-        _lib.call("augur_pm_accel", _p(st.pos), _p(st.acc), _p(st.mass), st.mass_value, st.n, self.G, self.L,
-                  self.Gconst, wp, wn, _stream(self.device))
-        return st.acc
+                  self.L, _p(st.ws), st.ws.numel(), _stream(self.device))
 
     def kdk(self, st, dt, steps):
         """`steps` KDK leapfrog steps in place (v += dt/2 a; x += dt v; a = PM(x); v += dt/2 a)."""
-        wp, wn = self._wsargs(st.n)
-        _lib.call("augur_pm_kdk", _p(st.pos), _p(st.vel), _p(st.acc), _p(st.mass), st.mass_value, st.n, self.G,
-                  self.L, self.Gconst, float(dt), int(steps), wp, wn, _stream(self.device))
+        _lib.call("augur_pm_kdk", _p(st.pos), _p(st.vel), _p(st.acc), _p(st.mass), _p(st.ids), st.mass_value, st.n,
+                  self.G, self.L, self.Gconst, float(dt), int(steps), _p(st.ws), st.ws.numel(),
+                  _stream(self.device))
         return st
 
     def unload(self, st):
@@ -115,19 +107,24 @@ class PeriodicPM:
         pos = pos.detach().to(self.device, torch.float32).reshape(-1, 3).contiguous()
         _dev(pos)
         n = pos.shape[0]
-        m = None
-        if torch.is_tensor(mass):
-            m = mass.detach().to(self.device, torch.float32).reshape(-1).contiguous()
-            mass_value = 0.0
-        elif mass is not None:
-            mass_value = float(mass)
-        elif mass_value is None:
-            mass_value = 1.0 / n
+        m, mv = _mass_args(mass, mass_value, n, self.device)
         acc = torch.empty_like(pos)
-        wp, wn = self._wsargs(n)
-        _lib.call("augur_pm_accel", _p(pos), _p(acc), _p(m), float(mass_value), n, self.G, self.L, self.Gconst, wp,
-                  wn, _stream(self.device))
+        ws = self._scratch_ws(n)
+        _lib.call("augur_pm_accel", _p(pos), _p(acc), _p(m), mv, n, self.G, self.L, self.Gconst, _p(ws), ws.numel(),
+                  _stream(self.device))
         return acc
+
+    def density(self, pos, mass=None, mass_value=None):
+        """Periodic CIC mass density (G, G, G) by the sort-by-cell tiled deposit."""
+        pos = pos.detach().to(self.device, torch.float32).reshape(-1, 3).contiguous()
+        _dev(pos)
+        n = pos.shape[0]
+        m, mv = _mass_args(mass, mass_value, n, self.device)
+        rho = torch.empty(self.G, self.G, self.G, dtype=torch.float32, device=self.device)
+        ws = self._scratch_ws(n)
+        _lib.call("augur_pm_density", _p(pos), _p(m), mv, n, self.G, self.L, _p(rho), _p(ws), ws.numel(),
+                  _stream(self.device))
+        return rho
 
     def rollout(self, pos, vel, mass=None, dt=1e-3, steps=1):
         """Load, run `steps` KDK steps, unload: returns (pos, vel, acc)."""

@@ -181,15 +181,20 @@ int augur_reduce_energy_momentum_f64(const float* vel, const float* mass, float 
 /* ----------------------------------------------- periodic PM stepper (a7) */
 
 /* The plain PM step on a periodic cube [0, L)^3, G^3 cells (G a power of two
- * in [8, 512]), particles held sorted by CIC base cell in caller-owned state:
+ * in [32, 512]), particles held sorted by CIC base cell in caller-owned state:
  *   pos, vel, acc: (n, 3) FP32; ids: (n) int32 original index; mass (n) or
- *   NULL (= mass_value).
+ *   NULL (= mass_value).  The workspace also carries the stepper's device
+ *   bookkeeping (cell offsets of the last sort, drift bound): use one
+ *   workspace per particle state.
  * augur_pm_prepare(): stable sort of the state by cell (physical reorder of
- *   pos, vel, acc, mass, ids) and the fixed-point scale of the deposit.
- * augur_pm_accel(): acc = PM force at pos (deposit -> rfftn -> Green's
- *   function -> spectral gradient -> irfftn -> CIC gather).
+ *   pos, vel, acc, mass, ids) and the cell offsets of the deposit.
+ * augur_pm_accel(): acc = PM force at pos in any order (index sort by cell,
+ *   tiled deposit -> FFT Poisson + spectral gradient -> CIC gather); uses
+ *   the workspace as scratch (it resets the stepper bookkeeping).
  * augur_pm_kdk(): nsteps of KDK leapfrog with the fused gather + kick + drift:
- *   v += dt/2 a; x += dt v (wrapped); a = PM(x); v += dt/2 a.
+ *   v += dt/2 a; x += dt v (wrapped); a = PM(x); v += dt/2 a.  The state is
+ *   re-sorted on the device (pos, vel, acc, mass, ids permuted together)
+ *   whenever the accumulated drift since the last sort could exceed one cell.
  * Together these replace ScatterField._step (verlet; scripts/scatter_field.py:
  * 677-688), pm_acc (scripts/pm_baseline_force_error.py:68-76) and
  * gravity_sim.rollout's KDK (scripts/gravity_sim.py:31-44) for 3D periodic PM. */
@@ -198,7 +203,12 @@ int augur_pm_prepare(float* pos, float* vel, float* acc, float* mass, int32_t* i
                      void* ws, size_t ws_bytes, void* stream);
 int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_value, int64_t n, int64_t G, float L,
                    float Gconst, void* ws, size_t ws_bytes, void* stream);
-int augur_pm_kdk(float* pos, float* vel, float* acc, const float* mass, float mass_value, int64_t n, int64_t G,
+/* Periodic CIC mass density rho (G^3, FP32) of particles in any order by the
+ * same sort-by-cell tiled deposit (scratch workspace as augur_pm_accel).
+ * Replaces cic_density (model/tiered/energy.py:9-34) for the periodic cube. */
+int augur_pm_density(const float* pos, const float* mass, float mass_value, int64_t n, int64_t G, float L, float* rho,
+                     void* ws, size_t ws_bytes, void* stream);
+int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n, int64_t G,
                  float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------ a11 / a12 surrogate network */

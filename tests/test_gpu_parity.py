@@ -231,12 +231,38 @@ def test_pm_acc_2d_reference_mode_parity(dev):
 
 
 # ------------------------------------------------------- 3D periodic PM path
-def test_pm_accel_3d_matches_oracle(dev):
+def scene3d(rng, n, centre=0.5, sigma=0.05):
+    pos = np.mod(np.concatenate([rng.uniform(0, 1, (n // 2, 3)), centre + sigma * rng.normal(size=(n - n // 2, 3))]),
+                 1.0).astype(F32)
+    return np.where(pos >= 1.0, np.float32(0.0), pos).astype(F32)
+
+
+@pytest.mark.parametrize("G,masses", [(32, False), (64, True), (128, False)])
+def test_pm_density_tiled_equals_fixed_point_deposit_bitwise(dev, G, masses):
+    # the sort-by-cell LDS-tiled deposit and the generic fixed-point deposit
+    # accumulate the same integers: bitwise equal grids
+    from augur_amd import ops
+    from augur_amd.pm import PeriodicPM
+    rng = np.random.default_rng(G)
+    n = 50000
+    pos = scene3d(rng, n, centre=0.02)          # clump across the periodic corner
+    mass = rng.uniform(0.5, 2.0, n).astype(F32) if masses else None
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    rho = pm.density(T(pos, dev), None if mass is None else T(mass, dev), mass_value=1.0 / n)
+    ref = ops.cic_deposit(T(pos[None], dev), None if mass is None else T(mass[None], dev), G, 1.0 / G, 0.0, True,
+                          mass_value=1.0 / n)[0, 0]
+    assert torch.equal(rho.view(torch.int32), ref.view(torch.int32))
+    orc = opm.cic_deposit(pos[None], (mass if masses else np.full(n, 1.0 / n))[None], np.ones((1, n)), G, 1.0 / G,
+                          0.0, True)[0, 0]
+    assert np.abs(rho.cpu().numpy() - orc).max() < 2e-6 * orc.max()
+
+
+@pytest.mark.parametrize("G", [32, 64, 128])
+def test_pm_accel_3d_matches_oracle(dev, G):
     from augur_amd.pm import PeriodicPM
     rng = np.random.default_rng(11)
-    n, G = 8192, 32
-    pos = np.mod(np.concatenate([rng.uniform(0, 1, (n // 2, 3)), 0.5 + 0.05 * rng.normal(size=(n // 2, 3))]), 1.0)
-    pos = pos.astype(F32)
+    n = 8192
+    pos = scene3d(rng, n)
     pm = PeriodicPM(G, 1.0, 1.0, dev)
     a = pm.accel(T(pos, dev)).cpu().numpy()
     ref = opm.pm_acc_3d_periodic(pos, 1.0 / n, G)
@@ -262,6 +288,27 @@ def test_pm_kdk_matches_oracle(dev):
     dp = np.abs(((p - rp) + 0.5) % 1.0 - 0.5).max()
     assert dp < 2e-5, dp
     assert np.abs(v - rv).max() < 2e-4 * np.abs(rv).max()
+
+
+def test_pm_kdk_with_device_resorts(dev):
+    # fast particles: the drift bound forces device-side re-sorts during the
+    # steps (pos, vel, acc, ids permuted together); results stay in parity and
+    # unload() returns the original particle order
+    from augur_amd.pm import PeriodicPM
+    rng = np.random.default_rng(21)
+    n, G, dt, steps = 6000, 64, 2e-3, 12
+    pos = scene3d(rng, n, sigma=0.1)
+    vel = (3.0 * rng.normal(size=(n, 3))).astype(F32)        # ~0.4 cells per step at G=64
+    mass = rng.uniform(0.5, 1.5, n).astype(F32) / n
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    st = pm.load(T(pos, dev), T(vel, dev), T(mass, dev))
+    pm.kdk(st, dt, steps)
+    p, v, _ = (x.cpu().numpy() for x in pm.unload(st))
+    assert sorted(st.ids.cpu().tolist()) == list(range(n))
+    rp, rv = opm.kdk_pm_rollout(pos, vel, mass, G, dt, steps)
+    dp = np.abs(((p - rp) + 0.5) % 1.0 - 0.5).max()
+    assert dp < 5e-5, dp
+    assert np.abs(v - rv).max() < 1e-4 * np.abs(rv).max()
 
 
 # --------------------------------------------------------------- a9 oracle
