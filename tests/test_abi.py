@@ -31,7 +31,10 @@ def test_workspace_queries_are_host_only(lib):
     ("augur_cic_deposit", (None, None, None, 1.0, None, 1, 10, 3, 12, 0.0, 1.0, 1, 1.0, None, None, 0, None),
      "power-of-two"),
     ("augur_cell_keys", (None, None, 1, 4, 5, 0.0, 1.0, 4, None, None, None), "D must"),
-    ("augur_pm_kdk", (None, None, None, None, 1.0, 100, 48, 1.0, 1.0, 0.1, 1, None, 0, None), "power of two"),
+    ("augur_pm_kdk", (None, None, None, None, None, 1.0, 100, 48, 1.0, 1.0, 0.1, 1, None, 0, None), "power of two"),
+    ("augur_pm_kdk", (None, None, None, None, None, 1.0, 100, 16, 1.0, 1.0, 0.1, 1, None, 0, None), "32, 512"),
+    ("augur_conv3_bf16_fwd", (None, 8, None, None, 1, 64, 0, 3, None, None, None, None, None), "bad arguments"),
+    ("augur_conv3_bf16_fwd", (None, 16, None, None, 1, 48, 0, 3, None, None, None, None, None), "power of two"),
     ("augur_poisson_green_grad", (None, 1, 1024, 1.0, 1.0, 1.0, None, None, None), "power-of-two"),
     ("augur_direct_sum_f64", (None, None, None, 10, 3, None, 4, 0.1, None, None, 0, None), "exactly one"),
 ])
