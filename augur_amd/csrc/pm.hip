@@ -127,7 +127,7 @@ __global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint
 // ---------------------------------------------------------- tile deposit
 constexpr int TD_X = 32, TD_Y = 32, TD_Z = 16;
 constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;  // 16384 int64 = 128 KB of LDS
-constexpr int TD_THREADS = 512;
+constexpr int TD_THREADS = 1024;
 constexpr int TD_MAXSEG = (TD_Y + 1 + 2 * DMAX) * (TD_Z + 1 + 2 * DMAX) * 2;
 constexpr int TD_UNROLL = 4;
 
@@ -219,22 +219,37 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(const float* __restric
   __syncthreads();
   const uint32_t total = seg_pre[nseg];
 
-  for (uint32_t w0 = tid; w0 < total; w0 += TD_THREADS * TD_UNROLL) {
+  // Each wave takes one contiguous slice of the tile's particle list and walks
+  // it 64 x TD_UNROLL particles at a time; a lane advances its segment index
+  // linearly (indices grow by at most 64 per sub-batch).  Runs of lanes with
+  // the same base cell (the list is cell-sorted) are summed in registers by a
+  // segmented scan over exact int64 values before the LDS atomics, which keeps
+  // dense cells from serialising the atomics.
+  const int lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = TD_THREADS / 64;
+  const uint32_t per = (total + NW - 1) / NW;
+  const uint32_t wlo = min(total, (uint32_t)wave * per), whi = min(total, wlo + per);
+  int seg = 0;
+  if (wlo < whi) {  // first segment of the slice: binary search, once per wave
+    int a = 0, b = nseg;
+    while (b - a > 1) {
+      const int m = (a + b) >> 1;
+      if (seg_pre[m] <= wlo) a = m; else b = m;
+    }
+    seg = a;
+  }
+  for (uint32_t base = wlo; base < whi; base += 64 * TD_UNROLL) {
     float px[TD_UNROLL][3], pm[TD_UNROLL];
     bool ok[TD_UNROLL];
 #pragma unroll
     for (int u = 0; u < TD_UNROLL; ++u) {
-      const uint32_t w = w0 + u * TD_THREADS;
-      ok[u] = w < total;
+      const uint32_t w = base + u * 64 + lane;
+      ok[u] = w < whi;
       pm[u] = mass_value;
       px[u][0] = px[u][1] = px[u][2] = 0.0f;
       if (ok[u]) {
-        int a = 0, b = nseg;  // seg_pre[a] <= w < seg_pre[b]
-        while (b - a > 1) {
-          const int m = (a + b) >> 1;
-          if (seg_pre[m] <= w) a = m; else b = m;
-        }
-        int64_t slot = (int64_t)seg_lo[a] + (w - seg_pre[a]);
+        while (seg_pre[seg + 1] <= w) ++seg;
+        const int64_t slot = (int64_t)seg_lo[seg] + (w - seg_pre[seg]);
         const int64_t p = perm ? (int64_t)perm[slot] : slot;
         px[u][0] = pos[3 * p];
         px[u][1] = pos[3 * p + 1];
@@ -242,11 +257,11 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(const float* __restric
         if (mass) pm[u] = mass[p];
       }
     }
+    seg = __shfl(seg, 63, 64);  // lanes resume from the furthest segment reached
 #pragma unroll
     for (int u = 0; u < TD_UNROLL; ++u) {
-      if (!ok[u]) continue;
-      int i0[3];
-      float wv[3];
+      int i0[3] = {0, 0, 0};
+      float wv[3] = {0.f, 0.f, 0.f};
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const float uu = cic_u(px[u][d], 0.0f, h);
@@ -266,16 +281,46 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(const float* __restric
         ly[o] = ay < TD_Y ? ay : -1;
         lz[o] = az < TD_Z ? az : -1;
       }
+      long long q[8];
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
         const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-        if (lx[ox] < 0 || ly[oy] < 0 || lz[oz] < 0) continue;
         const float wx = ox ? wv[0] : __fsub_rn(1.0f, wv[0]);
         const float wy = oy ? wv[1] : __fsub_rn(1.0f, wv[1]);
         const float wz = oz ? wv[2] : __fsub_rn(1.0f, wv[2]);
         const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
-        const long long q = (long long)rint((double)__fmul_rn(val, wt) * S);
-        if (q) atomicAdd(&acc[(lz[oz] * TD_Y + ly[oy]) * TD_X + lx[ox]], (unsigned long long)q);
+        q[o] = ok[u] ? (long long)rint((double)__fmul_rn(val, wt) * S) : 0ll;
+      }
+      // run detection on the wrapped base cell
+      const int key = ok[u] ? ((wrap(i0[2], G) * G + wrap(i0[1], G)) * G + wrap(i0[0], G)) : -1 - lane;
+      const int kprev = __shfl_up(key, 1, 64);
+      const bool head = lane == 0 || key != kprev;
+      const unsigned long long heads = __ballot(head);
+      bool tail = true;
+      if (heads != ~0ull) {  // some run longer than one lane: segmented scan
+        bool f = head;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          long long o8[8];
+#pragma unroll
+          for (int o = 0; o < 8; ++o) o8[o] = __shfl_up(q[o], off, 64);
+          const bool fo = __shfl_up((int)f, off, 64) != 0;
+          if (lane >= off && !f) {
+#pragma unroll
+            for (int o = 0; o < 8; ++o) q[o] += o8[o];
+          }
+          if (lane >= off) f = f || fo;
+        }
+        const bool next_head = __shfl_down((int)head, 1, 64) != 0;
+        tail = lane == 63 || next_head;
+      }
+      if (ok[u] && tail) {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+          if (lx[ox] < 0 || ly[oy] < 0 || lz[oz] < 0 || q[o] == 0) continue;
+          atomicAdd(&acc[(lz[oz] * TD_Y + ly[oy]) * TD_X + lx[ox]], (unsigned long long)q[o]);
+        }
       }
     }
   }
@@ -445,6 +490,20 @@ __global__ __launch_bounds__(PM_FFT_THREADS) void k_pm_xc2r(const float2* __rest
 }
 
 // ------------------------------------------------------------- gather + KDK
+// max of v over the block, then one atomicMax per block
+__device__ __forceinline__ void block_max_to(float v, uint32_t* target) {
+  __shared__ float red[16];
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmaxf(m, red[i]);
+    atomicMax(target, __float_as_uint(m));
+  }
+}
+
 __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, float* __restrict__ vel,
                                                     float* __restrict__ acc, const float4* __restrict__ force,
                                                     const uint32_t* __restrict__ perm, int64_t n, int G, float h,
@@ -502,10 +561,7 @@ __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, flo
       }
     }
   }
-  if (st && drift) {
-    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&st->vmax_bits, __float_as_uint(vmax));
-  }
+  if (st && drift) block_max_to(vmax, &st->vmax_bits);
 }
 
 __global__ void k_kick_drift(float* __restrict__ pos, float* __restrict__ vel, const float* __restrict__ acc, int64_t n,
@@ -523,8 +579,7 @@ __global__ void k_kick_drift(float* __restrict__ pos, float* __restrict__ vel, c
       vmax = fmaxf(vmax, fabsf(v));
     }
   }
-  for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(&st->vmax_bits, __float_as_uint(vmax));
+  block_max_to(vmax, &st->vmax_bits);
 }
 
 __global__ void k_pm_init(PmDev* st) {
@@ -586,14 +641,14 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   TimedScope ts_("pm_sort", s);
   const float h = (float)((double)L / (double)G);
   const int64_t G3 = G * G * G;
-  const dim3 g(blocks_for(n, 256)), bl(256);
+  const dim3 g(blocks_for(n, 256, 1024)), bl(256);
   hipLaunchKernelGGL(k_base_keys, g, bl, 0, s, pos, n, (int)G, h, b.keys, b.vals, gate);
   AUGUR_LAUNCHED("base_keys");
   uint32_t *ks, *perm;
   int rc = radix_sort_pairs(b.keys, b.vals, n, 3 * ilog2(G), b.rws, radix_workspace_bytes(n), &ks, &perm, s, gate);
   if (rc) return rc;
   // cell_start = exclusive scan of per-cell counts
-  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(G3 + 1, 256)), bl, 0, s, b.cell_start, G3 + 1, gate);
+  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(G3 + 1, 256, 1024)), bl, 0, s, b.cell_start, G3 + 1, gate);
   AUGUR_LAUNCHED("zero_cells");
   hipLaunchKernelGGL(k_count_cells, g, bl, 0, s, ks, n, b.cell_start, gate);
   AUGUR_LAUNCHED("count_cells");
@@ -609,7 +664,7 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
     if (!a) continue;
     hipLaunchKernelGGL((k_permute<float, 3>), g, bl, 0, s, a, b.tmp, perm, n, gate);
     AUGUR_LAUNCHED("permute3");
-    hipLaunchKernelGGL(k_copy<float>, dim3(blocks_for(3 * n, 256)), bl, 0, s, b.tmp, a, 3 * n, gate);
+    hipLaunchKernelGGL(k_copy<float>, dim3(blocks_for(3 * n, 256, 1024)), bl, 0, s, b.tmp, a, 3 * n, gate);
     AUGUR_LAUNCHED("copy3");
   }
   if (mass) {
@@ -753,7 +808,7 @@ int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_v
   if (rc) return rc;
   const float h = (float)((double)L / (double)G);
   TimedScope ts_("gather", s);
-  hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256)), dim3(256), 0, s, (float*)pos, (float*)nullptr, acc,
+  hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos, (float*)nullptr, acc,
                      b.force, b.perm, n, (int)G, h, 0.0f, 0.0f, L, 0, 0, (PmDev*)nullptr);
   AUGUR_LAUNCHED("gather");
   return AUGUR_OK;
@@ -791,7 +846,7 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, 
   const float hdt = 0.5f * dt;
   {
     TimedScope ts_("kick_drift", s);
-    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, vel, acc, n, hdt, dt, L, b.st);
+    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc, n, hdt, dt, L, b.st);
     AUGUR_LAUNCHED("kick_drift");
   }
   for (int st = 0; st < nsteps; ++st) {
@@ -806,7 +861,7 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, 
     if (rc) return rc;
     const int last = st == nsteps - 1;
     TimedScope ts_("gather_kdk", s);
-    hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, vel, acc, b.force,
+    hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc, b.force,
                        (const uint32_t*)nullptr, n, (int)G, h, hdt, dt, L, 1, last ? 0 : 1, b.st);
     AUGUR_LAUNCHED("gather_kdk");
   }
