@@ -22,21 +22,27 @@ struct RowMap {
   int64_t rows_per_batch;  // prod(shape[:-1])
   int64_t n_rows;          // batch * rows_per_batch
   int ny, nz;              // FFT outer extents (nz = 1 for 2D)
-  int in_ny, in_nz, in_nx; // input extents
+  int in_ny, in_nz, in_nx; // input (or cropped output) extents
+};
+
+template <int N>
+struct RowCfg {
+  static constexpr int T = FFT_THREADS;
+  static constexpr int NL = FftLines<N, T>::value;
+  static constexpr int LS = FftStride<N>::value;
 };
 
 template <int N>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows_r2c(const float* __restrict__ in, float2* __restrict__ out,
                                                               RowMap m) {
-  constexpr int L = FFT_TILE / N;
-  constexpr int LS = N + 1;
-  __shared__ float2 bufa[L * LS], bufb[L * LS];
+  using C = RowCfg<N>;
+  __shared__ float2 lds[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
-  const int64_t row0 = (int64_t)blockIdx.x * L;
-  const int nxc = N / 2 + 1;
-  for (int e = threadIdx.x; e < L * N; e += FFT_THREADS) {
-    const int l = e / N, x = e - l * N;
+  __syncthreads();
+  const int64_t row0 = (int64_t)blockIdx.x * C::NL;
+  constexpr int nxc = N / 2 + 1;
+  auto src = [&](int l, int x) -> float2 {
     const int64_t row = row0 + l;
     float v = 0.0f;
     if (row < m.n_rows && x < m.in_nx) {
@@ -45,15 +51,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows_r2c(const float* __res
       const int y = (int)(r % m.ny), z = (int)(r / m.ny);
       if (y < m.in_ny && z < m.in_nz) v = in[((b * m.in_nz + z) * m.in_ny + y) * (int64_t)m.in_nx + x];
     }
-    bufa[l * LS + x] = make_float2(v, 0.0f);
-  }
-  __syncthreads();
-  float2* r = lds_fft<N, false>(bufa, bufb, tw, L, LS);
-  for (int e = threadIdx.x; e < L * nxc; e += FFT_THREADS) {
-    const int l = e / nxc, k = e - l * nxc;
+    return make_float2(v, 0.0f);
+  };
+  auto dst = [&](int l, int k, float2 v) {
     const int64_t row = row0 + l;
-    if (row < m.n_rows) out[row * nxc + k] = r[l * LS + k];
-  }
+    if (row < m.n_rows && k < nxc) out[row * nxc + k] = v;
+  };
+  fft_lines<N, false>(lds, tw, C::NL, src, dst);
 }
 
 // ---------------------------------------------------------------- rows C2R
@@ -61,88 +65,81 @@ template <int N>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows_c2r(const float2* __restrict__ in, float* __restrict__ out,
                                                               RowMap m, float scale) {
   // m.in_* here describe the cropped OUTPUT extents.
-  constexpr int L = FFT_TILE / N;
-  constexpr int LS = N + 1;
-  __shared__ float2 bufa[L * LS], bufb[L * LS];
+  using C = RowCfg<N>;
+  __shared__ float2 lds[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
-  const int64_t row0 = (int64_t)blockIdx.x * L;
-  const int nxc = N / 2 + 1;
-  for (int e = threadIdx.x; e < L * N; e += FFT_THREADS) {
-    const int l = e / N, k = e - l * N;
+  __syncthreads();
+  const int64_t row0 = (int64_t)blockIdx.x * C::NL;
+  constexpr int nxc = N / 2 + 1;
+  auto src = [&](int l, int k) -> float2 {
     const int64_t row = row0 + l;
     float2 v = make_float2(0.f, 0.f);
     if (row < m.n_rows) {
-      if (k < nxc) {
-        v = in[row * nxc + k];
-      } else {
-        v = in[row * nxc + (N - k)];
-        v.y = -v.y;
-      }
-      // C2R ignores the imaginary parts of the DC and Nyquist bins
-      if (k == 0 || k == N / 2) v.y = 0.0f;
+      v = in[row * nxc + (k < nxc ? k : N - k)];
+      if (k >= nxc) v.y = -v.y;
+      if (k == 0 || k == N / 2) v.y = 0.0f;  // C2R ignores these imaginary parts
     }
-    bufa[l * LS + k] = v;
-  }
-  __syncthreads();
-  float2* r = lds_fft<N, true>(bufa, bufb, tw, L, LS);
-  for (int e = threadIdx.x; e < L * N; e += FFT_THREADS) {
-    const int l = e / N, x = e - l * N;
+    return v;
+  };
+  auto dst = [&](int l, int x, float2 v) {
     const int64_t row = row0 + l;
-    if (row >= m.n_rows || x >= m.in_nx) continue;
+    if (row >= m.n_rows || x >= m.in_nx) return;
     const int64_t b = row / m.rows_per_batch;
     const int64_t rr = row - b * m.rows_per_batch;
     const int y = (int)(rr % m.ny), z = (int)(rr / m.ny);
-    if (y < m.in_ny && z < m.in_nz) out[((b * m.in_nz + z) * m.in_ny + y) * (int64_t)m.in_nx + x] = r[l * LS + x].x * scale;
-  }
+    if (y < m.in_ny && z < m.in_nz) out[((b * m.in_nz + z) * m.in_ny + y) * (int64_t)m.in_nx + x] = v.x * scale;
+  };
+  fft_lines<N, true>(lds, tw, C::NL, src, dst);
 }
 
 // ------------------------------------------------------ strided columns C2C
 // Lines along a strided axis: for outer index o in [0, n_outer) the line base
 // is (o / q) * s1 + (o % q) * s2, element stride es; columns c in [0, ncols)
-// are adjacent (stride 1).  In-place.
+// are adjacent (stride 1) and one workgroup takes NL of them.  In-place.
 struct ColMap {
   int64_t n_outer, q, s1, s2, es;
   int ncols;
 };
 
+template <int N>
+struct ColCfg {
+  static constexpr int T = N >= 512 ? 1024 : FFT_THREADS;  // keep >= 16 adjacent columns per workgroup
+  static constexpr int NL = FftLines<N, T>::value > 64 ? 64 : FftLines<N, T>::value;
+  static constexpr int LS = FftStride<N>::value;
+};
+
 template <int N, bool INV>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(float2* __restrict__ data, ColMap m) {
-  constexpr int W0 = FFT_TILE / N;
-  constexpr int W = W0 > 64 ? 64 : W0;
-  constexpr int LS = N + 1;
-  __shared__ float2 bufa[W * LS], bufb[W * LS];
+__global__ __launch_bounds__(ColCfg<N>::T) void k_fft_cols(float2* __restrict__ data, ColMap m) {
+  using C = ColCfg<N>;
+  __shared__ float2 lds[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
-  const int nblk = (m.ncols + W - 1) / W;
-  const int64_t o = blockIdx.x / nblk;
-  const int c0 = (int)(blockIdx.x - o * nblk) * W;
-  const int64_t base = (o / m.q) * m.s1 + (o % m.q) * m.s2 + c0;
-  for (int e = threadIdx.x; e < W * N; e += FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    float2 v = make_float2(0.f, 0.f);
-    if (c0 + c < m.ncols) v = data[base + k * m.es + c];
-    bufa[c * LS + k] = v;
-  }
   __syncthreads();
-  float2* r = lds_fft<N, INV>(bufa, bufb, tw, W, LS);
-  for (int e = threadIdx.x; e < W * N; e += FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    if (c0 + c < m.ncols) data[base + k * m.es + c] = r[c * LS + k];
-  }
+  const int nblk = (m.ncols + C::NL - 1) / C::NL;
+  const int64_t o = blockIdx.x / nblk;
+  const int c0 = (int)(blockIdx.x - o * nblk) * C::NL;
+  const int64_t base = (o / m.q) * m.s1 + (o % m.q) * m.s2 + c0;
+  auto src = [&](int c, int k) -> float2 {
+    return (c0 + c < m.ncols) ? data[base + k * m.es + c] : make_float2(0.f, 0.f);
+  };
+  auto dst = [&](int c, int k, float2 v) {
+    if (c0 + c < m.ncols) data[base + k * m.es + c] = v;
+  };
+  fft_lines<N, INV, true>(lds, tw, C::NL, src, dst);
 }
 
 template <bool INV>
 int launch_cols(int n, float2* data, const ColMap& m, hipStream_t s) {
   TimedScope ts_(INV ? "fft_cols_inv" : "fft_cols_fwd", s);
-  const int W0 = FFT_TILE / n;
-  const int W = W0 > 64 ? 64 : W0;
-  const int64_t blocks = m.n_outer * ((m.ncols + W - 1) / W);
-  AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "fft: too many lines");
-  dim3 g((unsigned)blocks), b(FFT_THREADS);
   switch (n) {
-#define CASE(NN) \
-  case NN: hipLaunchKernelGGL((k_fft_cols<NN, INV>), g, b, 0, s, data, m); break;
+#define CASE(NN)                                                                                  \
+  case NN: {                                                                                      \
+    const int64_t blocks = m.n_outer * ((m.ncols + ColCfg<NN>::NL - 1) / ColCfg<NN>::NL);         \
+    AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "fft: too many lines");              \
+    hipLaunchKernelGGL((k_fft_cols<NN, INV>), dim3((unsigned)blocks), dim3(ColCfg<NN>::T), 0, s, data, m); \
+    break;                                                                                        \
+  }
     CASE(4) CASE(8) CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512)
 #undef CASE
     default: set_error("fft: unsupported length %d", n); return AUGUR_EUNSUPPORTED;
@@ -153,11 +150,12 @@ int launch_cols(int n, float2* data, const ColMap& m, hipStream_t s) {
 
 int launch_rows_r2c(int n, const float* in, float2* out, const RowMap& m, hipStream_t s) {
   TimedScope ts_("fft_rows_r2c", s);
-  const int L = FFT_TILE / n;
-  dim3 g((unsigned)((m.n_rows + L - 1) / L)), b(FFT_THREADS);
   switch (n) {
-#define CASE(NN) \
-  case NN: hipLaunchKernelGGL((k_fft_rows_r2c<NN>), g, b, 0, s, in, out, m); break;
+#define CASE(NN)                                                                                         \
+  case NN:                                                                                               \
+    hipLaunchKernelGGL((k_fft_rows_r2c<NN>), dim3((unsigned)((m.n_rows + RowCfg<NN>::NL - 1) / RowCfg<NN>::NL)), \
+                       dim3(FFT_THREADS), 0, s, in, out, m);                                             \
+    break;
     CASE(4) CASE(8) CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512)
 #undef CASE
     default: set_error("fft: unsupported length %d", n); return AUGUR_EUNSUPPORTED;
@@ -168,11 +166,12 @@ int launch_rows_r2c(int n, const float* in, float2* out, const RowMap& m, hipStr
 
 int launch_rows_c2r(int n, const float2* in, float* out, const RowMap& m, float scale, hipStream_t s) {
   TimedScope ts_("fft_rows_c2r", s);
-  const int L = FFT_TILE / n;
-  dim3 g((unsigned)((m.n_rows + L - 1) / L)), b(FFT_THREADS);
   switch (n) {
-#define CASE(NN) \
-  case NN: hipLaunchKernelGGL((k_fft_rows_c2r<NN>), g, b, 0, s, in, out, m, scale); break;
+#define CASE(NN)                                                                                         \
+  case NN:                                                                                               \
+    hipLaunchKernelGGL((k_fft_rows_c2r<NN>), dim3((unsigned)((m.n_rows + RowCfg<NN>::NL - 1) / RowCfg<NN>::NL)), \
+                       dim3(FFT_THREADS), 0, s, in, out, m, scale);                                      \
+    break;
     CASE(4) CASE(8) CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512)
 #undef CASE
     default: set_error("fft: unsupported length %d", n); return AUGUR_EUNSUPPORTED;

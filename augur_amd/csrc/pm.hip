@@ -125,9 +125,9 @@ __global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint
 }
 
 // ---------------------------------------------------------- tile deposit
-constexpr int TD_X = 32, TD_Y = 32, TD_Z = 16;
-constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;  // 16384 int64 = 128 KB of LDS
-constexpr int TD_THREADS = 1024;
+constexpr int TD_X = 32, TD_Y = 16, TD_Z = 16;
+constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;  // 8192 int64 = 64 KB of LDS: two workgroups per CU
+constexpr int TD_THREADS = 512;
 constexpr int TD_MAXSEG = (TD_Y + 1 + 2 * DMAX) * (TD_Z + 1 + 2 * DMAX) * 2;
 constexpr int TD_UNROLL = 4;
 
@@ -334,159 +334,147 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(const float* __restric
 
 // ------------------------------------------------------ fused FFT passes
 // Spectral layout after the x R2C pass: (z, y|ky, kx) with nxc = G/2 + 1.
-constexpr int PM_FFT_THREADS = 256;
-
+// Column tiles take NL adjacent kx columns (line-fastest thread mapping).
 template <int N>
-struct ColTile {
-  static constexpr int W = (4096 / N) > 16 ? 16 : (4096 / N);
-  static constexpr int LS = N + 1;
+struct PmCol {
+  static constexpr int T = N >= 512 ? 1024 : FFT_THREADS;
+  static constexpr int NL = FftLines<N, T>::value > 16 ? 16 : FftLines<N, T>::value;
+  static constexpr int LS = FftStride<N>::value;
+  static constexpr int nxc = N / 2 + 1;
+  static constexpr int nblk = (nxc + NL - 1) / NL;
 };
 
 // z columns at fixed ky: forward z FFT, Green's function g = -4 pi G / k^2 / G^3,
-// outputs A = IFFTz(g rho) and B = IFFTz(-i kz g rho) (Nyquist kz zeroed).
-// In place: A overwrites the input; B goes to outB.
+// outputs A = IFFTz(g rho) (in place of the input) and B = IFFTz(-i kz g rho)
+// (to outB; Nyquist kz zeroed).
 template <int N>
-__global__ __launch_bounds__(PM_FFT_THREADS) void k_pm_zpass(float2* __restrict__ spec, float2* __restrict__ outB,
-                                                             float kf, float green) {
-  constexpr int W = ColTile<N>::W, LS = ColTile<N>::LS, nxc = N / 2 + 1;
-  __shared__ float2 b0[W * LS], b1[W * LS], b2[W * LS];
+__global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ spec, float2* __restrict__ outB,
+                                                         float kf, float green) {
+  using C = PmCol<N>;
+  __shared__ float2 A[C::NL * C::LS], B[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
-  const int nblk = (nxc + W - 1) / W;
-  const int ky = blockIdx.x / nblk;
-  const int c0 = (blockIdx.x % nblk) * W;
-  const int64_t base = (int64_t)ky * nxc + c0, es = (int64_t)N * nxc;
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    b0[c * LS + k] = (c0 + c < nxc) ? spec[base + k * es + c] : make_float2(0.f, 0.f);
-  }
   __syncthreads();
-  float2* r = lds_fft<N, false>(b0, b1, tw, W, LS);  // result in b1 (odd #stages) or b0
-  float2* fa = r;
-  float2* fb = (r == b0) ? b1 : b0;
+  const int ky = blockIdx.x / C::nblk;
+  const int c0 = (blockIdx.x % C::nblk) * C::NL;
+  const int64_t base = (int64_t)ky * C::nxc + c0, es = (int64_t)N * C::nxc;
   const int ny = ky <= N / 2 ? ky : ky - N;
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e / N, kz = e % N;
+  auto gsrc = [&](int c, int k) -> float2 {
+    return (c0 + c < C::nxc) ? spec[base + k * es + c] : make_float2(0.f, 0.f);
+  };
+  // forward transform straight into B, scaled by the Green's function
+  auto green_dst = [&](int c, int kz, float2 v) {
     const int kx = c0 + c;
     const int nz = kz <= N / 2 ? kz : kz - N;
     const float fx = kf * kx, fy = kf * ny, fz = kf * nz;
     const float k2 = fx * fx + fy * fy + fz * fz;
-    const float2 v = fa[c * LS + kz];
     const float g = (kx == 0 && ky == 0 && kz == 0) ? 0.0f : green / k2;
-    const float2 p = make_float2(v.x * g, v.y * g);
-    fa[c * LS + kz] = p;
-    const float az = (kz == N / 2) ? 0.0f : fz;
-    fb[c * LS + kz] = make_float2(az * p.y, -az * p.x);  // -i kz p
+    B[c * C::LS + fft_pad(kz)] = make_float2(v.x * g, v.y * g);
+  };
+  fft_lines<N, false, true>(A, tw, C::NL, gsrc, green_dst);
+  // A = p, B = -i kz p
+  for (int e = threadIdx.x; e < C::NL * N; e += C::T) {
+    const int c = e % C::NL, kz = e / C::NL;
+    const int idx = c * C::LS + fft_pad(kz);
+    const float2 p = B[idx];
+    const int nz = kz <= N / 2 ? kz : kz - N;
+    const float az = (kz == N / 2) ? 0.0f : kf * nz;
+    A[idx] = p;
+    B[idx] = make_float2(az * p.y, -az * p.x);
   }
   __syncthreads();
-  float2* scratch = b2;
-  float2* ra = lds_fft<N, true>(fa, scratch, tw, W, LS);
-  float2* freeb = (ra == fa) ? scratch : fa;
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    if (c0 + c < nxc) spec[base + k * es + c] = ra[c * LS + k];
-  }
-  __syncthreads();
-  float2* rb = lds_fft<N, true>(fb, freeb, tw, W, LS);
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    if (c0 + c < nxc) outB[base + k * es + c] = rb[c * LS + k];
-  }
+  auto asrc = [&](int c, int k) { return A[c * C::LS + fft_pad(k)]; };
+  auto bsrc = [&](int c, int k) { return B[c * C::LS + fft_pad(k)]; };
+  auto adst = [&](int c, int k, float2 v) {
+    if (c0 + c < C::nxc) spec[base + k * es + c] = v;
+  };
+  auto bdst = [&](int c, int k, float2 v) {
+    if (c0 + c < C::nxc) outB[base + k * es + c] = v;
+  };
+  fft_lines<N, true, true, true>(A, tw, C::NL, asrc, adst);
+  fft_lines<N, true, true, true>(B, tw, C::NL, bsrc, bdst);
 }
 
-// y columns at fixed z: reads A (in0) and B (in2); writes
-//   out0 = IFFTy(A)            (in place of A; the x pass applies -i kx)
-//   out1 = IFFTy(-i ky A)      (Nyquist ky zeroed)
-//   out2 = IFFTy(B)            (in place of B)
+// y columns at fixed z: reads A (s0) and B (s2); writes
+//   s1 = IFFTy(-i ky A)      (Nyquist ky zeroed)
+//   s0 = IFFTy(A)            (in place; the x pass applies -i kx)
+//   s2 = IFFTy(B)            (in place)
 template <int N>
-__global__ __launch_bounds__(PM_FFT_THREADS) void k_pm_yinv(float2* __restrict__ s0, float2* __restrict__ s1,
-                                                            float2* __restrict__ s2, float kf) {
-  constexpr int W = ColTile<N>::W, LS = ColTile<N>::LS, nxc = N / 2 + 1;
-  __shared__ float2 b0[W * LS], b1[W * LS], b2[W * LS];
+__global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0, float2* __restrict__ s1,
+                                                        float2* __restrict__ s2, float kf) {
+  using C = PmCol<N>;
+  __shared__ float2 X[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
-  const int nblk = (nxc + W - 1) / W;
-  const int z = blockIdx.x / nblk;
-  const int c0 = (blockIdx.x % nblk) * W;
-  const int64_t base = (int64_t)z * N * nxc + c0, es = nxc;
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    const float2 v = (c0 + c < nxc) ? s0[base + k * es + c] : make_float2(0.f, 0.f);
-    b0[c * LS + k] = v;
-  }
   __syncthreads();
-  // b1 = -i ky A with ky = kf * signed index, Nyquist zeroed
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e / N, ky = e % N;
-    const float2 v = b0[c * LS + ky];
+  const int z = blockIdx.x / C::nblk;
+  const int c0 = (blockIdx.x % C::nblk) * C::NL;
+  const int64_t base = (int64_t)z * N * C::nxc + c0, es = C::nxc;
+  auto ld = [&](const float2* p, int c, int k) -> float2 {
+    return (c0 + c < C::nxc) ? p[base + k * es + c] : make_float2(0.f, 0.f);
+  };
+  auto st = [&](float2* p, int c, int k, float2 v) {
+    if (c0 + c < C::nxc) p[base + k * es + c] = v;
+  };
+  auto src_ky = [&](int c, int ky) -> float2 {
+    const float2 v = ld(s0, c, ky);
     const float f = (ky == N / 2) ? 0.0f : kf * (float)(ky <= N / 2 ? ky : ky - N);
-    b1[c * LS + ky] = make_float2(f * v.y, -f * v.x);
-  }
-  __syncthreads();
-  float2* r0 = lds_fft<N, true>(b0, b2, tw, W, LS);
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    if (c0 + c < nxc) s0[base + k * es + c] = r0[c * LS + k];
-  }
-  __syncthreads();
-  float2* free0 = (r0 == b0) ? b2 : b0;
-  float2* r1 = lds_fft<N, true>(b1, free0, tw, W, LS);
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    if (c0 + c < nxc) s1[base + k * es + c] = r1[c * LS + k];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    b0[c * LS + k] = (c0 + c < nxc) ? s2[base + k * es + c] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  float2* r2 = lds_fft<N, true>(b0, b2, tw, W, LS);
-  for (int e = threadIdx.x; e < W * N; e += PM_FFT_THREADS) {
-    const int c = e % W, k = e / W;
-    if (c0 + c < nxc) s2[base + k * es + c] = r2[c * LS + k];
-  }
+    return make_float2(f * v.y, -f * v.x);
+  };
+  fft_lines<N, true, true>(X, tw, C::NL, src_ky, [&](int c, int k, float2 v) { st(s1, c, k, v); });
+  fft_lines<N, true, true>(X, tw, C::NL, [&](int c, int k) { return ld(s0, c, k); },
+                           [&](int c, int k, float2 v) { st(s0, c, k, v); });
+  fft_lines<N, true, true>(X, tw, C::NL, [&](int c, int k) { return ld(s2, c, k); },
+                           [&](int c, int k, float2 v) { st(s2, c, k, v); });
 }
 
 // x rows: three half spectra (A~, -i ky A~, B~) -> real rows; the first gets
-// -i kx (Nyquist kx zeroed); ky's and kx's 2 pi / L factor is `kf`.
-// Output: float4 grid (a_x, a_y, a_z, 0).
+// -i kx (Nyquist kx zeroed).  Output: float4 grid (a_x, a_y, a_z, 0).
 template <int N>
-__global__ __launch_bounds__(PM_FFT_THREADS) void k_pm_xc2r(const float2* __restrict__ s0,
-                                                            const float2* __restrict__ s1,
-                                                            const float2* __restrict__ s2, float kf,
-                                                            float4* __restrict__ force) {
-  constexpr int L = (2048 / N) > 1 ? 2048 / N : 1;
-  constexpr int LS = N + 1, nxc = N / 2 + 1;
-  __shared__ float2 b0[L * LS], b1[L * LS];
-  __shared__ float res[3][L * N];
+struct PmRow {
+  static constexpr int NL = FftLines<N, FFT_THREADS>::value;
+  static constexpr int LS = FftStride<N>::value;
+  static constexpr int nxc = N / 2 + 1;
+};
+
+template <int N>
+__global__ __launch_bounds__(FFT_THREADS) void k_pm_xc2r(const float2* __restrict__ s0,
+                                                         const float2* __restrict__ s1,
+                                                         const float2* __restrict__ s2, float kf,
+                                                         float4* __restrict__ force) {
+  using C = PmRow<N>;
+  __shared__ float2 X[C::NL * C::LS];
+  __shared__ float res[2][C::NL * N];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
-  const int64_t row0 = (int64_t)blockIdx.x * L;
-  const float2* src[3] = {s0, s1, s2};
-#pragma unroll 1
-  for (int f = 0; f < 3; ++f) {
-    for (int e = threadIdx.x; e < L * N; e += PM_FFT_THREADS) {
-      const int l = e / N, k = e % N;
-      const int64_t row = row0 + l;
-      const int kk = k < nxc ? k : N - k;
-      float2 v = src[f][row * nxc + kk];
-      if (k >= nxc) v.y = -v.y;
-      if (f == 0) {
-        // -i kx v with signed kx; zero at Nyquist
-        const float fxk = (k == N / 2) ? 0.0f : kf * (float)(k < nxc ? k : k - N);
-        v = make_float2(fxk * v.y, -fxk * v.x);
-      }
-      if (k == 0 || k == N / 2) v.y = 0.0f;
-      b0[l * LS + k] = v;
-    }
-    __syncthreads();
-    float2* r = lds_fft<N, true>(b0, b1, tw, L, LS);
-    for (int e = threadIdx.x; e < L * N; e += PM_FFT_THREADS) res[f][e] = r[(e / N) * LS + (e % N)].x;
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < L * N; e += PM_FFT_THREADS)
-    force[row0 * N + e] = make_float4(res[0][e], res[1][e], res[2][e], 0.0f);
+  __syncthreads();
+  const int64_t row0 = (int64_t)blockIdx.x * C::NL;
+  auto herm = [&](const float2* p, int l, int k) -> float2 {
+    float2 v = p[(row0 + l) * C::nxc + (k < C::nxc ? k : N - k)];
+    if (k >= C::nxc) v.y = -v.y;
+    return v;
+  };
+  // -i kx v is zero at k = 0 and N/2, where C2R would drop imaginary parts
+  auto src0_full = [&](int l, int k) -> float2 {
+    const float2 v = herm(s0, l, k);
+    const float f = (k == N / 2) ? 0.0f : kf * (float)(k < C::nxc ? k : k - N);
+    return make_float2(f * v.y, -f * v.x);
+  };
+  auto src1 = [&](int l, int k) -> float2 {
+    float2 v = herm(s1, l, k);
+    if (k == 0 || k == N / 2) v.y = 0.0f;
+    return v;
+  };
+  auto src2 = [&](int l, int k) -> float2 {
+    float2 v = herm(s2, l, k);
+    if (k == 0 || k == N / 2) v.y = 0.0f;
+    return v;
+  };
+  fft_lines<N, true>(X, tw, C::NL, src0_full, [&](int l, int x, float2 v) { res[0][l * N + x] = v.x; });
+  fft_lines<N, true>(X, tw, C::NL, src1, [&](int l, int x, float2 v) { res[1][l * N + x] = v.x; });
+  fft_lines<N, true>(X, tw, C::NL, src2, [&](int l, int x, float2 v) {
+    force[(row0 + l) * N + x] = make_float4(res[0][l * N + x], res[1][l * N + x], v.x, 0.0f);
+  });
 }
 
 // ------------------------------------------------------------- gather + KDK
@@ -682,30 +670,16 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   return AUGUR_OK;
 }
 
-#define PM_SWITCH_G(G, KERNEL, grid, block, s, ...)                                     \
-  switch (G) {                                                                            \
-    case 32: hipLaunchKernelGGL(KERNEL<32>, grid, block, 0, s, __VA_ARGS__); break;       \
-    case 64: hipLaunchKernelGGL(KERNEL<64>, grid, block, 0, s, __VA_ARGS__); break;       \
-    case 128: hipLaunchKernelGGL(KERNEL<128>, grid, block, 0, s, __VA_ARGS__); break;     \
-    case 256: hipLaunchKernelGGL(KERNEL<256>, grid, block, 0, s, __VA_ARGS__); break;     \
-    case 512: hipLaunchKernelGGL(KERNEL<512>, grid, block, 0, s, __VA_ARGS__); break;     \
-    default: set_error("pm: unsupported G %d", (int)(G)); return AUGUR_EUNSUPPORTED;    \
+// launches KERNEL<GG> with GG = G as a compile-time constant usable in grid/block
+#define PM_SWITCH_G(G, KERNEL, grid, block, s, ...)                                                   \
+  switch (G) {                                                                                          \
+    case 32: { constexpr int GG = 32; hipLaunchKernelGGL(KERNEL<GG>, grid, block, 0, s, __VA_ARGS__); break; }   \
+    case 64: { constexpr int GG = 64; hipLaunchKernelGGL(KERNEL<GG>, grid, block, 0, s, __VA_ARGS__); break; }   \
+    case 128: { constexpr int GG = 128; hipLaunchKernelGGL(KERNEL<GG>, grid, block, 0, s, __VA_ARGS__); break; } \
+    case 256: { constexpr int GG = 256; hipLaunchKernelGGL(KERNEL<GG>, grid, block, 0, s, __VA_ARGS__); break; } \
+    case 512: { constexpr int GG = 512; hipLaunchKernelGGL(KERNEL<GG>, grid, block, 0, s, __VA_ARGS__); break; } \
+    default: set_error("pm: unsupported G %d", (int)(G)); return AUGUR_EUNSUPPORTED;                  \
   }
-
-template <int N>
-static int ncol_blocks() {
-  return (N / 2 + 1 + ColTile<N>::W - 1) / ColTile<N>::W;
-}
-
-static int col_blocks(int G) {
-  switch (G) {
-    case 32: return ncol_blocks<32>();
-    case 64: return ncol_blocks<64>();
-    case 128: return ncol_blocks<128>();
-    case 256: return ncol_blocks<256>();
-    default: return ncol_blocks<512>();
-  }
-}
 
 // deposit of sorted slots (optional perm) -> b.rho
 static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
@@ -743,22 +717,20 @@ static int pm_force_grid(const float* pos, const uint32_t* perm, const float* ma
   const double PI = 3.14159265358979323846;
   const float kf = (float)(2.0 * PI / (double)L);
   const float green = (float)(-4.0 * PI * (double)Gconst / ((double)G * G * G));
-  const dim3 cg((unsigned)(G * col_blocks((int)G))), fb(PM_FFT_THREADS);
   {
     TimedScope ts_("pm_zpass", s);
-    PM_SWITCH_G(G, k_pm_zpass, cg, fb, s, b.s0, b.s2, kf, green);
+    PM_SWITCH_G(G, k_pm_zpass, dim3((unsigned)(G * PmCol<GG>::nblk)), dim3(PmCol<GG>::T), s, b.s0, b.s2, kf, green);
     AUGUR_LAUNCHED("pm_zpass");
   }
   {
     TimedScope ts_("pm_yinv", s);
-    PM_SWITCH_G(G, k_pm_yinv, cg, fb, s, b.s0, b.s1, b.s2, kf);
+    PM_SWITCH_G(G, k_pm_yinv, dim3((unsigned)(G * PmCol<GG>::nblk)), dim3(PmCol<GG>::T), s, b.s0, b.s1, b.s2, kf);
     AUGUR_LAUNCHED("pm_yinv");
   }
   {
-    const int L_rows = (int)((2048 / G) > 1 ? 2048 / G : 1);
     TimedScope ts_("pm_xc2r", s);
-    PM_SWITCH_G(G, k_pm_xc2r, dim3((unsigned)(G * G / L_rows)), fb, s, (const float2*)b.s0, (const float2*)b.s1,
-                (const float2*)b.s2, kf, b.force);
+    PM_SWITCH_G(G, k_pm_xc2r, dim3((unsigned)(G * G / PmRow<GG>::NL)), dim3(FFT_THREADS), s, (const float2*)b.s0,
+                (const float2*)b.s1, (const float2*)b.s2, kf, b.force);
     AUGUR_LAUNCHED("pm_xc2r");
   }
   return AUGUR_OK;
