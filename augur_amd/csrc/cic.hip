@@ -304,17 +304,6 @@ __global__ void k_cic_base(const float* __restrict__ pos, int64_t n, int D, floa
 }
 
 // --------------------------------------------------------------- deposit
-// Fixed-point exponent: largest e with n * max|v| * 2^e <= 2^62.
-__device__ __forceinline__ double fixed_scale(unsigned maxbits, int64_t n) {
-  const float m = __uint_as_float(maxbits);
-  if (!(m > 0.0f)) return 1.0;
-  const double bound = (double)m * (double)(n > 0 ? n : 1);
-  int e = (int)floor(62.0 - log2(bound));
-  if (e > 1000) e = 1000;
-  if (e < -1000) e = -1000;
-  return ldexp(1.0, e);
-}
-
 __device__ __forceinline__ float chan_value(int c, float mm, const float* vel, int64_t i, int D, float scale) {
   // reference: stack([mm, mm*v0, mm*v1]) * (in_scale / h^D)  (scatter_field.py:241)
   const float base = (c == 0) ? mm : __fmul_rn(mm, vel[i * D + (c - 1)]);
@@ -345,8 +334,8 @@ __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos
                                                    const unsigned* __restrict__ maxbits,
                                                    unsigned long long* __restrict__ acc) {
   const int64_t tot = B * N;
-  double S[4];
-  for (int c = 0; c < C; ++c) S[c] = fixed_scale(maxbits[c], tot);
+  int E[4];
+  for (int c = 0; c < C; ++c) E[c] = fixed_exponent(maxbits[c], tot);
   int64_t GD = 1;
   for (int d = 0; d < D; ++d) GD *= G;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
@@ -386,7 +375,7 @@ __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos
       if (!ok) continue;
       for (int c = 0; c < C; ++c) {
         const float contrib = __fmul_rn(val[c], wt);
-        const long long q = (long long)rint((double)contrib * S[c]);
+        const long long q = fixed_round(contrib, E[c]);
         if (q) atomicAdd(&acc[((b * C) + c) * GD + flat], (unsigned long long)q);
       }
     }
@@ -398,8 +387,7 @@ __global__ void k_dep_convert(const unsigned long long* __restrict__ acc, float*
   const int64_t n = B * C * GD;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int c = (int)((i / GD) % C);
-    const double S = fixed_scale(maxbits[c], tot_particles);
-    out[i] = (float)((double)(long long)acc[i] / S);
+    out[i] = (float)ldexp((double)(long long)acc[i], -fixed_exponent(maxbits[c], tot_particles));
   }
 }
 

@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <cmath>
 
 #include "../../include/augur.h"
 
@@ -89,6 +90,39 @@ __device__ __forceinline__ float cic_u(float p, float off, float h) {
   float t = __fadd_rn(p, off);
   t = __fdiv_rn(t, h);
   return __fsub_rn(t, 0.5f);
+}
+
+// The same value when h is a power of two and inv_h = 1/h exactly: both
+// operations then round the same real number p/h once, and p + 0 only turns
+// -0 into +0, which the subtraction of 0.5 maps to the same result.
+__device__ __forceinline__ float cic_u_pow2(float p, float inv_h) { return __fsub_rn(__fmul_rn(p, inv_h), 0.5f); }
+
+// Fixed-point exponent of the order-independent deposits: the largest e with
+// n * max|v| * 2^e <= 2^62, so that any sum of n contributions fits in int64.
+__device__ __forceinline__ int fixed_exponent(unsigned maxbits, int64_t n) {
+  const float m = __uint_as_float(maxbits);
+  if (!(m > 0.0f)) return 0;
+  int e = (int)floor(62.0 - log2((double)m * (double)(n > 0 ? n : 1)));
+  return e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
+}
+
+// rint(v * 2^e) as a 64-bit integer, exact whenever |v * 2^e| < 2^62 (which
+// fixed_exponent guarantees for every contribution).  All FP32: the scaling by
+// 2^e is exact, rintf rounds half to even, and the magnitude splits exactly
+// into two 32-bit halves (a - hi * 2^32 is a multiple of ulp(a) below 2^32).
+__device__ __forceinline__ long long fixed_round(float v, int e) {
+  const float r = rintf(ldexpf(v, e));
+  const float a = fabsf(r);
+  const float hi = floorf(a * 2.3283064365386963e-10f);
+  const float lo = fmaf(hi, -4294967296.0f, a);
+  const unsigned long long q = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+  return r < 0.0f ? -(long long)q : (long long)q;
+}
+
+// h = L / G in FP32 is a power of two (then cic_u_pow2 applies)
+inline bool pow2_float(float h) {
+  int ex;
+  return h > 0.0f && std::frexp(h, &ex) == 0.5f;
 }
 
 }  // namespace augur_dev

@@ -3,12 +3,11 @@
 // State: particles physically sorted by CIC base cell (key = (z*G + y)*G + x
 // of floor((pos/h) - 0.5) wrapped), with cell_start[] (G^3 + 1 offsets) from
 // the last sort.  One step:
-//   1. deposit   k_dep_tiles: one workgroup per 32x32x16 tile of the grid
-//                reads the particles whose base cell (as of the last sort)
-//                lies in the tile's halo-extended region grown by the drift
-//                bound D, and accumulates exact fixed-point CIC contributions
-//                with 64-bit LDS integer atomics; every grid cell is written
-//                once (no global atomics, bitwise order-independent).
+//   1. deposit   k_dep_plan counts each 32x16x16 tile's particles (base cell,
+//                as of the last sort, in the tile's region grown by the drift
+//                bound D) and cuts them into work items; k_dep_tiles sums
+//                exact fixed-point CIC contributions per item in LDS with
+//                64-bit integer atomics (bitwise order-independent).
 //   2. FFT       x-rows R2C -> y-columns -> z-columns fused with the Green's
 //                function and the inverse z transform (two outputs) ->
 //                inverse y with three outputs -> x-rows C2R of the three
@@ -73,14 +72,6 @@ __global__ void k_pm_set_maxbits(PmDev* st, const float* mass, float mass_value,
   if ((threadIdx.x & 63) == 0) atomicMax(&st->maxbits, __float_as_uint(mx));
 }
 
-__device__ __forceinline__ double pm_fixed_scale(uint32_t maxbits, int64_t n) {
-  const float m = __uint_as_float(maxbits);
-  if (!(m > 0.0f)) return 1.0;
-  int e = (int)floor(62.0 - log2((double)m * (double)(n > 0 ? n : 1)));
-  e = e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
-  return ldexp(1.0, e);
-}
-
 // ------------------------------------------------------------- gated sort
 __global__ void k_base_keys(const float* __restrict__ pos, int64_t n, int G, float h, uint32_t* __restrict__ keys,
                             uint32_t* __restrict__ vals, const uint32_t* gate) {
@@ -125,11 +116,23 @@ __global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint
 }
 
 // ---------------------------------------------------------- tile deposit
+// The grid is cut into 32x16x16 tiles.  A tile's particles are those whose
+// base cell, as of the last sort, lies in the tile grown by 1 + D cells on
+// the low side and D on the high side (D = drift bound): a contiguous run of
+// the sorted list per (z, y) row of that region, found from cell_start.  The
+// visit list of a tile is cut into work items of at most TD_CHUNK particles,
+// so a dense clump is spread over many workgroups.  An item accumulates exact
+// fixed-point CIC contributions in a 64 KB LDS tile with 64-bit integer
+// atomics; a tile with one item writes its cells directly, the items of a
+// heavier tile add their LDS tiles into a zero-kept int64 tile image in HBM
+// and the last one to finish converts it and clears it again.  Integer sums
+// make the result bitwise independent of order and of the item split.
 constexpr int TD_X = 32, TD_Y = 16, TD_Z = 16;
 constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;  // 8192 int64 = 64 KB of LDS: two workgroups per CU
 constexpr int TD_THREADS = 512;
 constexpr int TD_MAXSEG = (TD_Y + 1 + 2 * DMAX) * (TD_Z + 1 + 2 * DMAX) * 2;
-constexpr int TD_UNROLL = 4;
+constexpr uint32_t TD_CHUNK = 32768;  // particle visits per work item
+constexpr int TD_U = 8;                // particles loaded per lane per batch
 
 struct AxisRange {
   int lo, len;  // wrapped start and length of the extended range (len <= G)
@@ -148,188 +151,253 @@ __device__ __forceinline__ AxisRange ext_range(int t0, int T, int D, int G) {
   return r;
 }
 
+struct TileGeo {
+  int x0, y0, z0;
+  AxisRange rx, ry, rz;
+  int nrow, xsplit;
+};
+
+__device__ __forceinline__ TileGeo tile_geo(int tile, int G, int D) {
+  TileGeo t;
+  const int ntx = G / TD_X, nty = G / TD_Y;
+  t.x0 = (tile % ntx) * TD_X;
+  t.y0 = ((tile / ntx) % nty) * TD_Y;
+  t.z0 = (tile / (ntx * nty)) * TD_Z;
+  t.rx = ext_range(t.x0, TD_X, D, G);
+  t.ry = ext_range(t.y0, TD_Y, D, G);
+  t.rz = ext_range(t.z0, TD_Z, D, G);
+  t.nrow = t.ry.len * t.rz.len;
+  t.xsplit = (t.rx.lo + t.rx.len > G) ? 2 : 1;
+  return t;
+}
+
+// sorted-slot range [lo, hi) of segment r (row r >> 1, x part r & 1)
+__device__ __forceinline__ void tile_segment(const TileGeo& t, int r, int G, const uint32_t* __restrict__ cell_start,
+                                             uint32_t& lo, uint32_t& hi) {
+  const int part = r & 1, row = r >> 1;
+  lo = hi = 0;
+  if (part >= t.xsplit) return;
+  const int yy = (t.ry.lo + row % t.ry.len) & (G - 1), zz = (t.rz.lo + row / t.ry.len) & (G - 1);
+  const uint32_t rbase = ((uint32_t)zz * G + yy) * G;
+  int xa, xb;  // [xa, xb) in row coordinates
+  if (t.xsplit == 1) {
+    xa = t.rx.lo;
+    xb = t.rx.lo + t.rx.len;
+  } else if (part == 0) {
+    xa = t.rx.lo;
+    xb = G;
+  } else {
+    xa = 0;
+    xb = t.rx.lo + t.rx.len - G;
+  }
+  lo = cell_start[rbase + xa];
+  hi = cell_start[rbase + xb];
+}
+
+// items[tile] = number of work items of the tile (one wave per tile);
+// items[ntiles] = 0 so that an exclusive scan leaves the total at the end.
+__global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ cell_start,
+                                                  const PmDev* __restrict__ st, int G, int ntiles,
+                                                  uint32_t* __restrict__ items) {
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (tile > ntiles) return;
+  if (tile == ntiles) {
+    if (lane == 0) items[ntiles] = 0u;
+    return;
+  }
+  const int D = min(drift_cells(st), DMAX);
+  const TileGeo t = tile_geo(tile, G, D);
+  uint32_t tot = 0;
+  for (int r = lane; r < t.nrow * 2; r += 64) {
+    uint32_t lo, hi;
+    tile_segment(t, r, G, cell_start, lo, hi);
+    tot += hi - lo;
+  }
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+  if (lane == 0) items[tile] = tot == 0u ? 1u : (tot + TD_CHUNK - 1) / TD_CHUNK;
+}
+
+// adds a lane's run sums (base cell at tile offsets ax, ay, az) into the LDS tile
+__device__ __forceinline__ void td_flush(unsigned long long* acc, const long long* a, int key, int G) {
+  if (key < 0) return;
+  const int ax = key & 1023, ay = (key >> 10) & 1023, az = key >> 20;
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    const int cx = (ax + (o & 1)) & (G - 1), cy = (ay + ((o >> 1) & 1)) & (G - 1), cz = (az + (o >> 2)) & (G - 1);
+    if (cx < TD_X && cy < TD_Y && cz < TD_Z && a[o] != 0)
+      atomicAdd(&acc[(cz * TD_Y + cy) * TD_X + cx], (unsigned long long)a[o]);
+  }
+}
+
 // perm (may be NULL): particle index of sorted slot i; otherwise slot i is particle i.
-__global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(const float* __restrict__ pos,
-                                                          const uint32_t* __restrict__ perm,
-                                                          const float* __restrict__ mass, float mass_value, int64_t n,
-                                                          int G, float h, float scale,
-                                                          const uint32_t* __restrict__ cell_start,
-                                                          const PmDev* __restrict__ st, float* __restrict__ rho) {
+// P2: h is a power of two and inv_h = 1/h (multiply instead of divide, same bits).
+template <bool P2>
+__global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
+    const float* __restrict__ pos, const uint32_t* __restrict__ perm, const float* __restrict__ mass,
+    float mass_value, int64_t n, int G, float h, float inv_h, float scale, const uint32_t* __restrict__ cell_start,
+    const PmDev* __restrict__ st, const uint32_t* __restrict__ item_start, int ntiles,
+    unsigned long long* __restrict__ fix, uint32_t* __restrict__ tile_done, float* __restrict__ rho) {
   __shared__ unsigned long long acc[TD_CELLS];
   __shared__ uint32_t seg_lo[TD_MAXSEG];
   __shared__ uint32_t seg_pre[TD_MAXSEG + 1];
-  __shared__ int nseg_sh;
+  __shared__ int last_sh;
   const int tid = threadIdx.x;
-  const int ntx = G / TD_X, nty = G / TD_Y;
-  const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
-  const int x0 = bx * TD_X, y0 = by * TD_Y, z0 = bz * TD_Z;
   // the re-sort gate keeps D <= DMAX; the clamp only protects the LDS segment table
   const int D = min(drift_cells(st), DMAX);
-  const double S = pm_fixed_scale(st->maxbits, n);
-  for (int i = tid; i < TD_CELLS; i += TD_THREADS) acc[i] = 0ull;
-
-  // segments: one per (z', y') row of the extended region, split at the x wrap
-  const AxisRange rx = ext_range(x0, TD_X, D, G), ry = ext_range(y0, TD_Y, D, G), rz = ext_range(z0, TD_Z, D, G);
-  const int nrow = ry.len * rz.len;
-  const int xsplit = (rx.lo + rx.len > G) ? 2 : 1;
-  for (int r = tid; r < nrow * 2; r += TD_THREADS) {
-    const int part = r & 1, row = r >> 1;
-    const int yy = (ry.lo + row % ry.len) & (G - 1), zz = (rz.lo + row / ry.len) & (G - 1);
-    const uint32_t rbase = ((uint32_t)zz * G + yy) * G;
-    uint32_t lo = 0, hi = 0;
-    if (part < xsplit) {
-      int xa, xb;  // [xa, xb) in row coordinates
-      if (xsplit == 1) {
-        xa = rx.lo;
-        xb = rx.lo + rx.len;
-      } else if (part == 0) {
-        xa = rx.lo;
-        xb = G;
-      } else {
-        xa = 0;
-        xb = rx.lo + rx.len - G;
-      }
-      lo = cell_start[rbase + xa];
-      hi = cell_start[rbase + xb];
-    }
-    seg_lo[r] = lo;
-    seg_pre[r + 1] = hi - lo;
-  }
-  if (tid == 0) {
-    seg_pre[0] = 0;
-    nseg_sh = nrow * 2;
-  }
-  __syncthreads();
-  // inclusive scan of segment lengths (serial over waves: nseg <= TD_MAXSEG)
-  const int nseg = nseg_sh;
-  if (tid < 64) {
-    uint32_t carry = 0;
-    for (int base = 1; base <= nseg; base += 64) {
-      const int j = base + tid;
-      uint32_t v = j <= nseg ? seg_pre[j] : 0u;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(v, o, 64);
-        if (tid >= o) v += y;
-      }
-      if (j <= nseg) seg_pre[j] = v + carry;
-      carry += __shfl(v, 63, 64);
-    }
-  }
-  __syncthreads();
-  const uint32_t total = seg_pre[nseg];
-
-  // Each wave takes one contiguous slice of the tile's particle list and walks
-  // it 64 x TD_UNROLL particles at a time; a lane advances its segment index
-  // linearly (indices grow by at most 64 per sub-batch).  Runs of lanes with
-  // the same base cell (the list is cell-sorted) are summed in registers by a
-  // segmented scan over exact int64 values before the LDS atomics, which keeps
-  // dense cells from serialising the atomics.
-  const int lane = tid & 63, wave = tid >> 6;
-  constexpr int NW = TD_THREADS / 64;
-  const uint32_t per = (total + NW - 1) / NW;
-  const uint32_t wlo = min(total, (uint32_t)wave * per), whi = min(total, wlo + per);
-  int seg = 0;
-  if (wlo < whi) {  // first segment of the slice: binary search, once per wave
-    int a = 0, b = nseg;
+  const int E = fixed_exponent(st->maxbits, n);
+  const uint32_t total_items = item_start[ntiles];
+  for (uint32_t item = blockIdx.x; item < total_items; item += gridDim.x) {
+    // tile of this item: last t with item_start[t] <= item
+    int a = 0, b = ntiles;
     while (b - a > 1) {
       const int m = (a + b) >> 1;
-      if (seg_pre[m] <= wlo) a = m; else b = m;
+      if (item_start[m] <= item) a = m; else b = m;
     }
-    seg = a;
-  }
-  for (uint32_t base = wlo; base < whi; base += 64 * TD_UNROLL) {
-    float px[TD_UNROLL][3], pm[TD_UNROLL];
-    bool ok[TD_UNROLL];
+    const int tile = a;
+    const uint32_t chunk = item - item_start[tile], nch = item_start[tile + 1] - item_start[tile];
+    const TileGeo t = tile_geo(tile, G, D);
+    for (int i = tid; i < TD_CELLS; i += TD_THREADS) acc[i] = 0ull;
+    const int nseg = t.nrow * 2;
+    for (int r = tid; r < nseg; r += TD_THREADS) {
+      uint32_t lo, hi;
+      tile_segment(t, r, G, cell_start, lo, hi);
+      seg_lo[r] = lo;
+      seg_pre[r + 1] = hi - lo;
+    }
+    if (tid == 0) seg_pre[0] = 0;
+    __syncthreads();
+    if (tid < 64) {  // inclusive scan of the segment lengths (one wave)
+      uint32_t carry = 0;
+      for (int base = 1; base <= nseg; base += 64) {
+        const int j = base + tid;
+        uint32_t v = j <= nseg ? seg_pre[j] : 0u;
 #pragma unroll
-    for (int u = 0; u < TD_UNROLL; ++u) {
-      const uint32_t w = base + u * 64 + lane;
-      ok[u] = w < whi;
-      pm[u] = mass_value;
-      px[u][0] = px[u][1] = px[u][2] = 0.0f;
-      if (ok[u]) {
-        while (seg_pre[seg + 1] <= w) ++seg;
-        const int64_t slot = (int64_t)seg_lo[seg] + (w - seg_pre[seg]);
-        const int64_t p = perm ? (int64_t)perm[slot] : slot;
-        px[u][0] = pos[3 * p];
-        px[u][1] = pos[3 * p + 1];
-        px[u][2] = pos[3 * p + 2];
-        if (mass) pm[u] = mass[p];
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(v, o, 64);
+          if (tid >= o) v += y;
+        }
+        if (j <= nseg) seg_pre[j] = v + carry;
+        carry += __shfl(v, 63, 64);
       }
     }
-    seg = __shfl(seg, 63, 64);  // lanes resume from the furthest segment reached
-#pragma unroll
-    for (int u = 0; u < TD_UNROLL; ++u) {
-      int i0[3] = {0, 0, 0};
-      float wv[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float uu = cic_u(px[u][d], 0.0f, h);
-        const float f = floorf(uu);
-        i0[d] = (int)f;
-        wv[d] = __fsub_rn(uu, f);
+    __syncthreads();
+    const uint32_t total = seg_pre[nseg];
+    const uint32_t vlo = min(total, chunk * TD_CHUNK), vhi = min(total, vlo + TD_CHUNK);
+    // each thread takes a contiguous run of the item's visits, so a thread
+    // meets the particles of a cell one after another and sums them in
+    // registers; it touches LDS only when the base cell changes
+    const uint32_t per = (vhi - vlo + TD_THREADS - 1) / TD_THREADS;
+    const uint32_t tlo = min(vhi, vlo + tid * per), thi = min(vhi, tlo + per);
+    if (tlo < thi) {
+      int seg = 0, bb = nseg;
+      while (bb - seg > 1) {
+        const int m = (seg + bb) >> 1;
+        if (seg_pre[m] <= tlo) seg = m; else bb = m;
       }
-      const float val = __fmul_rn(pm[u], scale);
-      // local tile coordinates of the two corners along each axis (-1 = outside)
-      int lx[2], ly[2], lz[2];
+      uint32_t snext = seg_pre[seg + 1];
+      int64_t sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
+      int rkey = -1;
+      long long run[8];
 #pragma unroll
-      for (int o = 0; o < 2; ++o) {
-        const int ax = (i0[0] + o - x0) & (G - 1);
-        const int ay = (i0[1] + o - y0) & (G - 1);
-        const int az = (i0[2] + o - z0) & (G - 1);
-        lx[o] = ax < TD_X ? ax : -1;
-        ly[o] = ay < TD_Y ? ay : -1;
-        lz[o] = az < TD_Z ? az : -1;
-      }
-      long long q[8];
+      for (int o = 0; o < 8; ++o) run[o] = 0;
+      for (uint32_t w = tlo; w < thi; w += TD_U) {
+        float px[TD_U][3], pm[TD_U];
 #pragma unroll
-      for (int o = 0; o < 8; ++o) {
-        const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-        const float wx = ox ? wv[0] : __fsub_rn(1.0f, wv[0]);
-        const float wy = oy ? wv[1] : __fsub_rn(1.0f, wv[1]);
-        const float wz = oz ? wv[2] : __fsub_rn(1.0f, wv[2]);
-        const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
-        q[o] = ok[u] ? (long long)rint((double)__fmul_rn(val, wt) * S) : 0ll;
-      }
-      // run detection on the wrapped base cell
-      const int key = ok[u] ? ((wrap(i0[2], G) * G + wrap(i0[1], G)) * G + wrap(i0[0], G)) : -1 - lane;
-      const int kprev = __shfl_up(key, 1, 64);
-      const bool head = lane == 0 || key != kprev;
-      const unsigned long long heads = __ballot(head);
-      bool tail = true;
-      if (heads != ~0ull) {  // some run longer than one lane: segmented scan
-        bool f = head;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          long long o8[8];
-#pragma unroll
-          for (int o = 0; o < 8; ++o) o8[o] = __shfl_up(q[o], off, 64);
-          const bool fo = __shfl_up((int)f, off, 64) != 0;
-          if (lane >= off && !f) {
-#pragma unroll
-            for (int o = 0; o < 8; ++o) q[o] += o8[o];
+        for (int u = 0; u < TD_U; ++u) {
+          const uint32_t ww = w + u;
+          pm[u] = mass_value;
+          px[u][0] = px[u][1] = px[u][2] = 0.0f;
+          if (ww < thi) {
+            while (ww >= snext) {
+              ++seg;
+              snext = seg_pre[seg + 1];
+              sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
+            }
+            const int64_t slot = sbase + ww;
+            const int64_t p = perm ? (int64_t)perm[slot] : slot;
+            px[u][0] = pos[3 * p];
+            px[u][1] = pos[3 * p + 1];
+            px[u][2] = pos[3 * p + 2];
+            if (mass) pm[u] = mass[p];
           }
-          if (lane >= off) f = f || fo;
         }
-        const bool next_head = __shfl_down((int)head, 1, 64) != 0;
-        tail = lane == 63 || next_head;
-      }
-      if (ok[u] && tail) {
 #pragma unroll
-        for (int o = 0; o < 8; ++o) {
-          const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-          if (lx[ox] < 0 || ly[oy] < 0 || lz[oz] < 0 || q[o] == 0) continue;
-          atomicAdd(&acc[(lz[oz] * TD_Y + ly[oy]) * TD_X + lx[ox]], (unsigned long long)q[o]);
+        for (int u = 0; u < TD_U; ++u) {
+          if (w + u >= thi) break;
+          int i0[3];
+          float wv[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const float uu = P2 ? cic_u_pow2(px[u][d], inv_h) : cic_u(px[u][d], 0.0f, h);
+            const float f = floorf(uu);
+            i0[d] = (int)f;
+            wv[d] = __fsub_rn(uu, f);
+          }
+          // base cell as offsets from the tile origin (wrapped into [0, G))
+          const int ax = (i0[0] - t.x0) & (G - 1), ay = (i0[1] - t.y0) & (G - 1), az = (i0[2] - t.z0) & (G - 1);
+          const int key = (az << 20) | (ay << 10) | ax;
+          if (key != rkey) {
+            td_flush(acc, run, rkey, G);
+#pragma unroll
+            for (int o = 0; o < 8; ++o) run[o] = 0;
+            rkey = key;
+          }
+          // a corner is inside when its offset (mod G) is below the tile size
+          const bool inx = ax < TD_X || ax == G - 1, iny = ay < TD_Y || ay == G - 1, inz = az < TD_Z || az == G - 1;
+          if (!(inx && iny && inz)) continue;
+          const float val = __fmul_rn(pm[u], scale);
+#pragma unroll
+          for (int o = 0; o < 8; ++o) {
+            const float wx = (o & 1) ? wv[0] : __fsub_rn(1.0f, wv[0]);
+            const float wy = ((o >> 1) & 1) ? wv[1] : __fsub_rn(1.0f, wv[1]);
+            const float wz = (o >> 2) ? wv[2] : __fsub_rn(1.0f, wv[2]);
+            const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
+            run[o] += fixed_round(__fmul_rn(val, wt), E);
+          }
         }
+      }
+      td_flush(acc, run, rkey, G);
+    }
+    __syncthreads();
+    if (nch == 1) {
+      for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
+        const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
+        const int64_t cell = ((int64_t)(t.z0 + lz) * G + (t.y0 + ly)) * G + (t.x0 + lx);
+        rho[cell] = (float)ldexp((double)(long long)acc[i], -E);
+      }
+    } else {
+      unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
+      for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
+        const unsigned long long v = acc[i];
+        if (v) atomicAdd(&img[i], v);
+      }
+      __threadfence();
+      __syncthreads();
+      if (tid == 0) {
+        const bool last = atomicAdd(&tile_done[tile], 1u) == nch - 1;
+        __threadfence();
+        last_sh = last;
+      }
+      __syncthreads();
+      if (last_sh) {
+        __threadfence();
+        for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
+          const unsigned long long v = __hip_atomic_load(&img[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
+          const int64_t cell = ((int64_t)(t.z0 + lz) * G + (t.y0 + ly)) * G + (t.x0 + lx);
+          rho[cell] = (float)ldexp((double)(long long)v, -E);
+          img[i] = 0ull;  // the image stays zero between launches
+        }
+        if (tid == 0) tile_done[tile] = 0u;
       }
     }
+    __syncthreads();  // LDS is reused by the next item
   }
-  __syncthreads();
-  for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
-    const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
-    const int64_t cell = ((int64_t)(z0 + lz) * G + (y0 + ly)) * G + (x0 + lx);
-    rho[cell] = (float)((double)(long long)acc[i] / S);
-  }
+}
+
+// zero-fills the heavy-tile images and completion counters (once per workspace set-up)
+__global__ void k_zero_u64(unsigned long long* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
 // ------------------------------------------------------ fused FFT passes
@@ -492,10 +560,11 @@ __device__ __forceinline__ void block_max_to(float v, uint32_t* target) {
   }
 }
 
+template <bool P2>
 __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, float* __restrict__ vel,
                                                     float* __restrict__ acc, const float4* __restrict__ force,
                                                     const uint32_t* __restrict__ perm, int64_t n, int G, float h,
-                                                    float hdt, float dt, float L, int do_kick, int drift,
+                                                    float inv_h, float hdt, float dt, float L, int do_kick, int drift,
                                                     PmDev* __restrict__ st) {
   float vmax = 0.0f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -506,7 +575,7 @@ __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, flo
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       p[d] = pos[3 * pi + d];
-      const float u = cic_u(p[d], 0.0f, h);
+      const float u = P2 ? cic_u_pow2(p[d], inv_h) : cic_u(p[d], 0.0f, h);
       const float f = floorf(u);
       i0[d] = (int)f;
       w[d] = __fsub_rn(u, f);
@@ -531,8 +600,12 @@ __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, flo
           a[1] += f.y * wt;
           a[2] += f.z * wt;
         }
+    // between steps of one call the acceleration lives only in the velocities
+    // (both half kicks are applied here); it is stored when no drift follows
+    if (!drift) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[3 * pi + d] = a[d];
+      for (int d = 0; d < 3; ++d) acc[3 * pi + d] = a[d];
+    }
     if (do_kick) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -594,7 +667,12 @@ struct PmBufs {
   void* scan_ws;         // scan scratch (G^3 + 1)
   float* tmp;            // 3n
   uint32_t* perm;        // n (for unsorted force evaluation)
+  uint32_t* items;       // ntiles + 1: work items per tile, then their exclusive scan
+  uint32_t* tile_done;   // ntiles: finished items of a heavy tile (kept zero between launches)
+  unsigned long long* fix;  // G^3: int64 tile images of heavy tiles (kept zero between launches)
 };
+
+static int64_t pm_tiles(int64_t G) { return (G / TD_X) * (G / TD_Y) * (G / TD_Z); }
 
 static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   const int64_t G3 = G * G * G, nc = G * G * (G / 2 + 1);
@@ -611,6 +689,9 @@ static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   b.scan_ws = c.take<char>(scan_workspace_bytes(G3 + 1));
   b.tmp = c.take<float>(3 * n);
   b.perm = c.take<uint32_t>(n);
+  b.items = c.take<uint32_t>(pm_tiles(G) + 1);
+  b.tile_done = c.take<uint32_t>(pm_tiles(G));
+  b.fix = c.take<unsigned long long>(G3);
   return c.ok();
 }
 
@@ -618,6 +699,19 @@ static int check_pm(int64_t n, int64_t G, float L) {
   AUGUR_REQUIRE(n >= 1 && n < ((int64_t)1 << 31), AUGUR_EINVAL, "pm: need 1 <= n < 2^31");
   AUGUR_REQUIRE(is_pow2(G) && G >= 32 && G <= 512, AUGUR_EUNSUPPORTED, "pm: G must be a power of two in [32, 512]");
   AUGUR_REQUIRE(L > 0.0f, AUGUR_EINVAL, "pm: L must be positive");
+  return AUGUR_OK;
+}
+
+// fresh state: bookkeeping reset, heavy-tile images and counters zeroed
+static int pm_init(PmBufs& b, int64_t G, hipStream_t s) {
+  hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
+  AUGUR_LAUNCHED("pm_init");
+  const int64_t G3 = G * G * G;
+  hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(G3, 256, 4096)), dim3(256), 0, s, b.fix, G3);
+  AUGUR_LAUNCHED("zero_fix");
+  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(pm_tiles(G), 256, 1024)), dim3(256), 0, s, b.tile_done, pm_tiles(G),
+                     (const uint32_t*)nullptr);
+  AUGUR_LAUNCHED("zero_tile_done");
   return AUGUR_OK;
 }
 
@@ -697,11 +791,30 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
       hipLaunchKernelGGL(k_pm_set_maxbits, dim3(1), dim3(64), 0, s, b.st, (const float*)nullptr, mass_value, scale, n);
     AUGUR_LAUNCHED("dep_bound");
   }
+  const int64_t ntiles = pm_tiles(G);
+  {
+    TimedScope ts_("dep_plan", s);
+    hipLaunchKernelGGL(k_dep_plan, dim3((unsigned)((ntiles + 1 + 3) / 4)), dim3(256), 0, s,
+                       (const uint32_t*)b.cell_start, (const PmDev*)b.st, (int)G, (int)ntiles, b.items);
+    AUGUR_LAUNCHED("dep_plan");
+    const int rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, nullptr);
+    if (rc) return rc;
+  }
   {
     TimedScope ts_("dep_tiles", s);
-    const int tiles = (int)((G / TD_X) * (G / TD_Y) * (G / TD_Z));
-    hipLaunchKernelGGL(k_dep_tiles, dim3(tiles), dim3(TD_THREADS), 0, s, pos, perm, mass, mass_value, n, (int)G, h,
-                       scale, b.cell_start, b.st, b.rho);
+    // one workgroup per tile plus enough for the items of heavy tiles in a
+    // typical scene; the kernel loops over any items beyond the grid
+    const int64_t grid = ntiles + (2 * n + TD_CHUNK - 1) / TD_CHUNK;
+    const bool p2 = pow2_float(h);
+    const float inv_h = p2 ? 1.0f / h : 0.0f;
+    if (p2)
+      hipLaunchKernelGGL(k_dep_tiles<true>, dim3((unsigned)grid), dim3(TD_THREADS), 0, s, pos, perm, mass,
+                         mass_value, n, (int)G, h, inv_h, scale, (const uint32_t*)b.cell_start, (const PmDev*)b.st,
+                         (const uint32_t*)b.items, (int)ntiles, b.fix, b.tile_done, b.rho);
+    else
+      hipLaunchKernelGGL(k_dep_tiles<false>, dim3((unsigned)grid), dim3(TD_THREADS), 0, s, pos, perm, mass,
+                         mass_value, n, (int)G, h, inv_h, scale, (const uint32_t*)b.cell_start, (const PmDev*)b.st,
+                         (const uint32_t*)b.items, (int)ntiles, b.fix, b.tile_done, b.rho);
     AUGUR_LAUNCHED("dep_tiles");
   }
   return AUGUR_OK;
@@ -757,8 +870,8 @@ int augur_pm_prepare(float* pos, float* vel, float* acc, float* mass, int32_t* i
   PmBufs b;
   AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_prepare: workspace too small (%zu < %zu)", ws_bytes,
                 c.used);
-  hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
-  AUGUR_LAUNCHED("pm_init");
+  rc = pm_init(b, G, s);
+  if (rc) return rc;
   return pm_sort(pos, vel, acc, mass, ids, n, G, L, b, nullptr, true, s);
 }
 
@@ -772,16 +885,22 @@ int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_v
   PmBufs b;
   AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_accel: workspace too small (%zu < %zu)", ws_bytes, c.used);
   // any input order: sort an index permutation (D = 0), deposit through it
-  hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
-  AUGUR_LAUNCHED("pm_init");
+  rc = pm_init(b, G, s);
+  if (rc) return rc;
   rc = pm_sort((float*)pos, nullptr, nullptr, nullptr, nullptr, n, G, L, b, nullptr, false, s);
   if (rc) return rc;
   rc = pm_force_grid(pos, b.perm, mass, mass_value, n, G, L, Gconst, b, s);
   if (rc) return rc;
   const float h = (float)((double)L / (double)G);
   TimedScope ts_("gather", s);
-  hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos, (float*)nullptr, acc,
-                     b.force, b.perm, n, (int)G, h, 0.0f, 0.0f, L, 0, 0, (PmDev*)nullptr);
+  if (pow2_float(h))
+    hipLaunchKernelGGL(k_gather_kdk<true>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos,
+                       (float*)nullptr, acc, (const float4*)b.force, (const uint32_t*)b.perm, n, (int)G, h, 1.0f / h,
+                       0.0f, 0.0f, L, 0, 0, (PmDev*)nullptr);
+  else
+    hipLaunchKernelGGL(k_gather_kdk<false>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos,
+                       (float*)nullptr, acc, (const float4*)b.force, (const uint32_t*)b.perm, n, (int)G, h, 0.0f,
+                       0.0f, 0.0f, L, 0, 0, (PmDev*)nullptr);
   AUGUR_LAUNCHED("gather");
   return AUGUR_OK;
 }
@@ -796,8 +915,8 @@ int augur_pm_density(const float* pos, const float* mass, float mass_value, int6
   PmBufs b;
   AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_density: workspace too small (%zu < %zu)", ws_bytes,
                 c.used);
-  hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
-  AUGUR_LAUNCHED("pm_init");
+  rc = pm_init(b, G, s);
+  if (rc) return rc;
   rc = pm_sort((float*)pos, nullptr, nullptr, nullptr, nullptr, n, G, L, b, nullptr, false, s);
   if (rc) return rc;
   b.rho = rho;
@@ -816,6 +935,7 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, 
   AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_kdk: workspace too small (%zu < %zu)", ws_bytes, c.used);
   const float h = (float)((double)L / (double)G);
   const float hdt = 0.5f * dt;
+  const bool p2 = pow2_float(h);
   {
     TimedScope ts_("kick_drift", s);
     hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc, n, hdt, dt, L, b.st);
@@ -824,8 +944,9 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, 
   for (int st = 0; st < nsteps; ++st) {
     hipLaunchKernelGGL(k_pm_decide, dim3(1), dim3(64), 0, s, b.st, dt / h, 0);
     AUGUR_LAUNCHED("pm_decide");
-    // the state arrays (ids included) are permuted in place when the gate opens
-    rc = pm_sort(pos, vel, acc, mass, ids, n, G, L, b, &b.st->need_sort, true, s);
+    // the state arrays (ids included) are permuted in place when the gate
+    // opens; acc is not (the last step rewrites all of it in the final order)
+    rc = pm_sort(pos, vel, nullptr, mass, ids, n, G, L, b, &b.st->need_sort, true, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st);
     AUGUR_LAUNCHED("pm_sorted");
@@ -833,8 +954,14 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, 
     if (rc) return rc;
     const int last = st == nsteps - 1;
     TimedScope ts_("gather_kdk", s);
-    hipLaunchKernelGGL(k_gather_kdk, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc, b.force,
-                       (const uint32_t*)nullptr, n, (int)G, h, hdt, dt, L, 1, last ? 0 : 1, b.st);
+    if (p2)
+      hipLaunchKernelGGL(k_gather_kdk<true>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc,
+                         (const float4*)b.force, (const uint32_t*)nullptr, n, (int)G, h, 1.0f / h, hdt, dt, L, 1,
+                         last ? 0 : 1, b.st);
+    else
+      hipLaunchKernelGGL(k_gather_kdk<false>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc,
+                         (const float4*)b.force, (const uint32_t*)nullptr, n, (int)G, h, 0.0f, hdt, dt, L, 1,
+                         last ? 0 : 1, b.st);
     AUGUR_LAUNCHED("gather_kdk");
   }
   return AUGUR_OK;

@@ -21,25 +21,28 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(name, N, G):
-    """Compulsory HBM bytes per launch of each kernel."""
+    """Compulsory HBM bytes per launch of each per-step kernel: every input
+    read once and every output written once (FP32 particles and grids,
+    complex64 half spectra of G*G*(G/2+1) points, the force grid counted at
+    its 12 payload bytes per cell)."""
     G3 = G ** 3
     spec = 8 * G * G * (G // 2 + 1)
     return {
-        "gather_kdk": 48 * N + 12 * G3,          # r/w pos+vel, read 3 force grids
-        "dep_fixed": 12 * N,                     # read pos
-        "fft_rows_r2c": 4 * G3 + spec,
-        "fft_cols_fwd": 2 * spec,
-        "poisson": spec + 3 * spec,
-        "fft_cols_inv": 2 * 3 * spec,
-        "fft_rows_c2r": 3 * (spec + 4 * G3),
+        "dep_tiles": 12 * N + 4 * G3,            # read positions, write the density grid
+        "fft_rows_r2c": 4 * G3 + spec,           # density -> x half spectrum
+        "fft_cols_fwd": 2 * spec,                # y transform in place
+        "pm_zpass": 3 * spec,                    # z fwd + Green + z inv -> A (in place), B
+        "pm_yinv": 5 * spec,                     # read A, B; write three y-inverse spectra
+        "pm_xc2r": 3 * spec + 12 * G3,           # three x C2R rows -> force grid
+        "gather_kdk": 48 * N + 12 * G3,          # r/w positions + velocities, read the force grid
     }.get(name)
 
 
-STAGES = {  # stage -> (kernels, algorithmic bytes per step)
-    "deposit": (("fill_u64", "dep_absmax", "dep_fixed", "dep_convert"), lambda N, G: 12 * N + 4 * G ** 3),
-    "fft_poisson_grad": (("fft_rows_r2c", "fft_cols_fwd", "poisson", "fft_cols_inv", "fft_rows_c2r"),
-                         lambda N, G: 4 * G ** 3 + 8 * G * G * (G // 2 + 1) + 3 * (8 * G * G * (G // 2 + 1) + 4 * G ** 3)),
-    "gather_kdk": (("gather_kdk",), lambda N, G: 48 * N + 12 * G ** 3),
+STAGES = {  # stage -> kernels (per-step algorithmic bytes are the sum of theirs)
+    "deposit": ("dep_bound", "dep_plan", "dep_tiles"),
+    "fft_poisson_grad": ("fft_rows_r2c", "fft_cols_fwd", "pm_zpass", "pm_yinv", "pm_xc2r"),
+    "gather_kdk": ("gather_kdk",),
+    "resort_gate": ("pm_sort",),
 }
 
 
@@ -131,12 +134,12 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
                 "avg_launch_us": avg_s * 1e6, "algorithmic_bytes": ab}
     stages = {}
-    for sname, (kern, fb) in STAGES.items():
+    for sname, kern in STAGES.items():
         ms = sum(timing.get(k, (0.0, 0))[0] for k in kern) / a.steps
-        b = fb(N, G)
+        b = sum(algorithmic_bytes(k, N, G) or 0 for k in kern)
         stages[sname] = {"ms_per_step": ms, "algorithmic_bytes": b,
-                         "GB_s": b / (ms / 1e3) / 1e9 if ms > 0 else None,
-                         "frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS if ms > 0 else None}
+                         "GB_s": b / (ms / 1e3) / 1e9 if ms > 0 and b else None,
+                         "frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS if ms > 0 and b else None}
     kernels = {k: {"ms_total": v[0], "launches": v[1], "us_avg": v[0] / v[1] * 1e3} for k, v in timing.items()}
 
     cpu = None
