@@ -3,11 +3,11 @@
 // State: particles physically sorted by CIC base cell (key = (z*G + y)*G + x
 // of floor((pos/h) - 0.5) wrapped), with cell_start[] (G^3 + 1 offsets) from
 // the last sort.  One step:
-//   1. deposit   k_dep_plan counts each 32x16x16 tile's particles (base cell,
-//                as of the last sort, in the tile's region grown by the drift
-//                bound D) and cuts them into work items; k_dep_tiles sums
-//                exact fixed-point CIC contributions per item in LDS with
-//                64-bit integer atomics (bitwise order-independent).
+//   1. deposit   k_dep_tiles: per 32x16x16 tile, the particles whose base
+//                cell (as of the last sort) lies in the tile's region grown
+//                by the drift bound D, cut into work items planned at sort
+//                time; exact fixed-point CIC sums in LDS with 64-bit integer
+//                atomics (bitwise order-independent).
 //   2. FFT       x-rows R2C -> y-columns -> z-columns fused with the Green's
 //                function and the inverse z transform (two outputs) ->
 //                inverse y with three outputs -> x-rows C2R of the three
@@ -117,22 +117,27 @@ __global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint
 
 // ---------------------------------------------------------- tile deposit
 // The grid is cut into 32x16x16 tiles.  A tile's particles are those whose
-// base cell, as of the last sort, lies in the tile grown by 1 + D cells on
-// the low side and D on the high side (D = drift bound): a contiguous run of
-// the sorted list per (z, y) row of that region, found from cell_start.  The
-// visit list of a tile is cut into work items of at most TD_CHUNK particles,
-// so a dense clump is spread over many workgroups.  An item accumulates exact
-// fixed-point CIC contributions in a 64 KB LDS tile with 64-bit integer
-// atomics; a tile with one item writes its cells directly, the items of a
-// heavier tile add their LDS tiles into a zero-kept int64 tile image in HBM
-// and the last one to finish converts it and clears it again.  Integer sums
-// make the result bitwise independent of order and of the item split.
+// base cell, as of the last sort, lies in the tile grown by 1 + DMAX cells on
+// the low side and DMAX on the high side (the drift bound never exceeds DMAX
+// between sorts): a contiguous run of the sorted list per (z, y) row of that
+// region.  At sort time k_dep_plan stores each tile's row segments and cuts
+// its visit list into work items, so a dense clump is spread over many
+// workgroups and a step only streams the stored tables.  An
+// item accumulates exact fixed-point CIC contributions in a 64 KB LDS tile
+// with 64-bit integer atomics; a tile with one item writes its cells
+// directly, the items of a heavier tile add their LDS tiles into a zero-kept
+// int64 tile image in HBM with device atomics, and k_dep_finish (the next
+// launch) converts and clears the images.
+// Integer sums make the result bitwise independent of order and of the split.
 constexpr int TD_X = 32, TD_Y = 16, TD_Z = 16;
 constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;  // 8192 int64 = 64 KB of LDS: two workgroups per CU
 constexpr int TD_THREADS = 512;
+constexpr int TD_NW = TD_THREADS / 64;
 constexpr int TD_MAXSEG = (TD_Y + 1 + 2 * DMAX) * (TD_Z + 1 + 2 * DMAX) * 2;
-constexpr uint32_t TD_CHUNK = 32768;  // particle visits per work item
-constexpr int TD_U = 8;                // particles loaded per lane per batch
+constexpr uint32_t TD_CHUNK = 32768;  // most particle visits per work item
+constexpr int TD_K = 8;               // consecutive visits per lane in a wave batch
+constexpr int TD_BLK = 64;            // visits per entry of the visit -> segment table
+constexpr int TD_NBLK = TD_CHUNK / TD_BLK;  // table entries per item
 
 struct AxisRange {
   int lo, len;  // wrapped start and length of the extended range (len <= G)
@@ -194,30 +199,79 @@ __device__ __forceinline__ void tile_segment(const TileGeo& t, int r, int G, con
   hi = cell_start[rbase + xb];
 }
 
-// items[tile] = number of work items of the tile (one wave per tile);
-// items[ntiles] = 0 so that an exclusive scan leaves the total at the end.
-__global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ cell_start,
-                                                  const PmDev* __restrict__ st, int G, int ntiles,
-                                                  uint32_t* __restrict__ items) {
+// Sort-time plan, one wave per tile: the tile's row segments (sorted-slot
+// start and inclusive prefix of lengths, TD_MAXSEG each) and its number of
+// work items; items[ntiles] = 0 so an exclusive scan leaves the total there.
+__global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ cell_start, int G, int ntiles,
+                                                  uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
+                                                  uint32_t* __restrict__ seg_pre, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (tile > ntiles) return;
   if (tile == ntiles) {
     if (lane == 0) items[ntiles] = 0u;
     return;
   }
-  const int D = min(drift_cells(st), DMAX);
-  const TileGeo t = tile_geo(tile, G, D);
-  uint32_t tot = 0;
-  for (int r = lane; r < t.nrow * 2; r += 64) {
-    uint32_t lo, hi;
-    tile_segment(t, r, G, cell_start, lo, hi);
-    tot += hi - lo;
+  const TileGeo t = tile_geo(tile, G, DMAX);
+  uint32_t* lo_t = seg_lo + (int64_t)tile * TD_MAXSEG;
+  uint32_t* pre_t = seg_pre + (int64_t)tile * (TD_MAXSEG + 1);
+  if (lane == 0) pre_t[0] = 0u;
+  uint32_t carry = 0;
+  for (int r0 = 0; r0 < TD_MAXSEG; r0 += 64) {
+    const int r = r0 + lane;
+    uint32_t lo = 0, hi = 0;
+    if (r < t.nrow * 2) tile_segment(t, r, G, cell_start, lo, hi);
+    uint32_t v = hi - lo;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(v, o, 64);
+      if (lane >= o) v += y;
+    }
+    if (r < TD_MAXSEG) {
+      lo_t[r] = lo;
+      pre_t[r + 1] = v + carry;
+    }
+    carry += __shfl(v, 63, 64);
   }
-  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
-  if (lane == 0) items[tile] = tot == 0u ? 1u : (tot + TD_CHUNK - 1) / TD_CHUNK;
+  if (lane == 0) items[tile] = carry == 0u ? 1u : (carry + TD_CHUNK - 1) / TD_CHUNK;
 }
 
-// adds a lane's run sums (base cell at tile offsets ax, ay, az) into the LDS tile
+// Per work item (one workgroup each; items beyond the total exit): its tile
+// and visit range [vlo, vhi) (the tile's visits cut into near-equal parts of
+// at most TD_CHUNK), and blk[j] = the segment holding visit vlo + TD_BLK j.
+struct DepItem {
+  uint32_t tile, vlo, vhi, pad;
+};
+
+__global__ __launch_bounds__(256) void k_dep_items(const uint32_t* __restrict__ item_start, int ntiles,
+                                                   const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
+                                                   uint16_t* __restrict__ blk, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  const uint32_t item = blockIdx.x;
+  if (item >= item_start[ntiles]) return;
+  int a = 0, b = ntiles;  // last tile with item_start[tile] <= item
+  while (b - a > 1) {
+    const int m = (a + b) >> 1;
+    if (item_start[m] <= item) a = m; else b = m;
+  }
+  const uint32_t* pre = tseg_pre + (int64_t)a * (TD_MAXSEG + 1);
+  const uint32_t chunk = item - item_start[a], nch = item_start[a + 1] - item_start[a];
+  const uint32_t total = pre[TD_MAXSEG];
+  const uint32_t per = ((total + nch - 1) / nch + TD_BLK - 1) / TD_BLK * TD_BLK;  // <= TD_CHUNK
+  const uint32_t vlo = min(total, chunk * per), vhi = min(total, vlo + per);
+  if (threadIdx.x == 0) info[item] = DepItem{(uint32_t)a, vlo, vhi, 0u};
+  for (uint32_t j = threadIdx.x; j < (vhi - vlo + TD_BLK - 1) / TD_BLK; j += blockDim.x) {
+    const uint32_t v = vlo + j * TD_BLK;  // last segment with pre[seg] <= v (non-empty: v < total)
+    int lo = 0, hi = TD_MAXSEG;
+    while (hi - lo > 1) {
+      const int m = (lo + hi) >> 1;
+      if (pre[m] <= v) lo = m; else hi = m;
+    }
+    blk[(int64_t)item * TD_NBLK + j] = (uint16_t)lo;
+  }
+}
+
+// adds a lane's run sums (base cell at tile offsets in key) into the LDS tile
 __device__ __forceinline__ void td_flush(unsigned long long* acc, const long long* a, int key, int G) {
   if (key < 0) return;
   const int ax = key & 1023, ay = (key >> 10) & 1023, az = key >> 20;
@@ -229,85 +283,66 @@ __device__ __forceinline__ void td_flush(unsigned long long* acc, const long lon
   }
 }
 
+// rint(v * 2^e) for v >= 0 (fixed_round without the sign step)
+__device__ __forceinline__ long long fixed_round_pos(float v, int e) {
+  const float r = rintf(ldexpf(v, e));
+  const float hi = floorf(r * 2.3283064365386963e-10f);
+  const float lo = fmaf(hi, -4294967296.0f, r);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // perm (may be NULL): particle index of sorted slot i; otherwise slot i is particle i.
 // P2: h is a power of two and inv_h = 1/h (multiply instead of divide, same bits).
 template <bool P2>
 __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     const float* __restrict__ pos, const uint32_t* __restrict__ perm, const float* __restrict__ mass,
-    float mass_value, int64_t n, int G, float h, float inv_h, float scale, const uint32_t* __restrict__ cell_start,
-    const PmDev* __restrict__ st, const uint32_t* __restrict__ item_start, int ntiles,
-    unsigned long long* __restrict__ fix, uint32_t* __restrict__ tile_done, float* __restrict__ rho) {
+    float mass_value, int64_t n, int G, float h, float inv_h, float scale, const PmDev* __restrict__ st,
+    const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, const uint16_t* __restrict__ iblk,
+    const uint32_t* __restrict__ tseg_lo, const uint32_t* __restrict__ tseg_pre, int ntiles,
+    unsigned long long* __restrict__ fix, float* __restrict__ rho) {
   __shared__ unsigned long long acc[TD_CELLS];
   __shared__ uint32_t seg_lo[TD_MAXSEG];
   __shared__ uint32_t seg_pre[TD_MAXSEG + 1];
-  __shared__ int last_sh;
-  const int tid = threadIdx.x;
-  // the re-sort gate keeps D <= DMAX; the clamp only protects the LDS segment table
-  const int D = min(drift_cells(st), DMAX);
+  __shared__ uint16_t blk_seg[TD_NBLK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int E = fixed_exponent(st->maxbits, n);
   const uint32_t total_items = item_start[ntiles];
   for (uint32_t item = blockIdx.x; item < total_items; item += gridDim.x) {
-    // tile of this item: last t with item_start[t] <= item
-    int a = 0, b = ntiles;
-    while (b - a > 1) {
-      const int m = (a + b) >> 1;
-      if (item_start[m] <= item) a = m; else b = m;
-    }
-    const int tile = a;
-    const uint32_t chunk = item - item_start[tile], nch = item_start[tile + 1] - item_start[tile];
-    const TileGeo t = tile_geo(tile, G, D);
+    const DepItem it = info[item];
+    const int tile = (int)it.tile;
+    const uint32_t nch = item_start[tile + 1] - item_start[tile];
+    const TileGeo t = tile_geo(tile, G, DMAX);
+    const uint32_t* lo_t = tseg_lo + (int64_t)tile * TD_MAXSEG;
+    const uint32_t* pre_t = tseg_pre + (int64_t)tile * (TD_MAXSEG + 1);
+    const uint16_t* blk_t = iblk + (int64_t)item * TD_NBLK;
+    const uint32_t plo = it.vlo, phi = it.vhi;
+    const uint32_t pblk = (phi - plo + TD_BLK - 1) / TD_BLK;
     for (int i = tid; i < TD_CELLS; i += TD_THREADS) acc[i] = 0ull;
-    const int nseg = t.nrow * 2;
-    for (int r = tid; r < nseg; r += TD_THREADS) {
-      uint32_t lo, hi;
-      tile_segment(t, r, G, cell_start, lo, hi);
-      seg_lo[r] = lo;
-      seg_pre[r + 1] = hi - lo;
+    for (int r = tid; r <= TD_MAXSEG; r += TD_THREADS) {
+      if (r < TD_MAXSEG) seg_lo[r] = lo_t[r];
+      seg_pre[r] = pre_t[r];
     }
-    if (tid == 0) seg_pre[0] = 0;
+    for (uint32_t j = tid; j < pblk; j += TD_THREADS) blk_seg[j] = blk_t[j];
     __syncthreads();
-    if (tid < 64) {  // inclusive scan of the segment lengths (one wave)
-      uint32_t carry = 0;
-      for (int base = 1; base <= nseg; base += 64) {
-        const int j = base + tid;
-        uint32_t v = j <= nseg ? seg_pre[j] : 0u;
+    {
+      // wave batches of 64 * TD_K consecutive visits, TD_K per lane: loads are
+      // coalesced, a lane sums a cell's particles in registers and touches LDS
+      // when its base cell changes, and rows outside the tile skip as a whole
+      for (uint32_t bb = plo + (uint32_t)wave * 64 * TD_K; bb < phi; bb += TD_NW * 64 * TD_K) {
+        const uint32_t l0 = bb + (uint32_t)lane * TD_K;
+        if (l0 >= phi) continue;
+        const uint32_t l1 = min(phi, l0 + TD_K);
+        int seg = blk_seg[(l0 - plo) / TD_BLK];
+        while (seg_pre[seg + 1] <= l0) ++seg;
+        uint32_t snext = seg_pre[seg + 1];
+        int64_t sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
+        float px[TD_K][3], pm[TD_K];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const uint32_t y = __shfl_up(v, o, 64);
-          if (tid >= o) v += y;
-        }
-        if (j <= nseg) seg_pre[j] = v + carry;
-        carry += __shfl(v, 63, 64);
-      }
-    }
-    __syncthreads();
-    const uint32_t total = seg_pre[nseg];
-    const uint32_t vlo = min(total, chunk * TD_CHUNK), vhi = min(total, vlo + TD_CHUNK);
-    // each thread takes a contiguous run of the item's visits, so a thread
-    // meets the particles of a cell one after another and sums them in
-    // registers; it touches LDS only when the base cell changes
-    const uint32_t per = (vhi - vlo + TD_THREADS - 1) / TD_THREADS;
-    const uint32_t tlo = min(vhi, vlo + tid * per), thi = min(vhi, tlo + per);
-    if (tlo < thi) {
-      int seg = 0, bb = nseg;
-      while (bb - seg > 1) {
-        const int m = (seg + bb) >> 1;
-        if (seg_pre[m] <= tlo) seg = m; else bb = m;
-      }
-      uint32_t snext = seg_pre[seg + 1];
-      int64_t sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
-      int rkey = -1;
-      long long run[8];
-#pragma unroll
-      for (int o = 0; o < 8; ++o) run[o] = 0;
-      for (uint32_t w = tlo; w < thi; w += TD_U) {
-        float px[TD_U][3], pm[TD_U];
-#pragma unroll
-        for (int u = 0; u < TD_U; ++u) {
-          const uint32_t ww = w + u;
+        for (int u = 0; u < TD_K; ++u) {
+          const uint32_t ww = l0 + u;
           pm[u] = mass_value;
           px[u][0] = px[u][1] = px[u][2] = 0.0f;
-          if (ww < thi) {
+          if (ww < l1) {
             while (ww >= snext) {
               ++seg;
               snext = seg_pre[seg + 1];
@@ -321,9 +356,13 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
             if (mass) pm[u] = mass[p];
           }
         }
+        int rkey = -1;
+        long long run[8];
 #pragma unroll
-        for (int u = 0; u < TD_U; ++u) {
-          if (w + u >= thi) break;
+        for (int o = 0; o < 8; ++o) run[o] = 0;
+#pragma unroll
+        for (int u = 0; u < TD_K; ++u) {
+          if (l0 + u >= l1) break;
           int i0[3];
           float wv[3];
 #pragma unroll
@@ -333,8 +372,11 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
             i0[d] = (int)f;
             wv[d] = __fsub_rn(uu, f);
           }
-          // base cell as offsets from the tile origin (wrapped into [0, G))
+          // base cell as offsets from the tile origin (wrapped into [0, G));
+          // a corner is inside when its offset is below the tile size
           const int ax = (i0[0] - t.x0) & (G - 1), ay = (i0[1] - t.y0) & (G - 1), az = (i0[2] - t.z0) & (G - 1);
+          const bool in = (ax < TD_X || ax == G - 1) && (ay < TD_Y || ay == G - 1) && (az < TD_Z || az == G - 1);
+          if (!in) continue;
           const int key = (az << 20) | (ay << 10) | ax;
           if (key != rkey) {
             td_flush(acc, run, rkey, G);
@@ -342,21 +384,25 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
             for (int o = 0; o < 8; ++o) run[o] = 0;
             rkey = key;
           }
-          // a corner is inside when its offset (mod G) is below the tile size
-          const bool inx = ax < TD_X || ax == G - 1, iny = ay < TD_Y || ay == G - 1, inz = az < TD_Z || az == G - 1;
-          if (!(inx && iny && inz)) continue;
-          const float val = __fmul_rn(pm[u], scale);
+          // contributions of |m| (weights are >= 0), negated for negative masses
+          const float val = fabsf(__fmul_rn(pm[u], scale));
+          long long q[8];
 #pragma unroll
           for (int o = 0; o < 8; ++o) {
             const float wx = (o & 1) ? wv[0] : __fsub_rn(1.0f, wv[0]);
             const float wy = ((o >> 1) & 1) ? wv[1] : __fsub_rn(1.0f, wv[1]);
             const float wz = (o >> 2) ? wv[2] : __fsub_rn(1.0f, wv[2]);
-            const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
-            run[o] += fixed_round(__fmul_rn(val, wt), E);
+            q[o] = fixed_round_pos(__fmul_rn(val, __fmul_rn(__fmul_rn(wx, wy), wz)), E);
           }
+          if (pm[u] < 0.0f) {
+#pragma unroll
+            for (int o = 0; o < 8; ++o) q[o] = -q[o];
+          }
+#pragma unroll
+          for (int o = 0; o < 8; ++o) run[o] += q[o];
         }
+        td_flush(acc, run, rkey, G);
       }
-      td_flush(acc, run, rkey, G);
     }
     __syncthreads();
     if (nch == 1) {
@@ -365,37 +411,44 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
         const int64_t cell = ((int64_t)(t.z0 + lz) * G + (t.y0 + ly)) * G + (t.x0 + lx);
         rho[cell] = (float)ldexp((double)(long long)acc[i], -E);
       }
-    } else {
+    } else {  // heavy tile: add into its image; k_dep_finish converts it
       unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
       for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
         const unsigned long long v = acc[i];
         if (v) atomicAdd(&img[i], v);
-      }
-      __threadfence();
-      __syncthreads();
-      if (tid == 0) {
-        const bool last = atomicAdd(&tile_done[tile], 1u) == nch - 1;
-        __threadfence();
-        last_sh = last;
-      }
-      __syncthreads();
-      if (last_sh) {
-        __threadfence();
-        for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
-          const unsigned long long v = __hip_atomic_load(&img[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
-          const int64_t cell = ((int64_t)(t.z0 + lz) * G + (t.y0 + ly)) * G + (t.x0 + lx);
-          rho[cell] = (float)ldexp((double)(long long)v, -E);
-          img[i] = 0ull;  // the image stays zero between launches
-        }
-        if (tid == 0) tile_done[tile] = 0u;
       }
     }
     __syncthreads();  // LDS is reused by the next item
   }
 }
 
-// zero-fills the heavy-tile images and completion counters (once per workspace set-up)
+// heavy tiles (more than one item): image -> density cells, image cleared
+// again for the next launch (one workgroup per tile; others exit)
+__global__ __launch_bounds__(256) void k_dep_finish(const uint32_t* __restrict__ item_start,
+                                                    unsigned long long* __restrict__ fix, const PmDev* __restrict__ st,
+                                                    int64_t n, int G, float* __restrict__ rho) {
+  const int tile = blockIdx.x;
+  if (item_start[tile + 1] - item_start[tile] <= 1u) return;
+  const int E = fixed_exponent(st->maxbits, n);
+  const int ntx = G / TD_X, nty = G / TD_Y;
+  const int x0 = (tile % ntx) * TD_X, y0 = ((tile / ntx) % nty) * TD_Y, z0 = (tile / (ntx * nty)) * TD_Z;
+  // plain loads: the atomics of k_dep_tiles are complete and visible at this
+  // launch; each thread reads its 32 cells before writing any
+  unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
+  constexpr int PER = TD_CELLS / 256;
+  unsigned long long v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) v[k] = img[threadIdx.x + 256 * k];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
+    rho[((int64_t)(z0 + lz) * G + (y0 + ly)) * G + (x0 + lx)] = (float)ldexp((double)(long long)v[k], -E);
+    img[i] = 0ull;  // the image stays zero between launches
+  }
+}
+
+// zero-fills the heavy-tile images (once per workspace set-up)
 __global__ void k_zero_u64(unsigned long long* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0ull;
 }
@@ -668,11 +721,17 @@ struct PmBufs {
   float* tmp;            // 3n
   uint32_t* perm;        // n (for unsorted force evaluation)
   uint32_t* items;       // ntiles + 1: work items per tile, then their exclusive scan
-  uint32_t* tile_done;   // ntiles: finished items of a heavy tile (kept zero between launches)
   unsigned long long* fix;  // G^3: int64 tile images of heavy tiles (kept zero between launches)
+  uint32_t* tseg_lo;     // ntiles * TD_MAXSEG: row segment starts per tile
+  uint32_t* tseg_pre;    // ntiles * (TD_MAXSEG + 1): inclusive prefix of their lengths
+  DepItem* info;         // max_items: tile and visit range of each work item
+  uint16_t* iblk;        // max_items * TD_NBLK: visit -> segment table of each item
 };
 
 static int64_t pm_tiles(int64_t G) { return (G / TD_X) * (G / TD_Y) * (G / TD_Z); }
+// a cell lies in at most 2 tile regions per axis, so the planned visits are
+// at most 8n and every tile has at least one item
+static int64_t pm_max_items(int64_t n, int64_t G) { return pm_tiles(G) + (8 * n + TD_CHUNK - 1) / TD_CHUNK; }
 
 static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   const int64_t G3 = G * G * G, nc = G * G * (G / 2 + 1);
@@ -690,8 +749,11 @@ static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   b.tmp = c.take<float>(3 * n);
   b.perm = c.take<uint32_t>(n);
   b.items = c.take<uint32_t>(pm_tiles(G) + 1);
-  b.tile_done = c.take<uint32_t>(pm_tiles(G));
   b.fix = c.take<unsigned long long>(G3);
+  b.tseg_lo = c.take<uint32_t>(pm_tiles(G) * TD_MAXSEG);
+  b.tseg_pre = c.take<uint32_t>(pm_tiles(G) * (TD_MAXSEG + 1));
+  b.info = c.take<DepItem>(pm_max_items(n, G));
+  b.iblk = c.take<uint16_t>(pm_max_items(n, G) * TD_NBLK);
   return c.ok();
 }
 
@@ -702,16 +764,13 @@ static int check_pm(int64_t n, int64_t G, float L) {
   return AUGUR_OK;
 }
 
-// fresh state: bookkeeping reset, heavy-tile images and counters zeroed
+// fresh state: bookkeeping reset, heavy-tile images zeroed
 static int pm_init(PmBufs& b, int64_t G, hipStream_t s) {
   hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
   AUGUR_LAUNCHED("pm_init");
   const int64_t G3 = G * G * G;
   hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(G3, 256, 4096)), dim3(256), 0, s, b.fix, G3);
   AUGUR_LAUNCHED("zero_fix");
-  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(pm_tiles(G), 256, 1024)), dim3(256), 0, s, b.tile_done, pm_tiles(G),
-                     (const uint32_t*)nullptr);
-  AUGUR_LAUNCHED("zero_tile_done");
   return AUGUR_OK;
 }
 
@@ -736,6 +795,16 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   AUGUR_LAUNCHED("count_cells");
   rc = exclusive_scan_u32(b.cell_start, b.cell_start, G3 + 1, b.scan_ws, s, gate);
   if (rc) return rc;
+  // deposit work items of each tile for the largest drift bound
+  const int64_t ntiles = pm_tiles(G);
+  hipLaunchKernelGGL(k_dep_plan, dim3((unsigned)((ntiles + 1 + 3) / 4)), dim3(256), 0, s,
+                     (const uint32_t*)b.cell_start, (int)G, (int)ntiles, b.items, b.tseg_lo, b.tseg_pre, gate);
+  AUGUR_LAUNCHED("dep_plan");
+  rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, gate);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dep_items, dim3((unsigned)pm_max_items(n, G)), dim3(256), 0, s, (const uint32_t*)b.items,
+                     (int)ntiles, (const uint32_t*)b.tseg_pre, b.info, b.iblk, gate);
+  AUGUR_LAUNCHED("dep_items");
   if (!physical) {
     hipLaunchKernelGGL(k_copy<uint32_t>, g, bl, 0, s, perm, b.perm, n, gate);
     AUGUR_LAUNCHED("copy_perm");
@@ -793,14 +862,6 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
   }
   const int64_t ntiles = pm_tiles(G);
   {
-    TimedScope ts_("dep_plan", s);
-    hipLaunchKernelGGL(k_dep_plan, dim3((unsigned)((ntiles + 1 + 3) / 4)), dim3(256), 0, s,
-                       (const uint32_t*)b.cell_start, (const PmDev*)b.st, (int)G, (int)ntiles, b.items);
-    AUGUR_LAUNCHED("dep_plan");
-    const int rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, nullptr);
-    if (rc) return rc;
-  }
-  {
     TimedScope ts_("dep_tiles", s);
     // one workgroup per tile plus enough for the items of heavy tiles in a
     // typical scene; the kernel loops over any items beyond the grid
@@ -809,13 +870,20 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
     const float inv_h = p2 ? 1.0f / h : 0.0f;
     if (p2)
       hipLaunchKernelGGL(k_dep_tiles<true>, dim3((unsigned)grid), dim3(TD_THREADS), 0, s, pos, perm, mass,
-                         mass_value, n, (int)G, h, inv_h, scale, (const uint32_t*)b.cell_start, (const PmDev*)b.st,
-                         (const uint32_t*)b.items, (int)ntiles, b.fix, b.tile_done, b.rho);
+                         mass_value, n, (int)G, h, inv_h, scale, (const PmDev*)b.st, (const uint32_t*)b.items,
+                         (const DepItem*)b.info, (const uint16_t*)b.iblk, (const uint32_t*)b.tseg_lo,
+                         (const uint32_t*)b.tseg_pre,
+                         (int)ntiles, b.fix, b.rho);
     else
       hipLaunchKernelGGL(k_dep_tiles<false>, dim3((unsigned)grid), dim3(TD_THREADS), 0, s, pos, perm, mass,
-                         mass_value, n, (int)G, h, inv_h, scale, (const uint32_t*)b.cell_start, (const PmDev*)b.st,
-                         (const uint32_t*)b.items, (int)ntiles, b.fix, b.tile_done, b.rho);
+                         mass_value, n, (int)G, h, inv_h, scale, (const PmDev*)b.st, (const uint32_t*)b.items,
+                         (const DepItem*)b.info, (const uint16_t*)b.iblk, (const uint32_t*)b.tseg_lo,
+                         (const uint32_t*)b.tseg_pre,
+                         (int)ntiles, b.fix, b.rho);
     AUGUR_LAUNCHED("dep_tiles");
+    hipLaunchKernelGGL(k_dep_finish, dim3((unsigned)ntiles), dim3(256), 0, s, (const uint32_t*)b.items, b.fix,
+                       (const PmDev*)b.st, n, (int)G, b.rho);
+    AUGUR_LAUNCHED("dep_finish");
   }
   return AUGUR_OK;
 }

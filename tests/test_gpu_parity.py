@@ -311,6 +311,28 @@ def test_pm_kdk_with_device_resorts(dev):
     assert np.abs(v - rv).max() < 1e-4 * np.abs(rv).max()
 
 
+def test_pm_kdk_heavy_tiles_over_steps(dev):
+    # a dense clump: its tiles hold several deposit work items whose int64
+    # images are merged in HBM and cleared for the next step; several steps
+    # reuse them and stay in parity, and the state's density equals the
+    # generic fixed-point deposit bitwise
+    from augur_amd import ops
+    from augur_amd.pm import PeriodicPM
+    rng = np.random.default_rng(31)
+    n, G, dt, steps = 150000, 64, 1e-3, 4
+    pos = np.mod(0.3 + 0.02 * rng.normal(size=(n, 3)), 1.0).astype(F32)
+    vel = (0.02 * rng.normal(size=(n, 3))).astype(F32)
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    rho = pm.density(T(pos, dev), mass_value=1.0 / n)
+    ref = ops.cic_deposit(T(pos[None], dev), None, G, 1.0 / G, 0.0, True, mass_value=1.0 / n)[0, 0]
+    assert torch.equal(rho.view(torch.int32), ref.view(torch.int32))
+    p, v, a = (x.cpu().numpy() for x in pm.rollout(T(pos, dev), T(vel, dev), dt=dt, steps=steps))
+    rp, rv = opm.kdk_pm_rollout(pos, vel, 1.0 / n, G, dt, steps)
+    dp = np.abs(((p - rp) + 0.5) % 1.0 - 0.5).max()
+    assert dp < 2e-5, dp
+    assert np.abs(v - rv).max() < 2e-4 * np.abs(rv).max()
+
+
 # --------------------------------------------------------------- a9 oracle
 def test_direct_sum_known_answer_and_oracle(dev):
     from augur_amd import ops
