@@ -4,7 +4,7 @@
 // Deposit determinism: every CIC contribution c (computed in FP32 with the
 // reference's op order) is converted to an exact fixed-point integer
 // rint(c * 2^e), with one power-of-two exponent e per channel chosen from an
-// order-independent bound (n * max|value| <= 2^62), and accumulated with
+// order-independent bound (n * max|value| * 2^e <= 2^52), and accumulated with
 // 64-bit integer adds.  Integer addition is associative, so the grid is
 // bitwise the same for any particle order, thread count or launch shape.
 #include "internal.hpp"
@@ -342,8 +342,8 @@ __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos
     const int64_t b = i / N;
     float mm = mass ? mass[i] : mass_value;
     if (mask) mm = __fmul_rn(mm, mask[i]);
-    float val[4];
-    for (int c = 0; c < C; ++c) val[c] = chan_value(c, mm, vel, i, D, scale);
+    float valE[4];
+    for (int c = 0; c < C; ++c) valE[c] = ldexpf(chan_value(c, mm, vel, i, D, scale), E[c]);
     int i0[D];
     float w[D];
 #pragma unroll
@@ -374,8 +374,8 @@ __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos
       }
       if (!ok) continue;
       for (int c = 0; c < C; ++c) {
-        const float contrib = __fmul_rn(val[c], wt);
-        const long long q = fixed_round(contrib, E[c]);
+        // rint(val * wt * 2^e) with the exact power-of-two scaling applied to val
+        const long long q = fixed_round(__fmul_rn(valE[c], wt), 0);
         if (q) atomicAdd(&acc[((b * C) + c) * GD + flat], (unsigned long long)q);
       }
     }

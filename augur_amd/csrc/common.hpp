@@ -98,11 +98,13 @@ __device__ __forceinline__ float cic_u(float p, float off, float h) {
 __device__ __forceinline__ float cic_u_pow2(float p, float inv_h) { return __fsub_rn(__fmul_rn(p, inv_h), 0.5f); }
 
 // Fixed-point exponent of the order-independent deposits: the largest e with
-// n * max|v| * 2^e <= 2^62, so that any sum of n contributions fits in int64.
+// n * max|v| * 2^e <= 2^52.  A contribution v counts rint(v * 2^e) units, and
+// any sum of n of them is an integer of magnitude below 2^53: exact in int64
+// and in FP64 alike, so every summation order gives the same bits.
 __device__ __forceinline__ int fixed_exponent(unsigned maxbits, int64_t n) {
   const float m = __uint_as_float(maxbits);
   if (!(m > 0.0f)) return 0;
-  int e = (int)floor(62.0 - log2((double)m * (double)(n > 0 ? n : 1)));
+  int e = (int)floor(52.0 - log2((double)m * (double)(n > 0 ? n : 1)));
   return e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
 }
 

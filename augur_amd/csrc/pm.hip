@@ -6,8 +6,8 @@
 //   1. deposit   k_dep_tiles: per 32x16x16 tile, the particles whose base
 //                cell (as of the last sort) lies in the tile's region grown
 //                by the drift bound D, cut into work items planned at sort
-//                time; exact fixed-point CIC sums in LDS with 64-bit integer
-//                atomics (bitwise order-independent).
+//                time; fixed-point CIC unit counts summed exactly in FP64
+//                (registers and LDS atomics), bitwise order-independent.
 //   2. FFT       x-rows R2C -> y-columns -> z-columns fused with the Green's
 //                function and the inverse z transform (two outputs) ->
 //                inverse y with three outputs -> x-rows C2R of the three
@@ -123,8 +123,9 @@ __global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint
 // region.  At sort time k_dep_plan stores each tile's row segments and cuts
 // its visit list into work items, so a dense clump is spread over many
 // workgroups and a step only streams the stored tables.  An
-// item accumulates exact fixed-point CIC contributions in a 64 KB LDS tile
-// with 64-bit integer atomics; a tile with one item writes its cells
+// item accumulates the fixed-point unit counts of its CIC contributions
+// (integers below 2^53, so FP64 sums are exact) in a 64 KB LDS tile with FP64
+// atomics; a tile with one item writes its cells
 // directly, the items of a heavier tile add their LDS tiles into a zero-kept
 // int64 tile image in HBM with device atomics, and k_dep_finish (the next
 // launch) converts and clears the images.
@@ -272,23 +273,14 @@ __global__ __launch_bounds__(256) void k_dep_items(const uint32_t* __restrict__ 
 }
 
 // adds a lane's run sums (base cell at tile offsets in key) into the LDS tile
-__device__ __forceinline__ void td_flush(unsigned long long* acc, const long long* a, int key, int G) {
+__device__ __forceinline__ void td_flush(double* acc, const double* a, int key, int G) {
   if (key < 0) return;
   const int ax = key & 1023, ay = (key >> 10) & 1023, az = key >> 20;
 #pragma unroll
   for (int o = 0; o < 8; ++o) {
     const int cx = (ax + (o & 1)) & (G - 1), cy = (ay + ((o >> 1) & 1)) & (G - 1), cz = (az + (o >> 2)) & (G - 1);
-    if (cx < TD_X && cy < TD_Y && cz < TD_Z && a[o] != 0)
-      atomicAdd(&acc[(cz * TD_Y + cy) * TD_X + cx], (unsigned long long)a[o]);
+    if (cx < TD_X && cy < TD_Y && cz < TD_Z && a[o] != 0.0) atomicAdd(&acc[(cz * TD_Y + cy) * TD_X + cx], a[o]);
   }
-}
-
-// rint(v * 2^e) for v >= 0 (fixed_round without the sign step)
-__device__ __forceinline__ long long fixed_round_pos(float v, int e) {
-  const float r = rintf(ldexpf(v, e));
-  const float hi = floorf(r * 2.3283064365386963e-10f);
-  const float lo = fmaf(hi, -4294967296.0f, r);
-  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
 // perm (may be NULL): particle index of sorted slot i; otherwise slot i is particle i.
@@ -300,7 +292,9 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, const uint16_t* __restrict__ iblk,
     const uint32_t* __restrict__ tseg_lo, const uint32_t* __restrict__ tseg_pre, int ntiles,
     unsigned long long* __restrict__ fix, float* __restrict__ rho) {
-  __shared__ unsigned long long acc[TD_CELLS];
+  // unit counts of the fixed-point contributions: integers below 2^53 in FP64,
+  // so the LDS float atomics and register sums are exact in any order
+  __shared__ double acc[TD_CELLS];
   __shared__ uint32_t seg_lo[TD_MAXSEG];
   __shared__ uint32_t seg_pre[TD_MAXSEG + 1];
   __shared__ uint16_t blk_seg[TD_NBLK];
@@ -317,7 +311,7 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     const uint16_t* blk_t = iblk + (int64_t)item * TD_NBLK;
     const uint32_t plo = it.vlo, phi = it.vhi;
     const uint32_t pblk = (phi - plo + TD_BLK - 1) / TD_BLK;
-    for (int i = tid; i < TD_CELLS; i += TD_THREADS) acc[i] = 0ull;
+    for (int i = tid; i < TD_CELLS; i += TD_THREADS) acc[i] = 0.0;
     for (int r = tid; r <= TD_MAXSEG; r += TD_THREADS) {
       if (r < TD_MAXSEG) seg_lo[r] = lo_t[r];
       seg_pre[r] = pre_t[r];
@@ -325,6 +319,9 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     for (uint32_t j = tid; j < pblk; j += TD_THREADS) blk_seg[j] = blk_t[j];
     __syncthreads();
     {
+      // wave batches of 64 * TD_K consecutive visits, TD_K per lane: loads are
+      // coalesced, a lane sums a cell's particles in registers and touches LDS
+      // when its base cell changes, and rows outside the tile skip as a whole
       // wave batches of 64 * TD_K consecutive visits, TD_K per lane: loads are
       // coalesced, a lane sums a cell's particles in registers and touches LDS
       // when its base cell changes, and rows outside the tile skip as a whole
@@ -357,9 +354,9 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
           }
         }
         int rkey = -1;
-        long long run[8];
+        double run[8];
 #pragma unroll
-        for (int o = 0; o < 8; ++o) run[o] = 0;
+        for (int o = 0; o < 8; ++o) run[o] = 0.0;
 #pragma unroll
         for (int u = 0; u < TD_K; ++u) {
           if (l0 + u >= l1) break;
@@ -381,25 +378,18 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
           if (key != rkey) {
             td_flush(acc, run, rkey, G);
 #pragma unroll
-            for (int o = 0; o < 8; ++o) run[o] = 0;
+            for (int o = 0; o < 8; ++o) run[o] = 0.0;
             rkey = key;
           }
-          // contributions of |m| (weights are >= 0), negated for negative masses
-          const float val = fabsf(__fmul_rn(pm[u], scale));
-          long long q[8];
+          // units rint(m * scale * w * 2^E), the power-of-two scaling applied to m * scale
+          const float valE = ldexpf(__fmul_rn(pm[u], scale), E);
 #pragma unroll
           for (int o = 0; o < 8; ++o) {
             const float wx = (o & 1) ? wv[0] : __fsub_rn(1.0f, wv[0]);
             const float wy = ((o >> 1) & 1) ? wv[1] : __fsub_rn(1.0f, wv[1]);
             const float wz = (o >> 2) ? wv[2] : __fsub_rn(1.0f, wv[2]);
-            q[o] = fixed_round_pos(__fmul_rn(val, __fmul_rn(__fmul_rn(wx, wy), wz)), E);
+            run[o] += (double)rintf(__fmul_rn(valE, __fmul_rn(__fmul_rn(wx, wy), wz)));
           }
-          if (pm[u] < 0.0f) {
-#pragma unroll
-            for (int o = 0; o < 8; ++o) q[o] = -q[o];
-          }
-#pragma unroll
-          for (int o = 0; o < 8; ++o) run[o] += q[o];
         }
         td_flush(acc, run, rkey, G);
       }
@@ -409,13 +399,13 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
       for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
         const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
         const int64_t cell = ((int64_t)(t.z0 + lz) * G + (t.y0 + ly)) * G + (t.x0 + lx);
-        rho[cell] = (float)ldexp((double)(long long)acc[i], -E);
+        rho[cell] = (float)ldexp(acc[i], -E);
       }
     } else {  // heavy tile: add into its image; k_dep_finish converts it
       unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
       for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
-        const unsigned long long v = acc[i];
-        if (v) atomicAdd(&img[i], v);
+        const double v = acc[i];
+        if (v != 0.0) atomicAdd(&img[i], (unsigned long long)(long long)v);
       }
     }
     __syncthreads();  // LDS is reused by the next item
