@@ -11,8 +11,8 @@
 //   2. FFT       x-rows R2C -> y-columns -> z-columns fused with the Green's
 //                function and the inverse z transform (two outputs) ->
 //                inverse y with three outputs -> x-rows C2R of the three
-//                acceleration components written as one float4 grid.
-//   3. gather    k_gather_kdk: CIC gather of the float4 grid fused with the
+//                acceleration components written as three FP32 planes.
+//   3. gather    k_gather_tiles: CIC gather of the force planes fused with the
 //                closing half kick, the next opening half kick and the drift,
 //                and the max-speed reduction that bounds the next D.
 // D (cells) is the cumulative max drift since the last sort; when it would
@@ -35,7 +35,20 @@ struct PmDev {
 };
 
 constexpr float DRIFT_MARGIN = 1e-3f;  // cells; covers position rounding
-constexpr int DMAX = 1;                // largest halo the deposit supports
+constexpr int DMAX = 1;                // largest drift (cells) between sorts
+
+// The stepper's sort order is tile-major: 32x16x16-cell tiles in x-fastest
+// order, and inside a tile its cells in x-fastest order.  A tile's particles
+// (and each run of whole z layers of it) are then one contiguous range.
+constexpr int TD_X = 32, TD_Y = 16, TD_Z = 16;
+constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;
+
+__device__ __forceinline__ uint32_t tile_key(int cx, int cy, int cz, int G) {
+  const uint32_t ntx = (uint32_t)G / TD_X, nty = (uint32_t)G / TD_Y;
+  const uint32_t t = ((uint32_t)(cz / TD_Z) * nty + (uint32_t)(cy / TD_Y)) * ntx + (uint32_t)(cx / TD_X);
+  const uint32_t l = ((uint32_t)(cz % TD_Z) * TD_Y + (uint32_t)(cy % TD_Y)) * TD_X + (uint32_t)(cx % TD_X);
+  return t * TD_CELLS + l;
+}
 
 __device__ __forceinline__ int drift_cells(const PmDev* st) {
   const float c = st->cum_cells;
@@ -80,7 +93,7 @@ __global__ void k_base_keys(const float* __restrict__ pos, int64_t n, int G, flo
     int c[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) c[d] = wrap((int)floorf(cic_u(pos[3 * i + d], 0.0f, h)), G);
-    keys[i] = ((uint32_t)c[2] * G + c[1]) * G + c[0];
+    keys[i] = tile_key(c[0], c[1], c[2], G);
     vals[i] = (uint32_t)i;
   }
 }
@@ -116,93 +129,67 @@ __global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint
 }
 
 // ---------------------------------------------------------- tile deposit
-// The grid is cut into 32x16x16 tiles.  A tile's particles are those whose
-// base cell, as of the last sort, lies in the tile grown by 1 + DMAX cells on
-// the low side and DMAX on the high side (the drift bound never exceeds DMAX
-// between sorts): a contiguous run of the sorted list per (z, y) row of that
-// region.  At sort time k_dep_plan stores each tile's row segments and cuts
-// its visit list into work items, so a dense clump is spread over many
-// workgroups and a step only streams the stored tables.  An
-// item accumulates the fixed-point unit counts of its CIC contributions
-// (integers below 2^53, so FP64 sums are exact) in a 64 KB LDS tile with FP64
-// atomics; a tile with one item writes its cells
-// directly, the items of a heavier tile add their LDS tiles into a zero-kept
-// int64 tile image in HBM with device atomics, and k_dep_finish (the next
-// launch) converts and clears the images.
-// Integer sums make the result bitwise independent of order and of the split.
-constexpr int TD_X = 32, TD_Y = 16, TD_Z = 16;
-constexpr int TD_CELLS = TD_X * TD_Y * TD_Z;  // 8192 int64 = 64 KB of LDS: two workgroups per CU
-constexpr int TD_THREADS = 512;
+// A tile's deposit reads the particles whose base cell, as of the last sort,
+// lies in the tile grown by TD_LO = 1 + DMAX cells below and TD_HI = DMAX
+// above on each axis (the drift bound never exceeds DMAX between sorts).  In
+// the tile-major order that region is at most TD_MAXSEG key runs over the 27
+// neighbouring tiles (the tile itself is one run).  At sort time k_dep_plan
+// stores each tile's runs and k_tile_items cuts its visit list into work
+// items, so a dense clump is spread over many workgroups and a step only
+// streams the stored tables.  An item accumulates the fixed-point unit
+// counts of its CIC contributions (integers below 2^53, so FP64 sums are
+// exact) in a 64 KB LDS tile with FP64 atomics; a tile with one item writes
+// its cells directly, the items of a heavier tile add their LDS tiles into a
+// zero-kept int64 tile image in HBM with device atomics, and k_dep_finish
+// (the next launch) converts and clears the images.  Integer sums make the
+// result bitwise independent of order and of the split.
+constexpr int TD_LO = 1 + DMAX, TD_HI = DMAX;
+constexpr int TD_THREADS = 512;  // 64 KB LDS tile: two workgroups per CU
 constexpr int TD_NW = TD_THREADS / 64;
-constexpr int TD_MAXSEG = (TD_Y + 1 + 2 * DMAX) * (TD_Z + 1 + 2 * DMAX) * 2;
+// runs: a piece of the region that is partial in x gives one run per (y, z)
+// row, one full in x but partial in y one run per z layer, one full in both
+// a single run (x pieces: 2 partial + 1 full; y and z: 2 partial + 1 full)
+constexpr int TD_REGY = TD_Y + TD_LO + TD_HI, TD_REGZ = TD_Z + TD_LO + TD_HI;
+constexpr int TD_MAXSEG = 2 * TD_REGY * TD_REGZ + 2 * TD_REGZ + 3;
 constexpr uint32_t TD_CHUNK = 32768;  // most particle visits per work item
 constexpr int TD_K = 8;               // consecutive visits per lane in a wave batch
 constexpr int TD_BLK = 64;            // visits per entry of the visit -> segment table
 constexpr int TD_NBLK = TD_CHUNK / TD_BLK;  // table entries per item
 
-struct AxisRange {
-  int lo, len;  // wrapped start and length of the extended range (len <= G)
-};
-
-__device__ __forceinline__ AxisRange ext_range(int t0, int T, int D, int G) {
-  AxisRange r;
-  r.len = T + 1 + 2 * D;
-  r.lo = t0 - 1 - D;
-  if (r.len >= G) {
-    r.lo = 0;
-    r.len = G;
-  } else {
-    r.lo &= (G - 1);
-  }
-  return r;
-}
-
-struct TileGeo {
+struct TileOrigin {
   int x0, y0, z0;
-  AxisRange rx, ry, rz;
-  int nrow, xsplit;
 };
-
-__device__ __forceinline__ TileGeo tile_geo(int tile, int G, int D) {
-  TileGeo t;
+__device__ __forceinline__ TileOrigin tile_origin(int tile, int G) {
   const int ntx = G / TD_X, nty = G / TD_Y;
-  t.x0 = (tile % ntx) * TD_X;
-  t.y0 = ((tile / ntx) % nty) * TD_Y;
-  t.z0 = (tile / (ntx * nty)) * TD_Z;
-  t.rx = ext_range(t.x0, TD_X, D, G);
-  t.ry = ext_range(t.y0, TD_Y, D, G);
-  t.rz = ext_range(t.z0, TD_Z, D, G);
-  t.nrow = t.ry.len * t.rz.len;
-  t.xsplit = (t.rx.lo + t.rx.len > G) ? 2 : 1;
-  return t;
+  return TileOrigin{(tile % ntx) * TD_X, ((tile / ntx) % nty) * TD_Y, (tile / (ntx * nty)) * TD_Z};
 }
 
-// sorted-slot range [lo, hi) of segment r (row r >> 1, x part r & 1)
-__device__ __forceinline__ void tile_segment(const TileGeo& t, int r, int G, const uint32_t* __restrict__ cell_start,
-                                             uint32_t& lo, uint32_t& hi) {
-  const int part = r & 1, row = r >> 1;
-  lo = hi = 0;
-  if (part >= t.xsplit) return;
-  const int yy = (t.ry.lo + row % t.ry.len) & (G - 1), zz = (t.rz.lo + row / t.ry.len) & (G - 1);
-  const uint32_t rbase = ((uint32_t)zz * G + yy) * G;
-  int xa, xb;  // [xa, xb) in row coordinates
-  if (t.xsplit == 1) {
-    xa = t.rx.lo;
-    xb = t.rx.lo + t.rx.len;
-  } else if (part == 0) {
-    xa = t.rx.lo;
-    xb = G;
-  } else {
-    xa = 0;
-    xb = t.rx.lo + t.rx.len - G;
+// one axis of the grown region, cut at tile boundaries: tile offset (-1, 0,
+// +1) and local cell range [a, b); a single piece when one tile spans the axis
+struct Piece {
+  int off, a, b;
+};
+__device__ __forceinline__ int axis_pieces(int ntile, int T, Piece* p) {
+  if (ntile == 1) {
+    p[0] = Piece{0, 0, T};
+    return 1;
   }
-  lo = cell_start[rbase + xa];
-  hi = cell_start[rbase + xb];
+  p[0] = Piece{-1, T - TD_LO, T};
+  p[1] = Piece{0, 0, T};
+  p[2] = Piece{1, 0, TD_HI};
+  return 3;
+}
+__device__ __forceinline__ int box_runs(const Piece& px, const Piece& py, const Piece& pz) {
+  const bool fx = px.b - px.a == TD_X, fy = py.b - py.a == TD_Y;
+  if (fx && fy) return 1;
+  if (fx) return pz.b - pz.a;
+  return (py.b - py.a) * (pz.b - pz.a);
 }
 
-// Sort-time plan, one wave per tile: the tile's row segments (sorted-slot
-// start and inclusive prefix of lengths, TD_MAXSEG each) and its number of
-// work items; items[ntiles] = 0 so an exclusive scan leaves the total there.
+// Sort-time plan, one wave per tile: the grown region's key runs as sorted-
+// slot starts and an inclusive prefix of their lengths (TD_MAXSEG each; the
+// unused tail has length 0), and the tile's number of work items of at most
+// TD_CHUNK visits; items[ntiles] = 0 so an exclusive scan leaves the total there.
 __global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ cell_start, int G, int ntiles,
                                                   uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
                                                   uint32_t* __restrict__ seg_pre, const uint32_t* gate) {
@@ -213,7 +200,16 @@ __global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ c
     if (lane == 0) items[ntiles] = 0u;
     return;
   }
-  const TileGeo t = tile_geo(tile, G, DMAX);
+  const int ntx = G / TD_X, nty = G / TD_Y, ntz = G / TD_Z;
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  Piece PX[3], PY[3], PZ[3];
+  const int nx = axis_pieces(ntx, TD_X, PX), ny = axis_pieces(nty, TD_Y, PY), nz = axis_pieces(ntz, TD_Z, PZ);
+  int pre[28];  // runs before box b (boxes in z, y, x piece order)
+  pre[0] = 0;
+  for (int bz = 0, bi = 0; bz < nz; ++bz)
+    for (int by = 0; by < ny; ++by)
+      for (int bx = 0; bx < nx; ++bx, ++bi) pre[bi + 1] = pre[bi] + box_runs(PX[bx], PY[by], PZ[bz]);
+  const int nbox = nx * ny * nz, total_runs = pre[nbox];
   uint32_t* lo_t = seg_lo + (int64_t)tile * TD_MAXSEG;
   uint32_t* pre_t = seg_pre + (int64_t)tile * (TD_MAXSEG + 1);
   if (lane == 0) pre_t[0] = 0u;
@@ -221,7 +217,29 @@ __global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ c
   for (int r0 = 0; r0 < TD_MAXSEG; r0 += 64) {
     const int r = r0 + lane;
     uint32_t lo = 0, hi = 0;
-    if (r < t.nrow * 2) tile_segment(t, r, G, cell_start, lo, hi);
+    if (r < total_runs) {
+      int bi = 0;
+      while (pre[bi + 1] <= r) ++bi;
+      const int bx = bi % nx, by = (bi / nx) % ny, bz = bi / (nx * ny), k = r - pre[bi];
+      const Piece &px = PX[bx], &py = PY[by], &pz = PZ[bz];
+      const int nt = (((tz + pz.off + ntz) % ntz) * nty + (ty + py.off + nty) % nty) * ntx + (tx + px.off + ntx) % ntx;
+      const uint32_t tb = (uint32_t)nt * TD_CELLS;
+      uint32_t k0, len;
+      const bool fx = px.b - px.a == TD_X, fy = py.b - py.a == TD_Y;
+      if (fx && fy) {
+        k0 = tb + (uint32_t)pz.a * TD_Y * TD_X;
+        len = (uint32_t)(pz.b - pz.a) * TD_Y * TD_X;
+      } else if (fx) {
+        k0 = tb + ((uint32_t)(pz.a + k) * TD_Y + py.a) * TD_X;
+        len = (uint32_t)(py.b - py.a) * TD_X;
+      } else {
+        const int nyr = py.b - py.a;
+        k0 = tb + ((uint32_t)(pz.a + k / nyr) * TD_Y + py.a + k % nyr) * TD_X + px.a;
+        len = (uint32_t)(px.b - px.a);
+      }
+      lo = cell_start[k0];
+      hi = cell_start[k0 + len];
+    }
     uint32_t v = hi - lo;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -239,14 +257,16 @@ __global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ c
 
 // Per work item (one workgroup each; items beyond the total exit): its tile
 // and visit range [vlo, vhi) (the tile's visits cut into near-equal parts of
-// at most TD_CHUNK), and blk[j] = the segment holding visit vlo + TD_BLK j.
+// at most CHUNK), and blk[j] = the segment holding visit vlo + TD_BLK j.
 struct DepItem {
   uint32_t tile, vlo, vhi, pad;
 };
 
-__global__ __launch_bounds__(256) void k_dep_items(const uint32_t* __restrict__ item_start, int ntiles,
-                                                   const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
-                                                   uint16_t* __restrict__ blk, const uint32_t* gate) {
+template <int MAXSEG, uint32_t CHUNK>
+__global__ __launch_bounds__(256) void k_tile_items(const uint32_t* __restrict__ item_start, int ntiles,
+                                                    const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
+                                                    uint16_t* __restrict__ blk, const uint32_t* gate) {
+  constexpr int NBLK = CHUNK / TD_BLK;
   if (gate && *gate == 0u) return;
   const uint32_t item = blockIdx.x;
   if (item >= item_start[ntiles]) return;
@@ -255,20 +275,20 @@ __global__ __launch_bounds__(256) void k_dep_items(const uint32_t* __restrict__ 
     const int m = (a + b) >> 1;
     if (item_start[m] <= item) a = m; else b = m;
   }
-  const uint32_t* pre = tseg_pre + (int64_t)a * (TD_MAXSEG + 1);
+  const uint32_t* pre = tseg_pre + (int64_t)a * (MAXSEG + 1);
   const uint32_t chunk = item - item_start[a], nch = item_start[a + 1] - item_start[a];
-  const uint32_t total = pre[TD_MAXSEG];
-  const uint32_t per = ((total + nch - 1) / nch + TD_BLK - 1) / TD_BLK * TD_BLK;  // <= TD_CHUNK
+  const uint32_t total = pre[MAXSEG];
+  const uint32_t per = ((total + nch - 1) / nch + TD_BLK - 1) / TD_BLK * TD_BLK;  // <= CHUNK
   const uint32_t vlo = min(total, chunk * per), vhi = min(total, vlo + per);
   if (threadIdx.x == 0) info[item] = DepItem{(uint32_t)a, vlo, vhi, 0u};
   for (uint32_t j = threadIdx.x; j < (vhi - vlo + TD_BLK - 1) / TD_BLK; j += blockDim.x) {
     const uint32_t v = vlo + j * TD_BLK;  // last segment with pre[seg] <= v (non-empty: v < total)
-    int lo = 0, hi = TD_MAXSEG;
+    int lo = 0, hi = MAXSEG;
     while (hi - lo > 1) {
       const int m = (lo + hi) >> 1;
       if (pre[m] <= v) lo = m; else hi = m;
     }
-    blk[(int64_t)item * TD_NBLK + j] = (uint16_t)lo;
+    blk[(int64_t)item * NBLK + j] = (uint16_t)lo;
   }
 }
 
@@ -305,7 +325,7 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     const DepItem it = info[item];
     const int tile = (int)it.tile;
     const uint32_t nch = item_start[tile + 1] - item_start[tile];
-    const TileGeo t = tile_geo(tile, G, DMAX);
+    const TileOrigin t = tile_origin(tile, G);
     const uint32_t* lo_t = tseg_lo + (int64_t)tile * TD_MAXSEG;
     const uint32_t* pre_t = tseg_pre + (int64_t)tile * (TD_MAXSEG + 1);
     const uint16_t* blk_t = iblk + (int64_t)item * TD_NBLK;
@@ -540,7 +560,7 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0
 }
 
 // x rows: three half spectra (A~, -i ky A~, B~) -> real rows; the first gets
-// -i kx (Nyquist kx zeroed).  Output: float4 grid (a_x, a_y, a_z, 0).
+// -i kx (Nyquist kx zeroed).  Output: three FP32 planes a_x, a_y, a_z of N^3.
 template <int N>
 struct PmRow {
   static constexpr int NL = FftLines<N, FFT_THREADS>::value;
@@ -552,11 +572,11 @@ template <int N>
 __global__ __launch_bounds__(FFT_THREADS) void k_pm_xc2r(const float2* __restrict__ s0,
                                                          const float2* __restrict__ s1,
                                                          const float2* __restrict__ s2, float kf,
-                                                         float4* __restrict__ force) {
+                                                         float* __restrict__ force) {
   using C = PmRow<N>;
   __shared__ float2 X[C::NL * C::LS];
-  __shared__ float res[2][C::NL * N];
   __shared__ float2 tw[N];
+  constexpr int64_t PLANE = (int64_t)N * N * N;
   make_twiddles<N>(tw);
   __syncthreads();
   const int64_t row0 = (int64_t)blockIdx.x * C::NL;
@@ -581,11 +601,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pm_xc2r(const float2* __restric
     if (k == 0 || k == N / 2) v.y = 0.0f;
     return v;
   };
-  fft_lines<N, true>(X, tw, C::NL, src0_full, [&](int l, int x, float2 v) { res[0][l * N + x] = v.x; });
-  fft_lines<N, true>(X, tw, C::NL, src1, [&](int l, int x, float2 v) { res[1][l * N + x] = v.x; });
-  fft_lines<N, true>(X, tw, C::NL, src2, [&](int l, int x, float2 v) {
-    force[(row0 + l) * N + x] = make_float4(res[0][l * N + x], res[1][l * N + x], v.x, 0.0f);
-  });
+  fft_lines<N, true>(X, tw, C::NL, src0_full, [&](int l, int x, float2 v) { force[(row0 + l) * N + x] = v.x; });
+  fft_lines<N, true>(X, tw, C::NL, src1, [&](int l, int x, float2 v) { force[PLANE + (row0 + l) * N + x] = v.x; });
+  fft_lines<N, true>(X, tw, C::NL, src2,
+                     [&](int l, int x, float2 v) { force[2 * PLANE + (row0 + l) * N + x] = v.x; });
 }
 
 // ------------------------------------------------------------- gather + KDK
@@ -603,69 +622,259 @@ __device__ __forceinline__ void block_max_to(float v, uint32_t* target) {
   }
 }
 
-template <bool P2>
+constexpr int PM_GU = 1;  // particles in flight per thread in the gather
+
+// CIC gather of the force planes fused with the kicks and the drift (grid
+// stride over the particles in order, so the cell-sorted state streams
+// contiguously); GU particles per thread have their loads in flight at once.
+template <bool P2, int GU>
 __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, float* __restrict__ vel,
-                                                    float* __restrict__ acc, const float4* __restrict__ force,
+                                                    float* __restrict__ acc, const float* __restrict__ force,
                                                     const uint32_t* __restrict__ perm, int64_t n, int G, float h,
                                                     float inv_h, float hdt, float dt, float L, int do_kick, int drift,
                                                     PmDev* __restrict__ st) {
+  const int64_t G3 = (int64_t)G * G * G;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   float vmax = 0.0f;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t pi = perm ? (int64_t)perm[i] : i;
-    float p[3];
-    int i0[3];
-    float w[3];
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += stride * GU) {
+    int64_t pi[GU];
+    float p[GU][3], vv[GU][3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      p[d] = pos[3 * pi + d];
-      const float u = P2 ? cic_u_pow2(p[d], inv_h) : cic_u(p[d], 0.0f, h);
-      const float f = floorf(u);
-      i0[d] = (int)f;
-      w[d] = __fsub_rn(u, f);
-    }
-    const int x0 = wrap(i0[0], G), x1 = wrap(i0[0] + 1, G);
-    const int y0 = wrap(i0[1], G), y1 = wrap(i0[1] + 1, G);
-    const int z0 = wrap(i0[2], G), z1 = wrap(i0[2] + 1, G);
-    float a[3] = {0.f, 0.f, 0.f};
+    for (int u = 0; u < GU; ++u) {
+      const int64_t i = i0 + u * stride;
+      pi[u] = -1;
+      if (i < n) {
+        pi[u] = perm ? (int64_t)perm[i] : i;
 #pragma unroll
-    for (int ox = 0; ox < 2; ++ox)
-#pragma unroll
-      for (int oy = 0; oy < 2; ++oy)
-#pragma unroll
-        for (int oz = 0; oz < 2; ++oz) {
-          const float wx = ox ? w[0] : __fsub_rn(1.0f, w[0]);
-          const float wy = oy ? w[1] : __fsub_rn(1.0f, w[1]);
-          const float wz = oz ? w[2] : __fsub_rn(1.0f, w[2]);
-          const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
-          const int64_t cell = ((int64_t)(oz ? z1 : z0) * G + (oy ? y1 : y0)) * G + (ox ? x1 : x0);
-          const float4 f = force[cell];
-          a[0] += f.x * wt;
-          a[1] += f.y * wt;
-          a[2] += f.z * wt;
+        for (int d = 0; d < 3; ++d) {
+          p[u][d] = pos[3 * pi[u] + d];
+          vv[u][d] = do_kick ? vel[3 * pi[u] + d] : 0.0f;
         }
-    // between steps of one call the acceleration lives only in the velocities
-    // (both half kicks are applied here); it is stored when no drift follows
-    if (!drift) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) acc[3 * pi + d] = a[d];
+      }
     }
-    if (do_kick) {
+    float a[GU][3];
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      a[u][0] = a[u][1] = a[u][2] = 0.0f;
+      if (pi[u] < 0) continue;
+      int i0c[3];
+      float w[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        float v = vel[3 * pi + d] + hdt * a[d];
-        if (drift) {
-          v = v + hdt * a[d];
-          float x = p[d] + dt * v;
-          if (x >= L) x -= L;
-          if (x < 0.0f) x += L;
-          pos[3 * pi + d] = x;
-          vmax = fmaxf(vmax, fabsf(v));
+        const float uu = P2 ? cic_u_pow2(p[u][d], inv_h) : cic_u(p[u][d], 0.0f, h);
+        const float f = floorf(uu);
+        i0c[d] = (int)f;
+        w[d] = __fsub_rn(uu, f);
+      }
+      const int x0 = wrap(i0c[0], G), x1 = wrap(i0c[0] + 1, G);
+      const int y0 = wrap(i0c[1], G), y1 = wrap(i0c[1] + 1, G);
+      const int z0 = wrap(i0c[2], G), z1 = wrap(i0c[2] + 1, G);
+#pragma unroll
+      for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+          for (int oz = 0; oz < 2; ++oz) {
+            const float wx = ox ? w[0] : __fsub_rn(1.0f, w[0]);
+            const float wy = oy ? w[1] : __fsub_rn(1.0f, w[1]);
+            const float wz = oz ? w[2] : __fsub_rn(1.0f, w[2]);
+            const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
+            const int64_t cell = ((int64_t)(oz ? z1 : z0) * G + (oy ? y1 : y0)) * G + (ox ? x1 : x0);
+            a[u][0] += force[cell] * wt;
+            a[u][1] += force[G3 + cell] * wt;
+            a[u][2] += force[2 * G3 + cell] * wt;
+          }
+    }
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      if (pi[u] < 0) continue;
+      const int64_t q = pi[u];
+      // between steps of one call the acceleration lives only in the
+      // velocities (both half kicks are applied here); it is stored when no
+      // drift follows
+      if (!drift) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[3 * q + d] = a[u][d];
+      }
+      if (do_kick) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          float v = vv[u][d] + hdt * a[u][d];
+          if (drift) {
+            v = v + hdt * a[u][d];
+            float x = p[u][d] + dt * v;
+            if (x >= L) x -= L;
+            if (x < 0.0f) x += L;
+            pos[3 * q + d] = x;
+            vmax = fmaxf(vmax, fabsf(v));
+          }
+          vel[3 * q + d] = v;
         }
-        vel[3 * pi + d] = v;
       }
     }
   }
   if (st && drift) block_max_to(vmax, &st->vmax_bits);
+}
+
+// Tile gather + KDK for the sorted (tile-major) stepper state.  A gather
+// piece is TG_SUB whole z layers of a tile (32x16x4 cells), so its particles
+// are one contiguous slot range.  A workgroup stages the force planes over
+// the piece grown by DMAX cells below and 1 + DMAX above (35x19x7 cells: the
+// CIC corners of every particle whose sorted cell is in the piece, for drift
+// up to DMAX cells) in LDS, then streams the piece's particles through the
+// same arithmetic as k_gather_kdk.  Two workgroups per CU.
+constexpr int TG_THREADS = 512;
+constexpr int TG_SUB = 4;  // z layers per piece
+constexpr int TG_NSUB = TD_Z / TG_SUB;
+constexpr uint32_t TG_CHUNK = 8192;  // most particles per work item
+constexpr int TG_U = 2;              // particles per thread in flight
+constexpr int TG_RX = TD_X + 1 + 2 * DMAX, TG_RY = TD_Y + 1 + 2 * DMAX, TG_RZ = TG_SUB + 1 + 2 * DMAX;
+constexpr int TG_REG = TG_RX * TG_RY * TG_RZ;
+
+// sort-time plan: items per piece (one thread each); items[npieces] = 0
+__global__ void k_gather_plan(const uint32_t* __restrict__ cell_start, int npieces, uint32_t* __restrict__ items,
+                              const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pc > npieces) return;
+  if (pc == npieces) {
+    items[npieces] = 0u;
+    return;
+  }
+  constexpr uint32_t PC = TG_SUB * TD_Y * TD_X;  // cells per piece; pieces are consecutive in key order
+  const uint32_t cnt = cell_start[(uint32_t)(pc + 1) * PC] - cell_start[(uint32_t)pc * PC];
+  items[pc] = cnt == 0u ? 1u : (cnt + TG_CHUNK - 1) / TG_CHUNK;
+}
+
+// item -> (piece, slot range): the piece's particles cut into near-equal parts
+__global__ void k_gather_items(const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ item_start,
+                               int npieces, DepItem* __restrict__ info, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pc >= npieces) return;
+  constexpr uint32_t PC = TG_SUB * TD_Y * TD_X;
+  const uint32_t lo = cell_start[(uint32_t)pc * PC], hi = cell_start[(uint32_t)(pc + 1) * PC];
+  const uint32_t i0 = item_start[pc], nch = item_start[pc + 1] - i0;
+  const uint32_t per = (hi - lo + nch - 1) / nch;
+  for (uint32_t c = 0; c < nch; ++c)
+    info[i0 + c] = DepItem{(uint32_t)pc, min(hi, lo + c * per), min(hi, lo + (c + 1) * per), 0u};
+}
+
+template <bool P2>
+__global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
+    float* __restrict__ pos, float* __restrict__ vel, float* __restrict__ acc, const float* __restrict__ force,
+    int64_t n, int G, float h, float inv_h, float hdt, float dt, float L, int drift,
+    const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, int npieces, PmDev* __restrict__ st) {
+  __shared__ float fx[TG_REG], fy[TG_REG], fz[TG_REG];
+  const int tid = threadIdx.x;
+  const int64_t G3 = (int64_t)G * G * G;
+  float vmax = 0.0f;
+  const uint32_t total_items = item_start[npieces];
+  for (uint32_t item = blockIdx.x; item < total_items; item += gridDim.x) {
+    const DepItem it = info[item];
+    const TileOrigin to = tile_origin((int)it.tile / TG_NSUB, G);
+    const int x0 = to.x0, y0 = to.y0, z0 = to.z0 + ((int)it.tile % TG_NSUB) * TG_SUB;
+    // force planes over the grown piece, origin (x0, y0, z0) - DMAX wrapped;
+    // the loads of a batch are issued before its LDS writes
+    {
+      constexpr int PER = (TG_REG + TG_THREADS - 1) / TG_THREADS, BATCH = 4;
+#pragma unroll
+      for (int k0 = 0; k0 < PER; k0 += BATCH) {
+        float r[BATCH][3];
+#pragma unroll
+        for (int k = 0; k < BATCH; ++k) {
+          const int c = tid + (k0 + k) * TG_THREADS;
+          if (k0 + k < PER && c < TG_REG) {
+            const int xx = c % TG_RX, yy = (c / TG_RX) % TG_RY, zz = c / (TG_RX * TG_RY);
+            const int gx = (x0 - DMAX + xx) & (G - 1), gy = (y0 - DMAX + yy) & (G - 1), gz = (z0 - DMAX + zz) & (G - 1);
+            const int64_t cell = ((int64_t)gz * G + gy) * G + gx;
+            r[k][0] = force[cell];
+            r[k][1] = force[G3 + cell];
+            r[k][2] = force[2 * G3 + cell];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; ++k) {
+          const int c = tid + (k0 + k) * TG_THREADS;
+          if (k0 + k < PER && c < TG_REG) {
+            fx[c] = r[k][0];
+            fy[c] = r[k][1];
+            fz[c] = r[k][2];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // the piece's particles: contiguous slots, TG_U per thread in flight
+    for (uint32_t v0 = it.vlo + tid; v0 < it.vhi; v0 += TG_THREADS * TG_U) {
+      float p[TG_U][3], vel0[TG_U][3];
+#pragma unroll
+      for (int u = 0; u < TG_U; ++u) {
+        const uint32_t v = v0 + u * TG_THREADS;
+        if (v < it.vhi) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            p[u][d] = pos[3 * (int64_t)v + d];
+            vel0[u][d] = vel[3 * (int64_t)v + d];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < TG_U; ++u) {
+        const uint32_t v = v0 + u * TG_THREADS;
+        if (v >= it.vhi) continue;
+        float w[3];
+        int lb[3];
+        const int t0[3] = {x0, y0, z0};
+        const int tmax[3] = {TD_X + 2 * DMAX - 1, TD_Y + 2 * DMAX - 1, TG_SUB + 2 * DMAX - 1};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float uu = P2 ? cic_u_pow2(p[u][d], inv_h) : cic_u(p[u][d], 0.0f, h);
+          const float f = floorf(uu);
+          w[d] = __fsub_rn(uu, f);
+          // base cell in region coordinates; the drift bound keeps it in range
+          // (the clamp only guards the LDS bounds)
+          lb[d] = min(((int)f - t0[d] + DMAX) & (G - 1), tmax[d]);
+        }
+        float a[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+          for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 2; ++oz) {
+              const float wx = ox ? w[0] : __fsub_rn(1.0f, w[0]);
+              const float wy = oy ? w[1] : __fsub_rn(1.0f, w[1]);
+              const float wz = oz ? w[2] : __fsub_rn(1.0f, w[2]);
+              const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
+              const int c = ((lb[2] + oz) * TG_RY + lb[1] + oy) * TG_RX + lb[0] + ox;
+              a[0] += fx[c] * wt;
+              a[1] += fy[c] * wt;
+              a[2] += fz[c] * wt;
+            }
+        const int64_t q = v;
+        if (!drift) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) acc[3 * q + d] = a[d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          float vv = vel0[u][d] + hdt * a[d];
+          if (drift) {
+            vv = vv + hdt * a[d];
+            float x = p[u][d] + dt * vv;
+            if (x >= L) x -= L;
+            if (x < 0.0f) x += L;
+            pos[3 * q + d] = x;
+            vmax = fmaxf(vmax, fabsf(vv));
+          }
+          vel[3 * q + d] = vv;
+        }
+      }
+    }
+    __syncthreads();  // LDS is reused by the next item
+  }
+  if (drift) block_max_to(vmax, &st->vmax_bits);
 }
 
 __global__ void k_kick_drift(float* __restrict__ pos, float* __restrict__ vel, const float* __restrict__ acc, int64_t n,
@@ -703,7 +912,7 @@ struct PmBufs {
   float2* s0;            // G*G*nxc complex (spectrum / A / out0)
   float2* s1;            // out1
   float2* s2;            // B / out2
-  float4* force;         // G^3 float4
+  float* force;          // 3 G^3: planes a_x, a_y, a_z
   uint32_t* keys;        // n
   uint32_t* vals;        // n
   void* rws;             // radix sort scratch
@@ -716,12 +925,17 @@ struct PmBufs {
   uint32_t* tseg_pre;    // ntiles * (TD_MAXSEG + 1): inclusive prefix of their lengths
   DepItem* info;         // max_items: tile and visit range of each work item
   uint16_t* iblk;        // max_items * TD_NBLK: visit -> segment table of each item
+  // gather plan: work items over the pieces (TG_SUB z layers of a tile)
+  uint32_t* gitems;      // npieces + 1
+  DepItem* ginfo;        // max_gitems
 };
 
 static int64_t pm_tiles(int64_t G) { return (G / TD_X) * (G / TD_Y) * (G / TD_Z); }
+static int64_t pm_pieces(int64_t G) { return pm_tiles(G) * TG_NSUB; }
 // a cell lies in at most 2 tile regions per axis, so the planned visits are
 // at most 8n and every tile has at least one item
 static int64_t pm_max_items(int64_t n, int64_t G) { return pm_tiles(G) + (8 * n + TD_CHUNK - 1) / TD_CHUNK; }
+static int64_t pm_max_gitems(int64_t n, int64_t G) { return pm_pieces(G) + (n + TG_CHUNK - 1) / TG_CHUNK; }
 
 static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   const int64_t G3 = G * G * G, nc = G * G * (G / 2 + 1);
@@ -731,7 +945,7 @@ static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   b.s0 = c.take<float2>(nc);
   b.s1 = c.take<float2>(nc);
   b.s2 = c.take<float2>(nc);
-  b.force = c.take<float4>(G3);
+  b.force = c.take<float>(3 * G3);
   b.keys = c.take<uint32_t>(n);
   b.vals = c.take<uint32_t>(n);
   b.rws = c.take<char>(radix_workspace_bytes(n));
@@ -744,6 +958,8 @@ static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   b.tseg_pre = c.take<uint32_t>(pm_tiles(G) * (TD_MAXSEG + 1));
   b.info = c.take<DepItem>(pm_max_items(n, G));
   b.iblk = c.take<uint16_t>(pm_max_items(n, G) * TD_NBLK);
+  b.gitems = c.take<uint32_t>(pm_pieces(G) + 1);
+  b.ginfo = c.take<DepItem>(pm_max_gitems(n, G));
   return c.ok();
 }
 
@@ -785,16 +1001,27 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   AUGUR_LAUNCHED("count_cells");
   rc = exclusive_scan_u32(b.cell_start, b.cell_start, G3 + 1, b.scan_ws, s, gate);
   if (rc) return rc;
-  // deposit work items of each tile for the largest drift bound
+  // work items: the deposit's over the tile regions grown for the largest
+  // drift bound, the gather's over the tile cores
   const int64_t ntiles = pm_tiles(G);
-  hipLaunchKernelGGL(k_dep_plan, dim3((unsigned)((ntiles + 1 + 3) / 4)), dim3(256), 0, s,
-                     (const uint32_t*)b.cell_start, (int)G, (int)ntiles, b.items, b.tseg_lo, b.tseg_pre, gate);
+  const dim3 pg((unsigned)((ntiles + 1 + 3) / 4));
+  hipLaunchKernelGGL(k_dep_plan, pg, dim3(256), 0, s, (const uint32_t*)b.cell_start, (int)G, (int)ntiles, b.items,
+                     b.tseg_lo, b.tseg_pre, gate);
   AUGUR_LAUNCHED("dep_plan");
   rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, gate);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_dep_items, dim3((unsigned)pm_max_items(n, G)), dim3(256), 0, s, (const uint32_t*)b.items,
-                     (int)ntiles, (const uint32_t*)b.tseg_pre, b.info, b.iblk, gate);
+  hipLaunchKernelGGL((k_tile_items<TD_MAXSEG, TD_CHUNK>), dim3((unsigned)pm_max_items(n, G)), dim3(256), 0, s,
+                     (const uint32_t*)b.items, (int)ntiles, (const uint32_t*)b.tseg_pre, b.info, b.iblk, gate);
   AUGUR_LAUNCHED("dep_items");
+  const int64_t npc = pm_pieces(G);
+  hipLaunchKernelGGL(k_gather_plan, dim3(blocks_for(npc + 1, 256)), dim3(256), 0, s, (const uint32_t*)b.cell_start,
+                     (int)npc, b.gitems, gate);
+  AUGUR_LAUNCHED("gather_plan");
+  rc = exclusive_scan_u32(b.gitems, b.gitems, npc + 1, b.scan_ws, s, gate);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_gather_items, dim3(blocks_for(npc, 256)), dim3(256), 0, s, (const uint32_t*)b.cell_start,
+                     (const uint32_t*)b.gitems, (int)npc, b.ginfo, gate);
+  AUGUR_LAUNCHED("gather_items");
   if (!physical) {
     hipLaunchKernelGGL(k_copy<uint32_t>, g, bl, 0, s, perm, b.perm, n, gate);
     AUGUR_LAUNCHED("copy_perm");
@@ -878,7 +1105,7 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
   return AUGUR_OK;
 }
 
-// deposit -> FFT Poisson + spectral gradient -> float4 force grid
+// deposit -> FFT Poisson + spectral gradient -> force planes
 static int pm_force_grid(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
                          int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
   int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s);
@@ -952,12 +1179,12 @@ int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_v
   const float h = (float)((double)L / (double)G);
   TimedScope ts_("gather", s);
   if (pow2_float(h))
-    hipLaunchKernelGGL(k_gather_kdk<true>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos,
-                       (float*)nullptr, acc, (const float4*)b.force, (const uint32_t*)b.perm, n, (int)G, h, 1.0f / h,
+    hipLaunchKernelGGL((k_gather_kdk<true, PM_GU>), dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos,
+                       (float*)nullptr, acc, (const float*)b.force, (const uint32_t*)b.perm, n, (int)G, h, 1.0f / h,
                        0.0f, 0.0f, L, 0, 0, (PmDev*)nullptr);
   else
-    hipLaunchKernelGGL(k_gather_kdk<false>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos,
-                       (float*)nullptr, acc, (const float4*)b.force, (const uint32_t*)b.perm, n, (int)G, h, 0.0f,
+    hipLaunchKernelGGL((k_gather_kdk<false, PM_GU>), dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, (float*)pos,
+                       (float*)nullptr, acc, (const float*)b.force, (const uint32_t*)b.perm, n, (int)G, h, 0.0f,
                        0.0f, 0.0f, L, 0, 0, (PmDev*)nullptr);
   AUGUR_LAUNCHED("gather");
   return AUGUR_OK;
@@ -994,6 +1221,7 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, 
   const float h = (float)((double)L / (double)G);
   const float hdt = 0.5f * dt;
   const bool p2 = pow2_float(h);
+  const int64_t npc = pm_pieces(G);
   {
     TimedScope ts_("kick_drift", s);
     hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc, n, hdt, dt, L, b.st);
@@ -1012,14 +1240,15 @@ int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, 
     if (rc) return rc;
     const int last = st == nsteps - 1;
     TimedScope ts_("gather_kdk", s);
+    const dim3 gg((unsigned)pm_max_gitems(n, G));
     if (p2)
-      hipLaunchKernelGGL(k_gather_kdk<true>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc,
-                         (const float4*)b.force, (const uint32_t*)nullptr, n, (int)G, h, 1.0f / h, hdt, dt, L, 1,
-                         last ? 0 : 1, b.st);
+      hipLaunchKernelGGL(k_gather_tiles<true>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
+                         (int)G, h, 1.0f / h, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
+                         (const DepItem*)b.ginfo, (int)npc, b.st);
     else
-      hipLaunchKernelGGL(k_gather_kdk<false>, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc,
-                         (const float4*)b.force, (const uint32_t*)nullptr, n, (int)G, h, 0.0f, hdt, dt, L, 1,
-                         last ? 0 : 1, b.st);
+      hipLaunchKernelGGL(k_gather_tiles<false>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
+                         (int)G, h, 0.0f, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
+                         (const DepItem*)b.ginfo, (int)npc, b.st);
     AUGUR_LAUNCHED("gather_kdk");
   }
   return AUGUR_OK;
