@@ -46,6 +46,21 @@ STAGES = {  # stage -> kernels (per-step algorithmic bytes are the sum of theirs
 }
 
 
+def measured_traffic(kernel, N, G):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by what the
+    same counters read for a copy of known size), or None when absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"pm_traffic_n{N}_g{G}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) >= 8 and w[0] == "kernel" and w[1] == kernel and w[6] == "traffic_bytes":
+                return float(w[7])
+    return None
+
+
 def cpu_baseline(pos, vel, G, dt, steps=1):
     """Time the PyTorch-CPU FP32 port of one KDK step on the full workload."""
     import torch
@@ -77,6 +92,9 @@ def main():
     ap.add_argument("--targets", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, run one 256 MiB device-to-device copy (the byte-counter "
+                         "calibration kernel of a rocprofv3 --pmc run)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -124,14 +142,23 @@ def main():
     print(f"[bench] {a.steps} steps in {sec * 1e3:.2f} ms -> {value:.4e} particle-steps/s", file=sys.stderr,
           flush=True)
 
+    if a.calibrate_copy:
+        src = torch.ones(1 << 26, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 268435456 bytes read, 268435456 bytes written", file=sys.stderr, flush=True)
+        del src, dst
+
     # dominant kernel and per-stage rates
     dom = max(timing.items(), key=lambda kv: kv[1][0])
     dname, (dms, dcnt) = dom
     avg_s = dms / dcnt / 1e3
     ab = algorithmic_bytes(dname, N, G)
     achieved = ab / avg_s / 1e9 if ab else None
+    traffic = measured_traffic(dname, N, G)
     roofline = {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                 "avg_launch_us": avg_s * 1e6, "algorithmic_bytes": ab}
     stages = {}
     for sname, kern in STAGES.items():

@@ -186,15 +186,18 @@ int augur_reduce_energy_momentum_f64(const float* vel, const float* mass, float 
  *   NULL (= mass_value).  The workspace also carries the stepper's device
  *   bookkeeping (cell offsets of the last sort, drift bound): use one
  *   workspace per particle state.
- * augur_pm_prepare(): stable sort of the state by cell (physical reorder of
- *   pos, vel, acc, mass, ids) and the cell offsets of the deposit.
+ * augur_pm_prepare(): stable sort of the state by CIC base cell in tile-major
+ *   order (32x16x16-cell tiles x-fastest, cells x-fastest inside a tile;
+ *   physical reorder of pos, vel, acc, mass, ids), the cell offsets and the
+ *   deposit / gather work plans.
  * augur_pm_accel(): acc = PM force at pos in any order (index sort by cell,
  *   tiled deposit -> FFT Poisson + spectral gradient -> CIC gather); uses
  *   the workspace as scratch (it resets the stepper bookkeeping).
  * augur_pm_kdk(): nsteps of KDK leapfrog with the fused gather + kick + drift:
  *   v += dt/2 a; x += dt v (wrapped); a = PM(x); v += dt/2 a.  The state is
- *   re-sorted on the device (pos, vel, acc, mass, ids permuted together)
- *   whenever the accumulated drift since the last sort could exceed one cell.
+ *   re-sorted on the device (pos, vel, mass, ids permuted together) whenever
+ *   the accumulated drift since the last sort could exceed one cell; acc
+ *   holds the acceleration at the final positions on return.
  * Together these replace ScatterField._step (verlet; scripts/scatter_field.py:
  * 677-688), pm_acc (scripts/pm_baseline_force_error.py:68-76) and
  * gravity_sim.rollout's KDK (scripts/gravity_sim.py:31-44) for 3D periodic PM. */
