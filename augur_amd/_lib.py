@@ -47,8 +47,13 @@ SIGNATURES = {
     "augur_pm_prepare": (c_int, [P, P, P, P, P, c_int64, c_int64, c_float, P, c_size_t, P]),
     "augur_pm_accel": (c_int, [P, P, P, c_float, c_int64, c_int64, c_float, c_float, P, c_size_t, P]),
     "augur_pm_density": (c_int, [P, P, c_float, c_int64, c_int64, c_float, P, P, c_size_t, P]),
+    "augur_pm_fields": (c_int, [P, P, c_float, c_int64, c_int64, c_float, c_float, P, P, P, c_size_t, P]),
     "augur_pm_kdk": (c_int, [P, P, P, P, P, c_float, c_int64, c_int64, c_float, c_float, c_float, c_int, P,
                              c_size_t, P]),
+    "augur_pm_step_begin": (c_int, [P, P, P, P, P, c_float, c_int64, c_int64, c_float, c_float, c_float, c_int, P,
+                                    c_size_t, P]),
+    "augur_pm_step_end": (c_int, [P, P, P, c_int64, c_int64, c_float, c_float, c_int, P, c_size_t, P]),
+    "augur_pm_grid_offsets": (c_int, [c_int64, c_int64, P, P]),
     "augur_conv3_bf16_fwd": (c_int, [P, c_int, P, P, c_int64, c_int64, c_int, c_int, P, P, P, P, P]),
     "augur_conv3_wgrad_workspace_bytes": (c_size_t, [c_int, c_int64, c_int64, c_int]),
     "augur_conv3_bf16_wgrad": (c_int, [P, c_int, P, c_int64, c_int64, P, P, c_int, P, c_size_t, P]),

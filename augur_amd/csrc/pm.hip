@@ -1190,6 +1190,25 @@ int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_v
   return AUGUR_OK;
 }
 
+int augur_pm_fields(const float* pos, const float* mass, float mass_value, int64_t n, int64_t G, float L, float Gconst,
+                    float* rho, float* force, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_pm(n, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(pos && rho && force, AUGUR_EINVAL, "pm_fields: null pointer");
+  hipStream_t s = as_stream(stream);
+  Carver c(ws, ws_bytes);
+  PmBufs b;
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_fields: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
+  rc = pm_init(b, G, s);
+  if (rc) return rc;
+  rc = pm_sort((float*)pos, nullptr, nullptr, nullptr, nullptr, n, G, L, b, nullptr, false, s);
+  if (rc) return rc;
+  b.rho = rho;      // the caller's grids receive the density and the force planes
+  b.force = force;
+  return pm_force_grid(pos, b.perm, mass, mass_value, n, G, L, Gconst, b, s);
+}
+
 int augur_pm_density(const float* pos, const float* mass, float mass_value, int64_t n, int64_t G, float L, float* rho,
                      void* ws, size_t ws_bytes, void* stream) {
   int rc = check_pm(n, G, L);
@@ -1208,48 +1227,85 @@ int augur_pm_density(const float* pos, const float* mass, float mass_value, int6
   return pm_deposit(pos, b.perm, mass, mass_value, n, G, L, b, s);
 }
 
-int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n, int64_t G,
-                 float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream) {
+int augur_pm_step_begin(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n,
+                        int64_t G, float L, float Gconst, float dt, int first, void* ws, size_t ws_bytes,
+                        void* stream) {
   int rc = check_pm(n, G, L);
   if (rc) return rc;
-  AUGUR_REQUIRE(pos && vel && acc && nsteps >= 0, AUGUR_EINVAL, "pm_kdk: bad arguments");
-  if (nsteps == 0) return AUGUR_OK;
+  AUGUR_REQUIRE(pos && vel && acc, AUGUR_EINVAL, "pm_step_begin: bad arguments");
   hipStream_t s = as_stream(stream);
   Carver c(ws, ws_bytes);
   PmBufs b;
-  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_kdk: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_step_begin: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
   const float h = (float)((double)L / (double)G);
-  const float hdt = 0.5f * dt;
-  const bool p2 = pow2_float(h);
-  const int64_t npc = pm_pieces(G);
-  {
+  if (first) {  // opening half kick and drift from the stored acceleration
     TimedScope ts_("kick_drift", s);
-    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc, n, hdt, dt, L, b.st);
+    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, pos, vel, acc, n, 0.5f * dt, dt,
+                       L, b.st);
     AUGUR_LAUNCHED("kick_drift");
   }
+  hipLaunchKernelGGL(k_pm_decide, dim3(1), dim3(64), 0, s, b.st, dt / h, 0);
+  AUGUR_LAUNCHED("pm_decide");
+  // the state arrays (ids included) are permuted in place when the gate
+  // opens; acc is not (the last step rewrites all of it in the final order)
+  rc = pm_sort(pos, vel, nullptr, mass, ids, n, G, L, b, &b.st->need_sort, true, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st);
+  AUGUR_LAUNCHED("pm_sorted");
+  return pm_force_grid(pos, nullptr, mass, mass_value, n, G, L, Gconst, b, s);
+}
+
+int augur_pm_step_end(float* pos, float* vel, float* acc, int64_t n, int64_t G, float L, float dt, int last, void* ws,
+                      size_t ws_bytes, void* stream) {
+  int rc = check_pm(n, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(pos && vel && acc, AUGUR_EINVAL, "pm_step_end: bad arguments");
+  hipStream_t s = as_stream(stream);
+  Carver c(ws, ws_bytes);
+  PmBufs b;
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_step_end: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
+  const float h = (float)((double)L / (double)G);
+  const float hdt = 0.5f * dt;
+  const int64_t npc = pm_pieces(G);
+  TimedScope ts_("gather_kdk", s);
+  const dim3 gg((unsigned)pm_max_gitems(n, G));
+  if (pow2_float(h))
+    hipLaunchKernelGGL(k_gather_tiles<true>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
+                       (int)G, h, 1.0f / h, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
+                       (const DepItem*)b.ginfo, (int)npc, b.st);
+  else
+    hipLaunchKernelGGL(k_gather_tiles<false>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
+                       (int)G, h, 0.0f, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
+                       (const DepItem*)b.ginfo, (int)npc, b.st);
+  AUGUR_LAUNCHED("gather_kdk");
+  return AUGUR_OK;
+}
+
+int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* force_offset) {
+  AUGUR_REQUIRE(rho_offset && force_offset, AUGUR_EINVAL, "pm_grid_offsets: null pointer");
+  // carve against a stand-in base address (never dereferenced) to read offsets
+  const uintptr_t base = 4096;
+  Carver c(reinterpret_cast<void*>(base), SIZE_MAX);
+  PmBufs b;
+  carve_pm(c, n, G, b);
+  *rho_offset = (size_t)(reinterpret_cast<uintptr_t>(b.rho) - base);
+  *force_offset = (size_t)(reinterpret_cast<uintptr_t>(b.force) - base);
+  return AUGUR_OK;
+}
+
+int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n, int64_t G,
+                 float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream) {
+  AUGUR_REQUIRE(nsteps >= 0, AUGUR_EINVAL, "pm_kdk: bad arguments");
+  int rc = check_pm(n, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(pos && vel && acc, AUGUR_EINVAL, "pm_kdk: bad arguments");
   for (int st = 0; st < nsteps; ++st) {
-    hipLaunchKernelGGL(k_pm_decide, dim3(1), dim3(64), 0, s, b.st, dt / h, 0);
-    AUGUR_LAUNCHED("pm_decide");
-    // the state arrays (ids included) are permuted in place when the gate
-    // opens; acc is not (the last step rewrites all of it in the final order)
-    rc = pm_sort(pos, vel, nullptr, mass, ids, n, G, L, b, &b.st->need_sort, true, s);
+    rc = augur_pm_step_begin(pos, vel, acc, mass, ids, mass_value, n, G, L, Gconst, dt, st == 0, ws, ws_bytes, stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st);
-    AUGUR_LAUNCHED("pm_sorted");
-    rc = pm_force_grid(pos, nullptr, mass, mass_value, n, G, L, Gconst, b, s);
+    rc = augur_pm_step_end(pos, vel, acc, n, G, L, dt, st == nsteps - 1, ws, ws_bytes, stream);
     if (rc) return rc;
-    const int last = st == nsteps - 1;
-    TimedScope ts_("gather_kdk", s);
-    const dim3 gg((unsigned)pm_max_gitems(n, G));
-    if (p2)
-      hipLaunchKernelGGL(k_gather_tiles<true>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
-                         (int)G, h, 1.0f / h, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
-                         (const DepItem*)b.ginfo, (int)npc, b.st);
-    else
-      hipLaunchKernelGGL(k_gather_tiles<false>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
-                         (int)G, h, 0.0f, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
-                         (const DepItem*)b.ginfo, (int)npc, b.st);
-    AUGUR_LAUNCHED("gather_kdk");
   }
   return AUGUR_OK;
 }

@@ -1,7 +1,8 @@
 """Plain periodic particle-mesh gravity on one MI355X.
 
 ``PeriodicPM`` runs the whole step through the HIP library: particles held
-sorted by CIC base cell, a tiled deterministic fixed-point deposit, the FFT
+sorted by CIC base cell (tile-major order), a tiled deterministic fixed-point
+deposit, the FFT
 Poisson solve with Green's function -4 pi G / k^2 and spectral gradient, and
 a CIC gather fused with the KDK kick and drift.  The state is re-sorted on the
 device whenever the accumulated drift could move a particle by more than one
@@ -12,6 +13,8 @@ It replaces, for the 3D periodic problem, the reference's pm_acc
 (scripts/gravity_sim.py:31-44, ScatterField._step verlet at
 scripts/scatter_field.py:677-688).
 """
+import ctypes
+
 import torch
 
 from . import _lib
@@ -91,6 +94,31 @@ class PeriodicPM:
                   _stream(self.device))
         return st
 
+    def step_begin(self, st, dt, first):
+        """First half of one KDK step: (with `first`) the opening half kick and
+        drift, the gated re-sort, the deposit and the FFT solve; the density
+        and force planes are then in grids(st)."""
+        _lib.call("augur_pm_step_begin", _p(st.pos), _p(st.vel), _p(st.acc), _p(st.mass), _p(st.ids), st.mass_value,
+                  st.n, self.G, self.L, self.Gconst, float(dt), int(bool(first)), _p(st.ws), st.ws.numel(),
+                  _stream(self.device))
+
+    def step_end(self, st, dt, last):
+        """Second half: gather of grids(st)'s force planes with the kicks (and,
+        unless `last`, the next drift)."""
+        _lib.call("augur_pm_step_end", _p(st.pos), _p(st.vel), _p(st.acc), st.n, self.G, self.L, float(dt),
+                  int(bool(last)), _p(st.ws), st.ws.numel(), _stream(self.device))
+
+    def grids(self, st):
+        """Views into the state's workspace: density (G, G, G) and force
+        planes (3, G, G, G), valid between step_begin and step_end."""
+        ro, fo = ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.call("augur_pm_grid_offsets", st.n, self.G, ctypes.byref(ro), ctypes.byref(fo))
+        G3 = self.G ** 3
+        base = st.ws.view(torch.uint8)
+        rho = base[ro.value: ro.value + 4 * G3].view(torch.float32).view(self.G, self.G, self.G)
+        force = base[fo.value: fo.value + 12 * G3].view(torch.float32).view(3, self.G, self.G, self.G)
+        return rho, force
+
     def unload(self, st):
         """(pos, vel, acc) in the original particle order."""
         idx = st.ids.long()
@@ -125,6 +153,20 @@ class PeriodicPM:
         _lib.call("augur_pm_density", _p(pos), _p(m), mv, n, self.G, self.L, _p(rho), _p(ws), ws.numel(),
                   _stream(self.device))
         return rho
+
+    def fields(self, pos, mass=None, mass_value=None):
+        """PM grids of particles in any order: density (G, G, G) and force
+        (3, G, G, G), the acceleration field the gather interpolates."""
+        pos = pos.detach().to(self.device, torch.float32).reshape(-1, 3).contiguous()
+        _dev(pos)
+        n = pos.shape[0]
+        m, mv = _mass_args(mass, mass_value, n, self.device)
+        rho = torch.empty(self.G, self.G, self.G, dtype=torch.float32, device=self.device)
+        force = torch.empty(3, self.G, self.G, self.G, dtype=torch.float32, device=self.device)
+        ws = self._scratch_ws(n)
+        _lib.call("augur_pm_fields", _p(pos), _p(m), mv, n, self.G, self.L, self.Gconst, _p(rho), _p(force), _p(ws),
+                  ws.numel(), _stream(self.device))
+        return rho, force
 
     def rollout(self, pos, vel, mass=None, dt=1e-3, steps=1):
         """Load, run `steps` KDK steps, unload: returns (pos, vel, acc)."""

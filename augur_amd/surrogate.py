@@ -214,3 +214,147 @@ class LocalNet3D:
             adam_update(w, dw.contiguous(), mw, vw, lr, self.step_count, gscale=gscale)
             adam_update(b, gb, mb, vb, lr, self.step_count, gscale=gscale)
         self.sync_bf16()
+
+
+# --------------------------------------------------------------- PM surrogate
+def rel_rms(a, b):
+    """sqrt(sum |a - b|^2 / sum |b|^2) over the rows (the force metric)."""
+    a, b = a.double(), b.double()
+    return float(((a - b) ** 2).sum().sqrt() / (b ** 2).sum().sqrt().clamp_min(1e-300))
+
+
+class PMSurrogate:
+    """The scatter-field CNN on top of 3D periodic PM (the grid half of
+    ScatterField._grid_force, scripts/scatter_field.py:620-666, with the
+    LocalNet of :15-26).
+
+    mode "correction": input channels [density contrast, PM force (3)], the
+        network predicts a correction added to the PM force field; trained
+        against FP64 (Ewald) forces it removes part of PM's sub-grid error.
+    mode "replace": input the density contrast alone, the network predicts
+        the force field itself (the inference architecture of a pure CNN
+        surrogate).
+
+    accel() gathers the field at the particles.  With ``probe`` indices it
+    also gathers the plain PM force there and falls back to plain PM for all
+    particles when the rel-RMS residual between the two exceeds ``threshold``.
+    """
+
+    def __init__(self, G, L=1.0, Gconst=1.0, device="cuda", mode="correction", layers=5, threshold=None,
+                 seed=4738, init_std=None):
+        from .pm import PeriodicPM
+        assert mode in ("correction", "replace")
+        self.mode = mode
+        self.pm = PeriodicPM(G, L, Gconst, device)
+        self.G, self.L, self.h = self.pm.G, self.pm.L, self.pm.h
+        self.device = self.pm.device
+        cin = 4 if mode == "correction" else 1
+        self.net = LocalNet3D(cin=cin, cout=3, layers=layers, device=self.device, seed=seed, init_std=init_std)
+        # residual budget against plain PM: a replacement must stay close to PM;
+        # a correction is expected to move forces by about PM's own error
+        self.threshold = threshold if threshold is not None else (0.25 if mode == "correction" else 2e-2)
+        self.rho_scale = None    # 1 / rms of the density contrast
+        self.force_scale = None  # rms of the PM force field
+
+    def state_dict(self):
+        return {"mode": self.mode, "net": self.net.state_dict(), "rho_scale": self.rho_scale,
+                "force_scale": self.force_scale, "threshold": self.threshold}
+
+    def load_state_dict(self, sd):
+        self.net.load_state_dict(sd["net"])
+        self.rho_scale, self.force_scale = sd["rho_scale"], sd["force_scale"]
+        self.threshold = sd["threshold"]
+
+    def calibrate(self, rho, force):
+        """Input/output scales from one scene's grids (kept for later scenes)."""
+        contrast = rho / rho.mean() - 1.0
+        self.rho_scale = float(1.0 / contrast.pow(2).mean().sqrt().clamp_min(1e-6))
+        self.force_scale = float(force.pow(2).mean().sqrt().clamp_min(1e-30))
+
+    def features(self, pos, mass=None, mass_value=None):
+        """(network input (1, C, G, G, G), PM force field (3, G, G, G))."""
+        rho, force = self.pm.fields(pos, mass, mass_value)
+        return self.features_from_grids(rho, force), force
+
+    def features_from_grids(self, rho, force):
+        if self.rho_scale is None:
+            self.calibrate(rho, force)
+        contrast = (rho / rho.mean() - 1.0) * self.rho_scale
+        if self.mode == "correction":
+            return torch.cat([contrast[None], force / self.force_scale], 0)[None].contiguous()
+        return contrast[None, None].contiguous()
+
+    def field(self, pos, mass=None, mass_value=None, save=False):
+        """(surrogate force field (3, G, G, G), PM force field)."""
+        x, force = self.features(pos, mass, mass_value)
+        y = self.net.forward(x, save=save)[0] * self.force_scale
+        return (force + y if self.mode == "correction" else y), force
+
+    def gather(self, field, pos):
+        return ops.cic_gather(field[None], pos.reshape(1, -1, 3), self.h, 0.0, True)[0]
+
+    def accel(self, pos, mass=None, mass_value=None, probe=None):
+        """Accelerations (n, 3); with probe indices also (fallback?, residual)."""
+        pos = pos.detach().to(self.device, torch.float32).reshape(-1, 3).contiguous()
+        f, force = self.field(pos, mass, mass_value)
+        a = self.gather(f, pos)
+        if probe is None:
+            return a
+        a_pm = self.gather(force, pos[probe])
+        res = rel_rms(a[probe], a_pm)
+        if res > self.threshold:
+            return self.gather(force, pos), True, res
+        return a, False, res
+
+    # ---------------------------------------------------------- stepping
+    def load(self, pos, vel=None, mass=None, mass_value=None):
+        """Sorted device state of a scene (PeriodicPM.load; acc from plain PM)."""
+        return self.pm.load(pos, vel, mass, mass_value)
+
+    def kdk(self, st, dt, steps, probe=None):
+        """`steps` KDK steps of the state with the surrogate force: each step
+        runs the PM half (re-sort, deposit, FFT) into the state's workspace,
+        the network on those grids, then the fused gather and kicks.  With
+        `probe` slot indices, the first step compares the surrogate with plain
+        PM at those particles and, past the threshold, the whole call runs
+        plain PM.  Returns (fell_back, residual or None)."""
+        fell_back, res = False, None
+        for s in range(steps):
+            self.pm.step_begin(st, dt, s == 0)
+            rho, force = self.pm.grids(st)
+            if not fell_back:
+                y = self.net.forward(self.features_from_grids(rho, force))[0] * self.force_scale
+                if s == 0 and probe is not None:
+                    pp = st.pos[probe].contiguous()
+                    a_pm = self.gather(force, pp)
+                    a_sur = self.gather(force + y if self.mode == "correction" else y, pp)
+                    res = rel_rms(a_sur, a_pm)
+                    fell_back = res > self.threshold
+                if not fell_back:
+                    if self.mode == "correction":
+                        force.add_(y)
+                    else:
+                        force.copy_(y)
+            self.pm.step_end(st, dt, s == steps - 1)
+        return fell_back, res
+
+    def train_step(self, pos, targets, a_true, mass=None, mass_value=None, lr=3e-4):
+        """One Adam step on the force loss sum |a - a_true|^2 / sum |a_true|^2
+        at the target particles.  The gradient reaches the grid through the
+        adjoint of the CIC gather (a CIC deposit of dL/da) and the network
+        through its own backward kernels.  Returns the loss."""
+        pos = pos.detach().to(self.device, torch.float32).reshape(-1, 3).contiguous()
+        f, _ = self.field(pos, mass, mass_value, save=True)
+        pt = pos[targets].contiguous()
+        a = self.gather(f, pt).double()
+        at = a_true.to(self.device, torch.float64)
+        norm = float((at ** 2).sum())
+        r = a - at
+        loss = float((r ** 2).sum()) / norm
+        g = (2.0 / norm) * r.float()
+        dfield = ops.cic_deposit(pt[None], None, self.G, self.h, 0.0, True, vel=g[None], scale=1.0,
+                                 mass_value=1.0)[0, 1:4]
+        dy = (dfield * self.force_scale)[None].contiguous()
+        grads = self.net.backward(dy)
+        self.net.adam_step(grads, lr=lr)
+        return loss

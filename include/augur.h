@@ -211,8 +211,33 @@ int augur_pm_accel(const float* pos, float* acc, const float* mass, float mass_v
  * Replaces cic_density (model/tiered/energy.py:9-34) for the periodic cube. */
 int augur_pm_density(const float* pos, const float* mass, float mass_value, int64_t n, int64_t G, float L, float* rho,
                      void* ws, size_t ws_bytes, void* stream);
+/* The PM grids of particles in any order: rho (G^3 FP32 mass density) and
+ * force (3 x G^3 FP32 planes a_x, a_y, a_z), as computed inside
+ * augur_pm_accel (scratch workspace as there).  Replaces the grid half of
+ * ScatterField.force (scripts/scatter_field.py:620-666: scatter + kernel_acc
+ * before the gather) for 3D periodic PM; the inputs of the surrogate. */
+int augur_pm_fields(const float* pos, const float* mass, float mass_value, int64_t n, int64_t G, float L, float Gconst,
+                    float* rho, float* force, void* ws, size_t ws_bytes, void* stream);
 int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n, int64_t G,
                  float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream);
+/* One KDK step of augur_pm_kdk in two halves, so a caller can change the
+ * force field between them (the surrogate's correction):
+ * augur_pm_step_begin(): with first != 0 the opening half kick and drift from
+ *   acc; then the gated re-sort, the deposit and the FFT solve, leaving the
+ *   density and the force planes in the workspace at the offsets
+ *   augur_pm_grid_offsets() reports (rho: G^3 FP32; force: 3 x G^3 FP32).
+ * augur_pm_step_end(): the fused gather of the workspace force planes with
+ *   the closing half kick and, unless last != 0, the next opening half kick
+ *   and drift (acc is written on the last step).
+ * Replace the force evaluation and update of ScatterField._step (verlet,
+ * scripts/scatter_field.py:677-688) when the field comes from _grid_force
+ * (:620-666) with a network applied to the PM grids. */
+int augur_pm_step_begin(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n,
+                        int64_t G, float L, float Gconst, float dt, int first, void* ws, size_t ws_bytes,
+                        void* stream);
+int augur_pm_step_end(float* pos, float* vel, float* acc, int64_t n, int64_t G, float L, float dt, int last, void* ws,
+                      size_t ws_bytes, void* stream);
+int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* force_offset);
 
 /* ------------------------------------------ a11 / a12 surrogate network */
 
