@@ -46,6 +46,90 @@ STAGES = {  # stage -> kernels (per-step algorithmic bytes are the sum of theirs
 }
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
+    """Train the PM correction network on the GPU (own conv kernels, Adam)
+    on scenes of the same mixture (other seeds) against FP64 Ewald forces,
+    state its force error on the workload's own targets, and time `a.steps`
+    surrogate KDK steps of the workload (probe residual check included)."""
+    import numpy as np
+    import torch
+    from augur_amd import _lib, ops, scenes
+    from augur_amd.surrogate import PMSurrogate, rel_rms
+
+    sur = PMSurrogate(G, 1.0, 1.0, dev, mode="correction")
+    t0 = time.perf_counter()
+    train = []
+    for k in range(a.sur_scenes):
+        p, _ = scenes.cfg2(seed=1000 + k, n=N)
+        pos = torch.from_numpy(p).to(dev)
+        rng = np.random.default_rng(2000 + k)
+        tgt = torch.from_numpy(np.sort(rng.choice(N, a.sur_targets, replace=False))).to(dev)
+        train.append((pos, tgt, ops.ewald_f64(pos, 1.0 / N, tgt, L=1.0, eps=1.0 / G)))
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = [sur.train_step(*train[i % len(train)], mass_value=1.0 / N, lr=a.sur_lr) for i in range(a.sur_train_steps)]
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del train
+    print(f"[bench] surrogate trained: {a.sur_train_steps} steps on {a.sur_scenes} scenes in {t_train:.1f}s "
+          f"(data {t_data:.1f}s), loss {losses[0]:.3e} -> {losses[-1]:.3e}", file=sys.stderr, flush=True)
+
+    pos0 = torch.from_numpy(pos_np).to(dev)
+    rel_sur = None
+    if oracle is not None:
+        tg, a_ex = oracle
+        f, _ = sur.field(pos0, mass_value=1.0 / N)
+        rel_sur = rel_rms(sur.gather(f, pos0[tg].contiguous()), a_ex)
+        del f
+        print(f"[bench] surrogate force rel-RMS vs FP64 Ewald: {rel_sur:.4e} (plain PM {pm_rel:.4e})",
+              file=sys.stderr, flush=True)
+
+    st = sur.load(pos0, torch.from_numpy(vel_np).to(dev), mass_value=1.0 / N)
+    probe = torch.arange(0, N, max(1, N // 4096), device=dev)
+    sur.kdk(st, a.dt, max(1, a.warmup), probe=probe)
+    torch.cuda.synchronize()
+    _lib.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fell_back, res = sur.kdk(st, a.dt, a.steps, probe=probe)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    timing = _lib.timing_collect()
+    _lib.timing_enable(False)
+    value = N * a.steps / sec
+    print(f"[bench] surrogate {a.steps} steps in {sec * 1e3:.2f} ms -> {value:.4e} particle-steps/s "
+          f"(fell back: {fell_back}, probe residual {res:.3e})", file=sys.stderr, flush=True)
+    conv = {}
+    flops = {"conv_fwd_res": 2 * 27 * 32 * 32, "conv_fwd_gelu": 2 * 27 * 16 * 32, "conv_fwd_out": 2 * 27 * 32 * 32}
+    for name, fpv in flops.items():
+        if name in timing:
+            ms, cnt = timing[name]
+            tf = fpv * G ** 3 / (ms / cnt / 1e3) / 1e12
+            conv[name] = {"us_avg": ms / cnt * 1e3, "tflops": tf, "frac": tf / MFMA_BF16_PEAK_TFLOPS}
+    res_k = conv.get("conv_fwd_res")
+    return {
+        "value": value, "unit": "particle-steps/s", "ms_per_step": sec * 1e3 / a.steps,
+        "speed_ratio_vs_plain_pm": value / pm_value,
+        "force_rel_rms_vs_fp64": rel_sur,
+        "rel_rms_ratio_vs_plain_pm": (rel_sur / pm_rel) if (rel_sur is not None and pm_rel) else None,
+        "fell_back": bool(fell_back), "probe_residual": res, "threshold": sur.threshold,
+        "model": "PM + LocalNet3D correction (6 convs 3^3, width 32, BF16/FP32-acc), input [density contrast, "
+                 "PM force], 3-channel correction",
+        "train": {"scenes": a.sur_scenes, "steps": a.sur_train_steps, "lr": a.sur_lr, "targets": a.sur_targets,
+                  "seconds": t_train, "loss_first": losses[0], "loss_last": losses[-1],
+                  "data": "clumpy scenes of the workload's mixture, seeds 1000+, FP64 Ewald eps=1/G"},
+        "roofline": {"kernel": "conv_fwd_res", "bound": "mfma",
+                     "achieved": res_k["tflops"] if res_k else None, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": res_k["frac"] if res_k else None, "traffic": None},
+        "conv_kernels": conv,
+        "kernels": {k: {"us_avg": v[0] / v[1] * 1e3, "launches": v[1]} for k, v in timing.items()},
+    }
+
+
 def measured_traffic(kernel, N, G):
     """HBM bytes per launch of `kernel` from the committed counter summary of
     this workload (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by what the
@@ -92,6 +176,11 @@ def main():
     ap.add_argument("--targets", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-scenes", type=int, default=3, help="training scenes of the surrogate")
+    ap.add_argument("--sur-train-steps", type=int, default=120)
+    ap.add_argument("--sur-lr", type=float, default=1e-3)
+    ap.add_argument("--sur-targets", type=int, default=4096, help="FP64 target particles per training scene")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run one 256 MiB device-to-device copy (the byte-counter "
                          "calibration kernel of a rocprofv3 --pmc run)")
@@ -121,9 +210,9 @@ def main():
         tg = torch.from_numpy(scenes.targets(N, a.targets)).to(dev)
         pos0 = torch.from_numpy(pos_np).to(dev)
         a_pm = pm.accel(pos0, mass_value=1.0 / N)[tg].double()
-        a_ex = ops.ewald_f64(pos0, 1.0 / N, tg, L=1.0, eps=1.0 / 256)
+        a_ex = ops.ewald_f64(pos0, 1.0 / N, tg, L=1.0, eps=1.0 / G)
         rel = float((((a_pm - a_ex) ** 2).sum(-1).mean() / (a_ex ** 2).sum(-1).mean()).sqrt())
-        del pos0, a_pm, a_ex
+        del pos0, a_pm
         print(f"[bench] PM force rel-RMS vs FP64 Ewald on {a.targets} targets: {rel:.4e}", file=sys.stderr,
               flush=True)
 
@@ -169,6 +258,10 @@ def main():
                          "frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS if ms > 0 and b else None}
     kernels = {k: {"ms_total": v[0], "launches": v[1], "us_avg": v[0] / v[1] * 1e3} for k, v in timing.items()}
 
+    sur = None
+    if not a.no_surrogate:
+        sur = surrogate_bench(a, N, G, pos_np, vel_np, dev, None if a.no_oracle else (tg, a_ex), value, rel)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(pos_np, vel_np, G, a.dt)
@@ -182,8 +275,8 @@ def main():
         "config": {"workload": f"plain periodic PM, N={N} (clumpy: 50% uniform + 64 Gaussian clumps sigma 0.02), "
                                f"{G}^3 grid, KDK dt={a.dt}, Maxwellian sigma_v=0.01",
                    "N": N, "grid": G, "dt": a.dt, "path": "plain_pm"},
-        "force_rel_rms_vs_fp64": rel, "oracle": "FP64 Ewald (periodic, eps=1/256)",
-        "surrogate": None,
+        "force_rel_rms_vs_fp64": rel, "oracle": f"FP64 Ewald (periodic, eps=1/{G})",
+        "surrogate": sur,
         "roofline": roofline, "stages": stages, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
