@@ -23,17 +23,24 @@ constexpr int HX = CT_X + 2, HY = CT_Y + 2, HZ = CT_Z + 2;
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// round-to-nearest-even FP32 -> BF16 (NaN kept quiet)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// round-to-nearest-even FP32 -> BF16 (the hardware conversion)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in its tanh form, 0.5 z (1 + tanh(u)) with u = sqrt(2/pi) (z + 0.044715 z^3),
+// written as z / (1 + e^(-2u)) on the hardware exp2 and reciprocal (relative
+// error ~1e-6, far below the 2^-8 of the BF16 activations)
+constexpr float GELU_K = -2.3022082f;           // -2 sqrt(2/pi) log2(e)
+constexpr float GELU_C = 0.044715f;
+constexpr float GELU_D = 1.5957691216057308f;   // 2 sqrt(2/pi)
+__device__ __forceinline__ float gelu(float z) {
+  const float e = __builtin_amdgcn_exp2f(z * fmaf(GELU_K * GELU_C, z * z, GELU_K));
+  return z * __builtin_amdgcn_rcpf(1.0f + e);
+}
+// d/dz of the same function: s + z s (1 - s) 2u'(z), s = 1 / (1 + e^(-2u))
 __device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+  const float e = __builtin_amdgcn_exp2f(z * fmaf(GELU_K * GELU_C, z * z, GELU_K));
+  const float sg = __builtin_amdgcn_rcpf(1.0f + e);
+  return fmaf(z * sg * (1.0f - sg), GELU_D * fmaf(3.0f * GELU_C, z * z, 1.0f), sg);
 }
 
 enum ConvMode : int {
@@ -43,18 +50,33 @@ enum ConvMode : int {
   CONV_OUT_F32_CL = 3,   // y[vox][co] = conv (+ addend[vox][co])                (FP32 channels-last, no bias)
 };
 
+// ------------------------------------------------------- z-marching forward
+// One workgroup (4 waves) computes a 32 x 16 (x, y) column of output voxels
+// over ZM_ZC consecutive z planes, all 32 output channels.  Each wave keeps
+// the whole weight tensor in registers (27 taps x CIN/16 MFMA B fragments)
+// and owns 4 output rows; the input planes z-1, z, z+1 (with a one-voxel
+// periodic halo) sit in a 3-plane LDS ring.  An input row fragment is loaded
+// once per (dz, dx, K-step) and feeds every output row it touches (dy = 0..2),
+// two MFMAs per LDS read on average.  The next plane is fetched into
+// registers while the current plane is computed, then written over the
+// plane that is no longer needed.
+constexpr int ZM_X = 32, ZM_Y = 16, ZM_ZC = 16, ZM_THREADS = 256, ZM_RPW = 4;
+constexpr int ZM_PX = ZM_X + 2, ZM_PY = ZM_Y + 2;
+
 template <int CIN, int MODE>
-__global__ __launch_bounds__(CONV_THREADS) void k_conv3_fwd(const uint16_t* __restrict__ x,
-                                                            const uint16_t* __restrict__ w,
-                                                            const float* __restrict__ bias, int G, int cout,
-                                                            uint16_t* __restrict__ z_out, uint16_t* __restrict__ h_out,
-                                                            float* __restrict__ y_out,
-                                                            const float* __restrict__ addend) {
+__global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restrict__ x,
+                                                         const uint16_t* __restrict__ w,
+                                                         const float* __restrict__ bias, int G, int cout,
+                                                         uint16_t* __restrict__ z_out, uint16_t* __restrict__ h_out,
+                                                         float* __restrict__ y_out, const float* __restrict__ addend) {
   constexpr int CP = CIN + 8;  // padded channel stride (conflict-free 16-byte LDS reads)
-  __shared__ __attribute__((aligned(16))) uint16_t wl[27 * 32 * CP];
-  __shared__ __attribute__((aligned(16))) uint16_t xl[HZ * HY * HX * CP];
+  constexpr int NH = CIN / 16;
+  constexpr int PLANE = ZM_PY * ZM_PX * CP;
+  constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
+  constexpr int PER = (CHUNKS + ZM_THREADS - 1) / ZM_THREADS;
+  __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
   const int tid = threadIdx.x;
-  const int ntx = G / CT_X, nty = G / CT_Y, ntz = G / CT_Z;
+  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / ZM_ZC;
   int t = blockIdx.x;
   const int bx = t % ntx;
   t /= ntx;
@@ -62,88 +84,122 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_fwd(const uint16_t* __re
   t /= nty;
   const int bz = t % ntz;
   const int b = t / ntz;
-  const int x0 = bx * CT_X, y0 = by * CT_Y, z0 = bz * CT_Z;
+  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * ZM_ZC;
   const int64_t G3 = (int64_t)G * G * G;
   const uint16_t* xb = x + (int64_t)b * G3 * CIN;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int i = lane & 31, kh = lane >> 5;
 
-  // weights -> LDS (16-byte chunks)
-  constexpr int WCH = 27 * 32 * (CIN / 8);
-  for (int c = tid; c < WCH; c += CONV_THREADS) {
-    const int row = c / (CIN / 8), ch = c % (CIN / 8);
-    *reinterpret_cast<uint4*>(&wl[row * CP + ch * 8]) = *reinterpret_cast<const uint4*>(&w[(int64_t)row * CIN + ch * 8]);
-  }
-  // input tile with periodic halo -> LDS
-  constexpr int XCH = HZ * HY * HX * (CIN / 8);
-  for (int c = tid; c < XCH; c += CONV_THREADS) {
-    const int ch = c % (CIN / 8);
-    int v = c / (CIN / 8);
-    const int tx = v % HX;
-    v /= HX;
-    const int ty = v % HY;
-    const int tz = v / HY;
-    const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1), gz = (z0 + tz - 1) & (G - 1);
-    const int64_t gv = ((int64_t)gz * G + gy) * G + gx;
-    *reinterpret_cast<uint4*>(&xl[((tz * HY + ty) * HX + tx) * CP + ch * 8]) =
-        *reinterpret_cast<const uint4*>(&xb[gv * CIN + ch * 8]);
+  bf16x8 wr[27][NH];
+#pragma unroll
+  for (int tap = 0; tap < 27; ++tap)
+#pragma unroll
+    for (int hh = 0; hh < NH; ++hh)
+      wr[tap][hh] = *reinterpret_cast<const bf16x8*>(&w[((int64_t)tap * 32 + i) * CIN + hh * 16 + kh * 8]);
+
+  auto load_plane = [&](int zz, uint4 (&r)[PER]) {
+    const int gz = zz & (G - 1);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * ZM_THREADS;
+      if (c < CHUNKS) {
+        const int ch = c % (CIN / 8), v = c / (CIN / 8);
+        const int tx = v % ZM_PX, ty = v / ZM_PX;
+        const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1);
+        r[k] = *reinterpret_cast<const uint4*>(&xb[(((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
+      }
+    }
+  };
+  auto store_plane = [&](int slot, const uint4 (&r)[PER]) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * ZM_THREADS;
+      if (c < CHUNKS) {
+        const int ch = c % (CIN / 8), v = c / (CIN / 8);
+        *reinterpret_cast<uint4*>(&ring[slot * PLANE + v * CP + ch * 8]) = r[k];
+      }
+    }
+  };
+  {  // planes zs - 1, zs, zs + 1 -> slots 0, 1, 2
+    uint4 r[PER];
+#pragma unroll 1
+    for (int s = 0; s < 3; ++s) {
+      load_plane(zs - 1 + s, r);
+      store_plane(s, r);
+    }
   }
   __syncthreads();
 
-  const int lane = tid & 63, wv = tid >> 6;
-  const int i = lane & 31, kh = lane >> 5;
-  int rz[2], ry[2];
-#pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
-    const int r = wv * 2 + rr;
-    rz[rr] = r / CT_Y;
-    ry[rr] = r % CT_Y;
-  }
-  f32x16 acc[2];
-#pragma unroll
-  for (int rr = 0; rr < 2; ++rr)
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc[rr][q] = 0.0f;
-
-#pragma unroll 1
-  for (int tap = 0; tap < 27; ++tap) {
-    const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-#pragma unroll
-    for (int hh = 0; hh < CIN / 16; ++hh) {
-      const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&wl[(tap * 32 + i) * CP + hh * 16 + kh * 8]);
-#pragma unroll
-      for (int rr = 0; rr < 2; ++rr) {
-        const bf16x8 av = *reinterpret_cast<const bf16x8*>(
-            &xl[(((rz[rr] + dz) * HY + (ry[rr] + dy)) * HX + (i + dx)) * CP + hh * 16 + kh * 8]);
-        acc[rr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[rr], 0, 0, 0);
-      }
-    }
-  }
-
-  // epilogue: acc[rr][q] holds (voxel = 8*(q/4) + 4*kh + q%4, co = i) of row rr
   const int co = i;
   const float bco = (MODE == CONV_OUT_F32_CL || co >= cout) ? 0.0f : bias[co];
+#pragma unroll 1
+  for (int zi = 0; zi < ZM_ZC; ++zi) {
+    const int z = zs + zi;
+    uint4 pre[PER];
+    if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);  // in flight during the MFMAs below
+    f32x16 acc[ZM_RPW];
 #pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
-    const int gz = z0 + rz[rr], gy = y0 + ry[rr];
+    for (int r = 0; r < ZM_RPW; ++r)
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int vx = 8 * (q / 4) + 4 * kh + (q % 4);
-      const int64_t gv = ((int64_t)gz * G + gy) * G + x0 + vx;
-      const float val = acc[rr][q] + bco;
+      for (int q = 0; q < 16; ++q) acc[r][q] = 0.0f;
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz) {
+      const uint16_t* P = ring + ((zi + dz) % 3) * PLANE;
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int yr = 0; yr < ZM_RPW + 2; ++yr)
+#pragma unroll
+          for (int hh = 0; hh < NH; ++hh) {
+            const bf16x8 av =
+                *reinterpret_cast<const bf16x8*>(&P[((wv * ZM_RPW + yr) * ZM_PX + i + dx) * CP + hh * 16 + kh * 8]);
+#pragma unroll
+            for (int r = 0; r < ZM_RPW; ++r) {
+              const int dy = yr - r;
+              if (dy < 0 || dy > 2) continue;
+              acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, wr[(dz * 3 + dy) * 3 + dx][hh], acc[r], 0, 0, 0);
+            }
+          }
+    }
+    // epilogue: acc[r][q] holds (voxel x = 8*(q/4) + 4*kh + q%4, co = i) of row r
+    const uint16_t* C = ring + ((zi + 1) % 3) * PLANE;  // input plane z (residual centre)
+#pragma unroll
+    for (int r = 0; r < ZM_RPW; ++r) {
+      const int ly = wv * ZM_RPW + r;
+      const int64_t row = ((int64_t)z * G + y0 + ly) * G + x0;  // voxel index of (x0, y, z)
       if (MODE == CONV_OUT_F32_CF) {
-        if (co < cout) y_out[((int64_t)b * cout + co) * G3 + gv] = val;
+        if (co < cout) {
+          float* yo = y_out + ((int64_t)b * cout + co) * G3 + row + 4 * kh;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) yo[8 * (q / 4) + (q % 4)] = acc[r][q] + bco;
+        }
       } else if (MODE == CONV_OUT_F32_CL) {
         if (co < cout) {
-          const int64_t o = ((int64_t)b * G3 + gv) * cout + co;
-          y_out[o] = addend ? val + addend[o] : val;
+          const int64_t o0 = ((int64_t)b * G3 + row + 4 * kh) * cout + co;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int64_t o = o0 + (int64_t)(8 * (q / 4) + (q % 4)) * cout;
+            y_out[o] = addend ? acc[r][q] + addend[o] : acc[r][q];
+          }
         }
       } else {
-        const int64_t o = ((int64_t)b * G3 + gv) * 32 + co;
-        if (z_out) z_out[o] = f2bf(val);
-        float hv = gelu(val);
-        if (MODE == CONV_RES_GELU)
-          hv += bf2f(xl[(((rz[rr] + 1) * HY + (ry[rr] + 1)) * HX + (vx + 1)) * CP + co]);
-        h_out[o] = f2bf(hv);
+        const int64_t o0 = ((int64_t)b * G3 + row + 4 * kh) * 32 + co;
+        const uint16_t* cr = C + ((ly + 1) * ZM_PX + 1 + 4 * kh) * CP + co;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int dv = 8 * (q / 4) + (q % 4);
+          const float val = acc[r][q] + bco;
+          if (z_out) z_out[o0 + dv * 32] = f2bf(val);
+          float hv = gelu(val);
+          if (MODE == CONV_RES_GELU) hv += bf2f(cr[dv * CP]);
+          h_out[o0 + dv * 32] = f2bf(hv);
+        }
       }
+    }
+    __syncthreads();  // every wave is done with the slot of plane z - 1
+    if (zi + 1 < ZM_ZC) {
+      store_plane(zi % 3, pre);  // plane z + 2 replaces plane z - 1
+      __syncthreads();
     }
   }
 }
@@ -338,10 +394,10 @@ int conv_tiles_ok(int64_t G) { return is_pow2(G) && G >= 32 && G <= 1024; }
 template <int CIN, int MODE>
 int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t B, int64_t G, int cout, uint16_t* z,
                uint16_t* h, float* y, const float* addend, hipStream_t s, const char* name) {
-  const int64_t blocks = B * (G / CT_X) * (G / CT_Y) * (G / CT_Z);
+  const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / ZM_ZC);
   AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
   TimedScope ts_(name, s);
-  hipLaunchKernelGGL((k_conv3_fwd<CIN, MODE>), dim3((unsigned)blocks), dim3(CONV_THREADS), 0, s, x, w, bias, (int)G,
+  hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
                      cout, z, h, y, addend);
   AUGUR_LAUNCHED(name);
   return AUGUR_OK;

@@ -3,7 +3,10 @@
 ``LocalNet3D`` is the reference LocalNet topology (scripts/scatter_field.py:
 15-26) in 3D with periodic padding: h = gelu(conv_in(x)); for each middle
 layer h = h + gelu(conv(h)); out = conv_out(h).  With ``layers=5`` it has six
-3^3 convolutions of width 32.  Weights are FP32 masters [27][32][cin]
+3^3 convolutions of width 32.  GELU is the tanh form (torch's
+``approximate='tanh'``), which the kernels evaluate on the hardware exp2 and
+reciprocal; it differs from the erf form by at most 3e-4, below the BF16
+resolution of the activations.  Weights are FP32 masters [27][32][cin]
 (tap-major) with BF16 copies used by the kernels; activations are BF16
 channels-last with FP32 accumulation.
 

@@ -91,7 +91,7 @@ def conv3d_periodic(x, w, b=None):
 def localnet3d(x, weights, biases):
     """LocalNet topology (scripts/scatter_field.py:15-26) in 3D with periodic padding:
     h = gelu(conv_in(x)); h = h + gelu(conv(h)) for each middle layer; out = conv_out(h)."""
-    h = F.gelu(conv3d_periodic(x, weights[0], biases[0]))
+    h = F.gelu(conv3d_periodic(x, weights[0], biases[0]), approximate='tanh')
     for w, b in zip(weights[1:-1], biases[1:-1]):
-        h = h + F.gelu(conv3d_periodic(h, w, b))
+        h = h + F.gelu(conv3d_periodic(h, w, b), approximate='tanh')
     return conv3d_periodic(h, weights[-1], biases[-1])
