@@ -23,6 +23,7 @@ struct RowMap {
   int64_t n_rows;          // batch * rows_per_batch
   int ny, nz;              // FFT outer extents (nz = 1 for 2D)
   int in_ny, in_nz, in_nx; // input (or cropped output) extents
+  int64_t pitch = 0;       // complex row stride of the half spectrum (0: N/2 + 1)
 };
 
 template <int N>
@@ -53,9 +54,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows_r2c(const float* __res
     }
     return make_float2(v, 0.0f);
   };
+  const int64_t pitch = m.pitch ? m.pitch : nxc;
   auto dst = [&](int l, int k, float2 v) {
     const int64_t row = row0 + l;
-    if (row < m.n_rows && k < nxc) out[row * nxc + k] = v;
+    if (row < m.n_rows && k < nxc) out[row * pitch + k] = v;
   };
   fft_lines<N, false>(lds, tw, C::NL, src, dst);
 }
@@ -181,18 +183,20 @@ int launch_rows_c2r(int n, const float2* in, float* out, const RowMap& m, float 
 }
 
 // x-row R2C and y-column forward transforms of one real G^3 grid (the z
-// transform is left to the fused Poisson pass of the PM stepper).
-int fft_r2c_xy(const float* in, float2* out, int64_t G, hipStream_t s) {
+// transform is left to the fused Poisson pass of the PM stepper).  The half
+// spectrum's rows are `pitch` complex apart (>= G/2 + 1).
+int fft_r2c_xy(const float* in, float2* out, int64_t G, int64_t pitch, hipStream_t s) {
   RowMap m;
   m.ny = (int)G;
   m.nz = (int)G;
   m.rows_per_batch = G * G;
   m.n_rows = G * G;
   m.in_nx = m.in_ny = m.in_nz = (int)G;
+  m.pitch = pitch;
   int rc = launch_rows_r2c((int)G, in, out, m, s);
   if (rc) return rc;
   const int nxc = (int)(G / 2 + 1);
-  ColMap my{G, 1, G * nxc, 0, nxc, nxc};
+  ColMap my{G, 1, G * pitch, 0, pitch, nxc};
   return launch_cols<false>((int)G, out, my, s);
 }
 

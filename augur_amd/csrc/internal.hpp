@@ -13,7 +13,7 @@ size_t radix_workspace_bytes(int64_t n);
 int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* ws, size_t ws_bytes,
                      uint32_t** keys_out, uint32_t** vals_out, hipStream_t s, const uint32_t* gate = nullptr);
 
-int fft_r2c_xy(const float* in, float2* out, int64_t G, hipStream_t s);
+int fft_r2c_xy(const float* in, float2* out, int64_t G, int64_t pitch, hipStream_t s);
 
 // Per-launch event timing (see timing.hip).  A TimedScope around a launch
 // records a start/stop event pair on the stream when timing is enabled and

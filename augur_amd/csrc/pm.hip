@@ -464,7 +464,10 @@ __global__ void k_zero_u64(unsigned long long* p, int64_t n) {
 }
 
 // ------------------------------------------------------ fused FFT passes
-// Spectral layout after the x R2C pass: (z, y|ky, kx) with nxc = G/2 + 1.
+// Spectral layout after the x R2C pass: (z, y|ky, kx) with nxc = G/2 + 1
+// columns per row and a row pitch RP rounded up to 16 complex (128 bytes), so
+// a column tile's row segment is one aligned cache line.
+__host__ __device__ constexpr int64_t pm_pitch(int64_t G) { return (G / 2 + 1 + 15) / 16 * 16; }
 // Column tiles take NL adjacent kx columns (line-fastest thread mapping).
 template <int N>
 struct PmCol {
@@ -472,6 +475,7 @@ struct PmCol {
   static constexpr int NL = FftLines<N, T>::value > 16 ? 16 : FftLines<N, T>::value;
   static constexpr int LS = FftStride<N>::value;
   static constexpr int nxc = N / 2 + 1;
+  static constexpr int RP = (int)pm_pitch(N);
   static constexpr int nblk = (nxc + NL - 1) / NL;
 };
 
@@ -488,7 +492,7 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ s
   __syncthreads();
   const int ky = blockIdx.x / C::nblk;
   const int c0 = (blockIdx.x % C::nblk) * C::NL;
-  const int64_t base = (int64_t)ky * C::nxc + c0, es = (int64_t)N * C::nxc;
+  const int64_t base = (int64_t)ky * C::RP + c0, es = (int64_t)N * C::RP;
   const int ny = ky <= N / 2 ? ky : ky - N;
   auto gsrc = [&](int c, int k) -> float2 {
     return (c0 + c < C::nxc) ? spec[base + k * es + c] : make_float2(0.f, 0.f);
@@ -540,7 +544,7 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0
   __syncthreads();
   const int z = blockIdx.x / C::nblk;
   const int c0 = (blockIdx.x % C::nblk) * C::NL;
-  const int64_t base = (int64_t)z * N * C::nxc + c0, es = C::nxc;
+  const int64_t base = (int64_t)z * N * C::RP + c0, es = C::RP;
   auto ld = [&](const float2* p, int c, int k) -> float2 {
     return (c0 + c < C::nxc) ? p[base + k * es + c] : make_float2(0.f, 0.f);
   };
@@ -566,6 +570,7 @@ struct PmRow {
   static constexpr int NL = FftLines<N, FFT_THREADS>::value;
   static constexpr int LS = FftStride<N>::value;
   static constexpr int nxc = N / 2 + 1;
+  static constexpr int RP = (int)pm_pitch(N);
 };
 
 template <int N>
@@ -581,7 +586,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pm_xc2r(const float2* __restric
   __syncthreads();
   const int64_t row0 = (int64_t)blockIdx.x * C::NL;
   auto herm = [&](const float2* p, int l, int k) -> float2 {
-    float2 v = p[(row0 + l) * C::nxc + (k < C::nxc ? k : N - k)];
+    float2 v = p[(row0 + l) * C::RP + (k < C::nxc ? k : N - k)];
     if (k >= C::nxc) v.y = -v.y;
     return v;
   };
@@ -909,7 +914,7 @@ struct PmBufs {
   PmDev* st;
   uint32_t* cell_start;  // G^3 + 1
   float* rho;            // G^3
-  float2* s0;            // G*G*nxc complex (spectrum / A / out0)
+  float2* s0;            // G*G*pitch complex (spectrum / A / out0)
   float2* s1;            // out1
   float2* s2;            // B / out2
   float* force;          // 3 G^3: planes a_x, a_y, a_z
@@ -938,7 +943,7 @@ static int64_t pm_max_items(int64_t n, int64_t G) { return pm_tiles(G) + (8 * n 
 static int64_t pm_max_gitems(int64_t n, int64_t G) { return pm_pieces(G) + (n + TG_CHUNK - 1) / TG_CHUNK; }
 
 static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
-  const int64_t G3 = G * G * G, nc = G * G * (G / 2 + 1);
+  const int64_t G3 = G * G * G, nc = G * G * pm_pitch(G);
   b.st = c.take<PmDev>(1);
   b.cell_start = c.take<uint32_t>(G3 + 1);
   b.rho = c.take<float>(G3);
@@ -1110,7 +1115,7 @@ static int pm_force_grid(const float* pos, const uint32_t* perm, const float* ma
                          int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
   int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s);
   if (rc) return rc;
-  rc = fft_r2c_xy(b.rho, b.s0, G, s);  // x rows R2C + y columns forward
+  rc = fft_r2c_xy(b.rho, b.s0, G, pm_pitch(G), s);  // x rows R2C + y columns forward
   if (rc) return rc;
   const double PI = 3.14159265358979323846;
   const float kf = (float)(2.0 * PI / (double)L);
