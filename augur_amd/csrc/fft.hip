@@ -200,6 +200,12 @@ int fft_r2c_xy(const float* in, float2* out, int64_t G, int64_t pitch, hipStream
   return launch_cols<false>((int)G, out, my, s);
 }
 
+// y-column forward transforms of a (G, G, pitch) half spectrum (G/2 + 1 columns)
+int fft_cols_y(float2* data, int64_t G, int64_t pitch, hipStream_t s) {
+  ColMap my{G, 1, G * pitch, 0, pitch, (int)(G / 2 + 1)};
+  return launch_cols<false>((int)G, data, my, s);
+}
+
 static int check_shape(int ndim, const int64_t* shape) {
   AUGUR_REQUIRE(ndim == 2 || ndim == 3, AUGUR_EINVAL, "fft: ndim must be 2 or 3");
   AUGUR_REQUIRE(shape != nullptr, AUGUR_EINVAL, "fft: null shape");

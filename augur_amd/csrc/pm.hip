@@ -612,6 +612,31 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pm_xc2r(const float2* __restric
                      [&](int l, int x, float2 v) { force[2 * PLANE + (row0 + l) * N + x] = v.x; });
 }
 
+// x rows R2C of the density, two real rows per complex transform:
+// FFT(a + i b) = Z, A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / 2i.
+// Output rows at the padded pitch.
+template <int N>
+__global__ __launch_bounds__(FFT_THREADS) void k_pm_rows_r2c(const float* __restrict__ rho,
+                                                             float2* __restrict__ out) {
+  using C = PmRow<N>;
+  constexpr int H = C::NL / 2;
+  __shared__ float2 X[H * C::LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  __syncthreads();
+  const int64_t row0 = (int64_t)blockIdx.x * C::NL;
+  fft_lines<N, false, false, false, true>(
+      X, tw, H,
+      [&](int l, int x) { return make_float2(rho[(row0 + l) * N + x], rho[(row0 + l + H) * N + x]); },
+      [&](int l, int k, float2 v) { X[l * C::LS + fft_pad(k)] = v; });
+  for (int e = threadIdx.x; e < H * C::nxc; e += FFT_THREADS) {
+    const int l = e / C::nxc, k = e - l * C::nxc;
+    const float2 z = X[l * C::LS + fft_pad(k)], w = X[l * C::LS + fft_pad((N - k) & (N - 1))];
+    out[(row0 + l) * C::RP + k] = make_float2(0.5f * (z.x + w.x), 0.5f * (z.y - w.y));
+    out[(row0 + l + H) * C::RP + k] = make_float2(0.5f * (z.y + w.y), 0.5f * (w.x - z.x));
+  }
+}
+
 // ------------------------------------------------------------- gather + KDK
 // max of v over the block, then one atomicMax per block
 __device__ __forceinline__ void block_max_to(float v, uint32_t* target) {
@@ -1115,7 +1140,13 @@ static int pm_force_grid(const float* pos, const uint32_t* perm, const float* ma
                          int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
   int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s);
   if (rc) return rc;
-  rc = fft_r2c_xy(b.rho, b.s0, G, pm_pitch(G), s);  // x rows R2C + y columns forward
+  {
+    TimedScope ts_("fft_rows_r2c", s);
+    PM_SWITCH_G(G, k_pm_rows_r2c, dim3((unsigned)(G * G / PmRow<GG>::NL)), dim3(FFT_THREADS), s,
+                (const float*)b.rho, b.s0);
+    AUGUR_LAUNCHED("fft_rows_r2c");
+  }
+  rc = fft_cols_y(b.s0, G, pm_pitch(G), s);  // y columns forward
   if (rc) return rc;
   const double PI = 3.14159265358979323846;
   const float kf = (float)(2.0 * PI / (double)L);
