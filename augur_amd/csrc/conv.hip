@@ -16,6 +16,7 @@ namespace augur {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
 constexpr int CT_X = 32, CT_Y = 4, CT_Z = 2;
 constexpr int CONV_THREADS = 256;
@@ -75,6 +76,7 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
   constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
   constexpr int PER = (CHUNKS + ZM_THREADS - 1) / ZM_THREADS;
   __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
+  __shared__ __attribute__((aligned(16))) float sbias[32];
   const int tid = threadIdx.x;
   const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / ZM_ZC;
   int t = blockIdx.x;
@@ -89,7 +91,9 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
   const uint16_t* xb = x + (int64_t)b * G3 * CIN;
   const int lane = tid & 63, wv = tid >> 6;
   const int i = lane & 31, kh = lane >> 5;
+  if (tid < 32) sbias[tid] = (MODE == CONV_OUT_F32_CL || tid >= cout) ? 0.0f : bias[tid];
 
+  // A operand: w[tap][co = i][k-slice of lane]
   bf16x8 wr[27][NH];
 #pragma unroll
   for (int tap = 0; tap < 27; ++tap)
@@ -97,7 +101,9 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
     for (int hh = 0; hh < NH; ++hh)
       wr[tap][hh] = *reinterpret_cast<const bf16x8*>(&w[((int64_t)tap * 32 + i) * CIN + hh * 16 + kh * 8]);
 
-  auto load_plane = [&](int zz, uint4 (&r)[PER]) {
+  // the plane's 16-byte chunks travel global -> registers -> LDS as native
+  // vectors (plain loads and stores, so the arrays stay in registers)
+  auto load_plane = [&](int zz, u32x4 (&r)[PER]) __attribute__((always_inline)) {
     const int gz = zz & (G - 1);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
@@ -106,36 +112,35 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
         const int ch = c % (CIN / 8), v = c / (CIN / 8);
         const int tx = v % ZM_PX, ty = v / ZM_PX;
         const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1);
-        r[k] = *reinterpret_cast<const uint4*>(&xb[(((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
+        r[k] = *reinterpret_cast<const u32x4*>(&xb[(((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
       }
     }
   };
-  auto store_plane = [&](int slot, const uint4 (&r)[PER]) {
+  auto store_plane = [&](int slot, const u32x4 (&r)[PER]) __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int c = tid + k * ZM_THREADS;
       if (c < CHUNKS) {
         const int ch = c % (CIN / 8), v = c / (CIN / 8);
-        *reinterpret_cast<uint4*>(&ring[slot * PLANE + v * CP + ch * 8]) = r[k];
+        *reinterpret_cast<u32x4*>(&ring[slot * PLANE + v * CP + ch * 8]) = r[k];
       }
     }
   };
   {  // planes zs - 1, zs, zs + 1 -> slots 0, 1, 2
-    uint4 r[PER];
-#pragma unroll 1
-    for (int s = 0; s < 3; ++s) {
-      load_plane(zs - 1 + s, r);
-      store_plane(s, r);
-    }
+    u32x4 r0[PER], r1[PER];
+    load_plane(zs - 1, r0);
+    load_plane(zs, r1);
+    store_plane(0, r0);
+    load_plane(zs + 1, r0);
+    store_plane(1, r1);
+    store_plane(2, r0);
   }
   __syncthreads();
 
-  const int co = i;
-  const float bco = (MODE == CONV_OUT_F32_CL || co >= cout) ? 0.0f : bias[co];
 #pragma unroll 1
   for (int zi = 0; zi < ZM_ZC; ++zi) {
     const int z = zs + zi;
-    uint4 pre[PER];
+    u32x4 pre[PER];
     if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);  // in flight during the MFMAs below
     f32x16 acc[ZM_RPW];
 #pragma unroll
@@ -151,48 +156,77 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
         for (int yr = 0; yr < ZM_RPW + 2; ++yr)
 #pragma unroll
           for (int hh = 0; hh < NH; ++hh) {
+            // B operand: input row voxel i + dx - 1, k-slice of lane
             const bf16x8 av =
                 *reinterpret_cast<const bf16x8*>(&P[((wv * ZM_RPW + yr) * ZM_PX + i + dx) * CP + hh * 16 + kh * 8]);
 #pragma unroll
             for (int r = 0; r < ZM_RPW; ++r) {
               const int dy = yr - r;
               if (dy < 0 || dy > 2) continue;
-              acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, wr[(dz * 3 + dy) * 3 + dx][hh], acc[r], 0, 0, 0);
+              acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[(dz * 3 + dy) * 3 + dx][hh], av, acc[r], 0, 0, 0);
             }
           }
     }
-    // epilogue: acc[r][q] holds (voxel x = 8*(q/4) + 4*kh + q%4, co = i) of row r
+    // epilogue: acc[r][4 j + e] holds output channel co = 8 j + 4 kh + e of
+    // voxel (x0 + i, y0 + row, z): four consecutive channels per j
     const uint16_t* C = ring + ((zi + 1) % 3) * PLANE;  // input plane z (residual centre)
 #pragma unroll
     for (int r = 0; r < ZM_RPW; ++r) {
       const int ly = wv * ZM_RPW + r;
-      const int64_t row = ((int64_t)z * G + y0 + ly) * G + x0;  // voxel index of (x0, y, z)
-      if (MODE == CONV_OUT_F32_CF) {
-        if (co < cout) {
-          float* yo = y_out + ((int64_t)b * cout + co) * G3 + row + 4 * kh;
+      const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + i;  // voxel index within the scene
 #pragma unroll
-          for (int q = 0; q < 16; ++q) yo[8 * (q / 4) + (q % 4)] = acc[r][q] + bco;
-        }
-      } else if (MODE == CONV_OUT_F32_CL) {
-        if (co < cout) {
-          const int64_t o0 = ((int64_t)b * G3 + row + 4 * kh) * cout + co;
+      for (int j = 0; j < 4; ++j) {
+        const int co0 = 8 * j + 4 * kh;
+        const float4 bj = *reinterpret_cast<const float4*>(&sbias[co0]);
+        float v[4] = {acc[r][4 * j] + bj.x, acc[r][4 * j + 1] + bj.y, acc[r][4 * j + 2] + bj.z,
+                      acc[r][4 * j + 3] + bj.w};
+        if (MODE == CONV_OUT_F32_CF) {
+          float* yb = y_out + (int64_t)b * cout * G3 + sv;
 #pragma unroll
-          for (int q = 0; q < 16; ++q) {
-            const int64_t o = o0 + (int64_t)(8 * (q / 4) + (q % 4)) * cout;
-            y_out[o] = addend ? acc[r][q] + addend[o] : acc[r][q];
+          for (int e = 0; e < 4; ++e)
+            if (co0 + e < cout) yb[(int64_t)(co0 + e) * G3] = v[e];
+        } else if (MODE == CONV_OUT_F32_CL) {
+          if (cout == 32) {
+            const int64_t o = ((int64_t)b * G3 + sv) * 32 + co0;
+            float4 val = {v[0], v[1], v[2], v[3]};
+            if (addend) {
+              const float4 ad = *reinterpret_cast<const float4*>(&addend[o]);
+              val.x += ad.x;
+              val.y += ad.y;
+              val.z += ad.z;
+              val.w += ad.w;
+            }
+            *reinterpret_cast<float4*>(&y_out[o]) = val;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (co0 + e < cout) {
+                const int64_t o = ((int64_t)b * G3 + sv) * cout + co0 + e;
+                y_out[o] = addend ? v[e] + addend[o] : v[e];
+              }
           }
-        }
-      } else {
-        const int64_t o0 = ((int64_t)b * G3 + row + 4 * kh) * 32 + co;
-        const uint16_t* cr = C + ((ly + 1) * ZM_PX + 1 + 4 * kh) * CP + co;
+        } else {
+          const int64_t o = ((int64_t)b * G3 + sv) * 32 + co0;
+          if (z_out) {
+            uint2 zp;
+            zp.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+            zp.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+            *reinterpret_cast<uint2*>(&z_out[o]) = zp;
+          }
+          float hv[4];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int dv = 8 * (q / 4) + (q % 4);
-          const float val = acc[r][q] + bco;
-          if (z_out) z_out[o0 + dv * 32] = f2bf(val);
-          float hv = gelu(val);
-          if (MODE == CONV_RES_GELU) hv += bf2f(cr[dv * CP]);
-          h_out[o0 + dv * 32] = f2bf(hv);
+          for (int e = 0; e < 4; ++e) hv[e] = gelu(v[e]);
+          if (MODE == CONV_RES_GELU) {
+            const uint2 xr = *reinterpret_cast<const uint2*>(&C[((ly + 1) * ZM_PX + 1 + i) * CP + co0]);
+            hv[0] += bf2f((uint16_t)(xr.x & 0xffffu));
+            hv[1] += bf2f((uint16_t)(xr.x >> 16));
+            hv[2] += bf2f((uint16_t)(xr.y & 0xffffu));
+            hv[3] += bf2f((uint16_t)(xr.y >> 16));
+          }
+          uint2 hp;
+          hp.x = (uint32_t)f2bf(hv[0]) | ((uint32_t)f2bf(hv[1]) << 16);
+          hp.y = (uint32_t)f2bf(hv[2]) | ((uint32_t)f2bf(hv[3]) << 16);
+          *reinterpret_cast<uint2*>(&h_out[o]) = hp;
         }
       }
     }
