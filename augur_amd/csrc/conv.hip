@@ -26,6 +26,13 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 
 // round-to-nearest-even FP32 -> BF16 (the hardware conversion)
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+// two values in one packed conversion: lo in bits 0..15, hi in bits 16..31
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  const f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
 
 // GELU in its tanh form, 0.5 z (1 + tanh(u)) with u = sqrt(2/pi) (z + 0.044715 z^3),
 // written as z / (1 + e^(-2u)) on the hardware exp2 and reciprocal (relative
@@ -209,8 +216,8 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
           const int64_t o = ((int64_t)b * G3 + sv) * 32 + co0;
           if (z_out) {
             uint2 zp;
-            zp.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-            zp.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+            zp.x = pack_bf16x2(v[0], v[1]);
+            zp.y = pack_bf16x2(v[2], v[3]);
             *reinterpret_cast<uint2*>(&z_out[o]) = zp;
           }
           float hv[4];
@@ -218,14 +225,14 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
           for (int e = 0; e < 4; ++e) hv[e] = gelu(v[e]);
           if (MODE == CONV_RES_GELU) {
             const uint2 xr = *reinterpret_cast<const uint2*>(&C[((ly + 1) * ZM_PX + 1 + i) * CP + co0]);
-            hv[0] += bf2f((uint16_t)(xr.x & 0xffffu));
-            hv[1] += bf2f((uint16_t)(xr.x >> 16));
-            hv[2] += bf2f((uint16_t)(xr.y & 0xffffu));
-            hv[3] += bf2f((uint16_t)(xr.y >> 16));
+            hv[0] += __uint_as_float(xr.x << 16);
+            hv[1] += __uint_as_float(xr.x & 0xffff0000u);
+            hv[2] += __uint_as_float(xr.y << 16);
+            hv[3] += __uint_as_float(xr.y & 0xffff0000u);
           }
           uint2 hp;
-          hp.x = (uint32_t)f2bf(hv[0]) | ((uint32_t)f2bf(hv[1]) << 16);
-          hp.y = (uint32_t)f2bf(hv[2]) | ((uint32_t)f2bf(hv[3]) << 16);
+          hp.x = pack_bf16x2(hv[0], hv[1]);
+          hp.y = pack_bf16x2(hv[2], hv[3]);
           *reinterpret_cast<uint2*>(&h_out[o]) = hp;
         }
       }
@@ -233,6 +240,142 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
     __syncthreads();  // every wave is done with the slot of plane z - 1
     if (zi + 1 < ZM_ZC) {
       store_plane(zi % 3, pre);  // plane z + 2 replaces plane z - 1
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------- narrow output forward
+// The same z-marching column for a layer with at most 16 output channels and
+// CIN = 32 (the network's output convolution): v_mfma_f32_16x16x32_bf16 with
+// A = weights (16 output channels x 32 input channels of one tap) and B = 16
+// voxels of an input row, so no matrix work is spent on padded channels.
+// Each wave keeps the 27 weight fragments in registers and owns 4 output
+// rows, each as two 16-voxel halves.
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+template <int MODE>
+__global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* __restrict__ x,
+                                                                const uint16_t* __restrict__ w,
+                                                                const float* __restrict__ bias, int G, int cout,
+                                                                float* __restrict__ y_out,
+                                                                const float* __restrict__ addend) {
+  constexpr int CIN = 32, CP = CIN + 8;
+  constexpr int PLANE = ZM_PY * ZM_PX * CP;
+  constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
+  constexpr int PER = (CHUNKS + ZM_THREADS - 1) / ZM_THREADS;
+  __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
+  __shared__ __attribute__((aligned(16))) float sbias[32];
+  const int tid = threadIdx.x;
+  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / ZM_ZC;
+  int t = blockIdx.x;
+  const int bx = t % ntx;
+  t /= ntx;
+  const int by = t % nty;
+  t /= nty;
+  const int bz = t % ntz;
+  const int b = t / ntz;
+  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * ZM_ZC;
+  const int64_t G3 = (int64_t)G * G * G;
+  const uint16_t* xb = x + (int64_t)b * G3 * CIN;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int n16 = lane & 15, kq = lane >> 4;
+  if (tid < 32) sbias[tid] = (MODE == CONV_OUT_F32_CL || tid >= cout) ? 0.0f : bias[tid];
+
+  // A operand: w[tap][co = n16][ci = 8 kq .. 8 kq + 7]
+  bf16x8 wr[27];
+#pragma unroll
+  for (int tap = 0; tap < 27; ++tap)
+    wr[tap] = *reinterpret_cast<const bf16x8*>(&w[((int64_t)tap * 32 + n16) * CIN + kq * 8]);
+
+  auto load_plane = [&](int zz, u32x4 (&r)[PER]) {
+    const int gz = zz & (G - 1);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * ZM_THREADS;
+      if (c < CHUNKS) {
+        const int ch = c % (CIN / 8), v = c / (CIN / 8);
+        const int tx = v % ZM_PX, ty = v / ZM_PX;
+        const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1);
+        r[k] = *reinterpret_cast<const u32x4*>(&xb[(((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
+      }
+    }
+  };
+  auto store_plane = [&](int slot, const u32x4 (&r)[PER]) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * ZM_THREADS;
+      if (c < CHUNKS) {
+        const int ch = c % (CIN / 8), v = c / (CIN / 8);
+        *reinterpret_cast<u32x4*>(&ring[slot * PLANE + v * CP + ch * 8]) = r[k];
+      }
+    }
+  };
+  {
+    u32x4 r[PER];
+#pragma unroll 1
+    for (int s = 0; s < 3; ++s) {
+      load_plane(zs - 1 + s, r);
+      store_plane(s, r);
+    }
+  }
+  __syncthreads();
+
+#pragma unroll 1
+  for (int zi = 0; zi < ZM_ZC; ++zi) {
+    const int z = zs + zi;
+    u32x4 pre[PER];
+    if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);
+    f32x4 acc[ZM_RPW][2];
+#pragma unroll
+    for (int r = 0; r < ZM_RPW; ++r)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) acc[r][hf] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz) {
+      const uint16_t* P = ring + ((zi + dz) % 3) * PLANE;
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int yr = 0; yr < ZM_RPW + 2; ++yr)
+#pragma unroll
+          for (int hf = 0; hf < 2; ++hf) {
+            // B operand: voxel 16 hf + n16 + dx - 1 of the row, ci = 8 kq .. 8 kq + 7
+            const bf16x8 av =
+                *reinterpret_cast<const bf16x8*>(&P[((wv * ZM_RPW + yr) * ZM_PX + hf * 16 + n16 + dx) * CP + kq * 8]);
+#pragma unroll
+            for (int r = 0; r < ZM_RPW; ++r) {
+              const int dy = yr - r;
+              if (dy < 0 || dy > 2) continue;
+              acc[r][hf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[(dz * 3 + dy) * 3 + dx], av, acc[r][hf], 0, 0, 0);
+            }
+          }
+    }
+    // acc[r][hf][q] holds output channel 4 kq + q of voxel (x0 + 16 hf + n16, y0 + row, z)
+#pragma unroll
+    for (int r = 0; r < ZM_RPW; ++r) {
+      const int ly = wv * ZM_RPW + r;
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + hf * 16 + n16;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int co = 4 * kq + q;
+          if (co < cout) {
+            const float val = acc[r][hf][q] + sbias[co];
+            if (MODE == CONV_OUT_F32_CF) {
+              y_out[((int64_t)b * cout + co) * G3 + sv] = val;
+            } else {
+              const int64_t o = ((int64_t)b * G3 + sv) * cout + co;
+              y_out[o] = addend ? val + addend[o] : val;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (zi + 1 < ZM_ZC) {
+      store_plane(zi % 3, pre);
       __syncthreads();
     }
   }
@@ -437,6 +580,18 @@ int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t 
   return AUGUR_OK;
 }
 
+template <int MODE>
+int launch_fwd_narrow(const uint16_t* x, const uint16_t* w, const float* bias, int64_t B, int64_t G, int cout,
+                      float* y, const float* addend, hipStream_t s, const char* name) {
+  const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / ZM_ZC);
+  AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
+  TimedScope ts_(name, s);
+  hipLaunchKernelGGL((k_conv3_zm_narrow<MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
+                     cout, y, addend);
+  AUGUR_LAUNCHED(name);
+  return AUGUR_OK;
+}
+
 }  // namespace augur
 
 extern "C" {
@@ -452,6 +607,8 @@ int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const fl
   switch (mode) {
     case CONV_OUT_F32_CF:
       AUGUR_REQUIRE(y_out && bias, AUGUR_EINVAL, "conv: mode 0 needs y_out and bias");
+      if (cin == 32 && cout <= 16)
+        return launch_fwd_narrow<CONV_OUT_F32_CF>(x, w, bias, B, G, cout, y_out, nullptr, s, "conv_fwd_out");
       return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr, s, "conv_fwd_out")
                        : launch_fwd<32, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr, s, "conv_fwd_out");
     case CONV_GELU:
@@ -463,6 +620,8 @@ int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const fl
       return launch_fwd<32, CONV_RES_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, s, "conv_fwd_res");
     case CONV_OUT_F32_CL:
       AUGUR_REQUIRE(y_out, AUGUR_EINVAL, "conv: mode 3 needs y_out");
+      if (cin == 32 && cout <= 16)
+        return launch_fwd_narrow<CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, y_out, addend, s, "conv_bwd_data");
       return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend, s, "conv_bwd_data")
                        : launch_fwd<32, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend, s, "conv_bwd_data");
     default:

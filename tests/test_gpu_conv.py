@@ -51,6 +51,16 @@ def test_conv_forward_modes(dev, cin):
         h2 = S.conv3_fwd(xc, wk, b.to(dev), S.MODE_RES_GELU, 32)
         exp = x.double().permute(0, 2, 3, 4, 1) + F.gelu(refz, approximate='tanh')
         assert (h2.cpu().double() - exp).abs().max() < 1.5e-2 * exp.abs().max()
+        # narrow outputs (cout <= 16 runs the 16-channel kernel), channels-first and channels-last
+        y16 = S.conv3_fwd(xc, wk, b.to(dev), S.MODE_OUT_CF, 16).cpu().double()
+        assert (y16 - ref[:, :16]).abs().max() < 1e-4 * ref.abs().max()
+        y17 = S.conv3_fwd(xc, wk, b.to(dev), S.MODE_OUT_CF, 17).cpu().double()
+        assert (y17 - ref[:, :17]).abs().max() < 1e-4 * ref.abs().max()
+        ref_nb = (ref - b.double().view(1, 32, 1, 1, 1)).permute(0, 2, 3, 4, 1)   # no bias in mode 3
+        add = torch.randn(B, G, G, G, 5, generator=g)
+        y5 = S.conv3_fwd(xc, wk, None, S.MODE_OUT_CL, 5, addend=add.to(dev)).cpu().double()
+        exp5 = ref_nb[..., :5] + add.double()
+        assert (y5 - exp5).abs().max() < 1e-4 * exp5.abs().max()
 
 
 def test_conv_data_gradient_is_transposed_conv(dev):
