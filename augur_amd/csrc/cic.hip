@@ -45,6 +45,18 @@ int fill_u64(void* p, int64_t n, unsigned long long v, hipStream_t s) {
   AUGUR_LAUNCHED("fill_u64");
   return AUGUR_OK;
 }
+int device_cu_count() {
+  // per-device cache; a device index beyond the table is queried each call
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (dev >= 0 && dev < 64) cached[dev] = cus;
+  return cus;
+}
+
 int fill_u32(void* p, int64_t n, uint32_t v, hipStream_t s) {
   if (n <= 0) return AUGUR_OK;
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, (uint32_t*)p, n, v);

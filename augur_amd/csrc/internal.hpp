@@ -14,6 +14,9 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* 
                      uint32_t** keys_out, uint32_t** vals_out, hipStream_t s, const uint32_t* gate = nullptr);
 
 int fft_r2c_xy(const float* in, float2* out, int64_t G, int64_t pitch, hipStream_t s);
+
+// compute units of the current device (cached per device; 256 if the query fails)
+int device_cu_count();
 int fft_cols_y(float2* data, int64_t G, int64_t pitch, hipStream_t s);
 
 // Per-launch event timing (see timing.hip).  A TimedScope around a launch

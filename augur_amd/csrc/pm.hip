@@ -292,14 +292,26 @@ __global__ __launch_bounds__(256) void k_tile_items(const uint32_t* __restrict__
   }
 }
 
-// adds a lane's run sums (base cell at tile offsets in key) into the LDS tile
-__device__ __forceinline__ void td_flush(double* acc, const double* a, int key, int G) {
+// adds a lane's run sums (base cell at tile offsets in key) into the LDS tile.
+// Branch-free: a corner outside the tile is redirected to the lane's own trash
+// cell past the tile (never read), so every lane issues the same 8 atomics.
+constexpr int TD_TRASH = 64;  // one trash cell per lane of a wave
+__device__ __forceinline__ void td_flush(double* acc, const double* a, int key, int G, int lane) {
   if (key < 0) return;
   const int ax = key & 1023, ay = (key >> 10) & 1023, az = key >> 20;
+  const int x1 = (ax + 1) & (G - 1), y1 = (ay + 1) & (G - 1), z1 = (az + 1) & (G - 1);
+  const bool vx[2] = {ax < TD_X, x1 < TD_X};
+  const int cx[2] = {ax, x1};
+  const bool vy[2] = {ay < TD_Y, y1 < TD_Y};
+  const int cy[2] = {ay * TD_X, y1 * TD_X};
+  const bool vz[2] = {az < TD_Z, z1 < TD_Z};
+  const int cz[2] = {az * (TD_X * TD_Y), z1 * (TD_X * TD_Y)};
+  const int trash = TD_CELLS + lane;
 #pragma unroll
   for (int o = 0; o < 8; ++o) {
-    const int cx = (ax + (o & 1)) & (G - 1), cy = (ay + ((o >> 1) & 1)) & (G - 1), cz = (az + (o >> 2)) & (G - 1);
-    if (cx < TD_X && cy < TD_Y && cz < TD_Z && a[o] != 0.0) atomicAdd(&acc[(cz * TD_Y + cy) * TD_X + cx], a[o]);
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+    const bool in = vx[ox] && vy[oy] && vz[oz];
+    atomicAdd(&acc[in ? cz[oz] + cy[oy] + cx[ox] : trash], a[o]);
   }
 }
 
@@ -314,12 +326,13 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     unsigned long long* __restrict__ fix, float* __restrict__ rho) {
   // unit counts of the fixed-point contributions: integers below 2^53 in FP64,
   // so the LDS float atomics and register sums are exact in any order
-  __shared__ double acc[TD_CELLS];
+  __shared__ double acc[TD_CELLS + TD_TRASH];
   __shared__ uint32_t seg_lo[TD_MAXSEG];
   __shared__ uint32_t seg_pre[TD_MAXSEG + 1];
   __shared__ uint16_t blk_seg[TD_NBLK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int E = fixed_exponent(st->maxbits, n);
+  const float valE0 = ldexpf(__fmul_rn(mass_value, scale), E);
   const uint32_t total_items = item_start[ntiles];
   for (uint32_t item = blockIdx.x; item < total_items; item += gridDim.x) {
     const DepItem it = info[item];
@@ -396,13 +409,13 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
           if (!in) continue;
           const int key = (az << 20) | (ay << 10) | ax;
           if (key != rkey) {
-            td_flush(acc, run, rkey, G);
+            td_flush(acc, run, rkey, G, lane);
 #pragma unroll
             for (int o = 0; o < 8; ++o) run[o] = 0.0;
             rkey = key;
           }
           // units rint(m * scale * w * 2^E), the power-of-two scaling applied to m * scale
-          const float valE = ldexpf(__fmul_rn(pm[u], scale), E);
+          const float valE = mass ? ldexpf(__fmul_rn(pm[u], scale), E) : valE0;
 #pragma unroll
           for (int o = 0; o < 8; ++o) {
             const float wx = (o & 1) ? wv[0] : __fsub_rn(1.0f, wv[0]);
@@ -411,7 +424,7 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
             run[o] += (double)rintf(__fmul_rn(valE, __fmul_rn(__fmul_rn(wx, wy), wz)));
           }
         }
-        td_flush(acc, run, rkey, G);
+        td_flush(acc, run, rkey, G, lane);
       }
     }
     __syncthreads();

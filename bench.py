@@ -104,12 +104,16 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
     print(f"[bench] surrogate {a.steps} steps in {sec * 1e3:.2f} ms -> {value:.4e} particle-steps/s "
           f"(fell back: {fell_back}, probe residual {res:.3e})", file=sys.stderr, flush=True)
     conv = {}
-    flops = {"conv_fwd_res": 2 * 27 * 32 * 32, "conv_fwd_gelu": 2 * 27 * 16 * 32, "conv_fwd_out": 2 * 27 * 32 * 32}
+    # algorithmic flop per voxel (2 * 27 * cin * cout of the layer; the kernels pad the 4 input
+    # channels of the first layer to 16 and the 3 outputs of the last to 16, which is not counted)
+    cin0 = sur.net.cin
+    flops = {"conv_fwd_res": 2 * 27 * 32 * 32, "conv_fwd_gelu": 2 * 27 * cin0 * 32, "conv_fwd_out": 2 * 27 * 32 * 3}
     for name, fpv in flops.items():
         if name in timing:
             ms, cnt = timing[name]
             tf = fpv * G ** 3 / (ms / cnt / 1e3) / 1e12
-            conv[name] = {"us_avg": ms / cnt * 1e3, "tflops": tf, "frac": tf / MFMA_BF16_PEAK_TFLOPS}
+            conv[name] = {"us_avg": ms / cnt * 1e3, "flop_per_voxel": fpv, "tflops": tf,
+                          "frac": tf / MFMA_BF16_PEAK_TFLOPS}
     res_k = conv.get("conv_fwd_res")
     return {
         "value": value, "unit": "particle-steps/s", "ms_per_step": sec * 1e3 / a.steps,
