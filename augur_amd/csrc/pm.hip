@@ -151,7 +151,7 @@ constexpr int TD_NW = TD_THREADS / 64;
 // a single run (x pieces: 2 partial + 1 full; y and z: 2 partial + 1 full)
 constexpr int TD_REGY = TD_Y + TD_LO + TD_HI, TD_REGZ = TD_Z + TD_LO + TD_HI;
 constexpr int TD_MAXSEG = 2 * TD_REGY * TD_REGZ + 2 * TD_REGZ + 3;
-constexpr uint32_t TD_CHUNK = 32768;  // most particle visits per work item
+constexpr uint32_t TD_CHUNK = 16384;  // most particle visits per work item
 constexpr int TD_K = 8;               // consecutive visits per lane in a wave batch
 constexpr int TD_BLK = 64;            // visits per entry of the visit -> segment table
 constexpr int TD_NBLK = TD_CHUNK / TD_BLK;  // table entries per item
@@ -770,7 +770,7 @@ constexpr int TG_THREADS = 512;
 constexpr int TG_SUB = 4;  // z layers per piece
 constexpr int TG_NSUB = TD_Z / TG_SUB;
 constexpr uint32_t TG_CHUNK = 8192;  // most particles per work item
-constexpr int TG_U = 2;              // particles per thread in flight
+constexpr int TG_U = 4;              // particles per thread in flight
 constexpr int TG_RX = TD_X + 1 + 2 * DMAX, TG_RY = TD_Y + 1 + 2 * DMAX, TG_RZ = TG_SUB + 1 + 2 * DMAX;
 constexpr int TG_REG = TG_RX * TG_RY * TG_RZ;
 
@@ -813,44 +813,36 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
   const int64_t G3 = (int64_t)G * G * G;
   float vmax = 0.0f;
   const uint32_t total_items = item_start[npieces];
-  for (uint32_t item = blockIdx.x; item < total_items; item += gridDim.x) {
+  // XCD-coherent order: workgroups are dealt to the 8 XCDs round-robin, so
+  // workgroup b takes item (b % 8) * per + b / 8 and each XCD works through
+  // one contiguous range of pieces, whose shared region halos then hit in
+  // that XCD's L2 (half the fetched bytes of the round-robin order)
+  const uint32_t per = (total_items + 7) / 8;
+  const uint32_t xcd = blockIdx.x % 8, slot0 = blockIdx.x / 8, slots = (gridDim.x + 7 - xcd) / 8;
+  const uint32_t lim = min(total_items, (xcd + 1) * per);
+  for (uint32_t item = xcd * per + slot0; item < lim; item += slots) {
     const DepItem it = info[item];
     const TileOrigin to = tile_origin((int)it.tile / TG_NSUB, G);
     const int x0 = to.x0, y0 = to.y0, z0 = to.z0 + ((int)it.tile % TG_NSUB) * TG_SUB;
-    // force planes over the grown piece, origin (x0, y0, z0) - DMAX wrapped;
-    // the loads of a batch are issued before its LDS writes
-    {
-      constexpr int PER = (TG_REG + TG_THREADS - 1) / TG_THREADS, BATCH = 4;
+    // force planes over the grown piece, origin (x0, y0, z0) - DMAX wrapped:
+    // every load of the region issued at once
+    constexpr int PER = (TG_REG + TG_THREADS - 1) / TG_THREADS;
+    float r[PER][3];
 #pragma unroll
-      for (int k0 = 0; k0 < PER; k0 += BATCH) {
-        float r[BATCH][3];
-#pragma unroll
-        for (int k = 0; k < BATCH; ++k) {
-          const int c = tid + (k0 + k) * TG_THREADS;
-          if (k0 + k < PER && c < TG_REG) {
-            const int xx = c % TG_RX, yy = (c / TG_RX) % TG_RY, zz = c / (TG_RX * TG_RY);
-            const int gx = (x0 - DMAX + xx) & (G - 1), gy = (y0 - DMAX + yy) & (G - 1), gz = (z0 - DMAX + zz) & (G - 1);
-            const int64_t cell = ((int64_t)gz * G + gy) * G + gx;
-            r[k][0] = force[cell];
-            r[k][1] = force[G3 + cell];
-            r[k][2] = force[2 * G3 + cell];
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < BATCH; ++k) {
-          const int c = tid + (k0 + k) * TG_THREADS;
-          if (k0 + k < PER && c < TG_REG) {
-            fx[c] = r[k][0];
-            fy[c] = r[k][1];
-            fz[c] = r[k][2];
-          }
-        }
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * TG_THREADS;
+      if (c < TG_REG) {
+        const int xx = c % TG_RX, yy = (c / TG_RX) % TG_RY, zz = c / (TG_RX * TG_RY);
+        const int gx = (x0 - DMAX + xx) & (G - 1), gy = (y0 - DMAX + yy) & (G - 1), gz = (z0 - DMAX + zz) & (G - 1);
+        const int64_t cell = ((int64_t)gz * G + gy) * G + gx;
+        r[k][0] = force[cell];
+        r[k][1] = force[G3 + cell];
+        r[k][2] = force[2 * G3 + cell];
       }
     }
-    __syncthreads();
-    // the piece's particles: contiguous slots, TG_U per thread in flight
-    for (uint32_t v0 = it.vlo + tid; v0 < it.vhi; v0 += TG_THREADS * TG_U) {
-      float p[TG_U][3], vel0[TG_U][3];
+    // the item's first particles load alongside
+    float p[TG_U][3], vel0[TG_U][3];
+    auto load_particles = [&](uint32_t v0) {
 #pragma unroll
       for (int u = 0; u < TG_U; ++u) {
         const uint32_t v = v0 + u * TG_THREADS;
@@ -862,6 +854,21 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
           }
         }
       }
+    };
+    load_particles(it.vlo + tid);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * TG_THREADS;
+      if (c < TG_REG) {
+        fx[c] = r[k][0];
+        fy[c] = r[k][1];
+        fz[c] = r[k][2];
+      }
+    }
+    __syncthreads();
+    // the piece's particles: contiguous slots, TG_U per thread in flight
+    for (uint32_t v0 = it.vlo + tid; v0 < it.vhi; v0 += TG_THREADS * TG_U) {
+      if (v0 != it.vlo + tid) load_particles(v0);
 #pragma unroll
       for (int u = 0; u < TG_U; ++u) {
         const uint32_t v = v0 + u * TG_THREADS;
