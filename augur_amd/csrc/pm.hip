@@ -748,6 +748,7 @@ __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, flo
             float x = p[u][d] + dt * v;
             if (x >= L) x -= L;
             if (x < 0.0f) x += L;
+            if (x >= L) x = 0.0f;  // -tiny + L rounds to L
             pos[3 * q + d] = x;
             vmax = fmaxf(vmax, fabsf(v));
           }
@@ -915,6 +916,7 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
             float x = p[u][d] + dt * vv;
             if (x >= L) x -= L;
             if (x < 0.0f) x += L;
+            if (x >= L) x = 0.0f;  // -tiny + L rounds to L
             pos[3 * q + d] = x;
             vmax = fmaxf(vmax, fabsf(vv));
           }
@@ -937,6 +939,7 @@ __global__ void k_kick_drift(float* __restrict__ pos, float* __restrict__ vel, c
       float x = pos[3 * i + d] + dt * v;
       if (x >= L) x -= L;
       if (x < 0.0f) x += L;
+      if (x >= L) x = 0.0f;  // -tiny + L rounds to L
       vel[3 * i + d] = v;
       pos[3 * i + d] = x;
       vmax = fmaxf(vmax, fabsf(v));

@@ -78,8 +78,9 @@ class PeriodicPM:
         m, mv = _mass_args(mass, mass_value, n, self.device)
         st = PMState(pos, vel, acc, ids, m, mv, _ws(self.workspace_bytes(n), self.device))
         self.sort(st)
-        if compute_acc:
-            st.acc = self.accel(st.pos, st.mass if st.mass is not None else None, st.mass_value)
+        if compute_acc:   # a zero-length step through the state's own workspace: acc = PM(x), x and v unchanged
+            self.step_begin(st, 0.0, False)
+            self.step_end(st, 0.0, True)
         return st
 
     def sort(self, st):

@@ -1,4 +1,6 @@
-"""Seeded 3D scenes for the PM workloads (host numpy, FP64 then FP32).
+"""Seeded 3D scenes for the PM workloads (host numpy, FP64 then FP32; the
+N = 2^30 scene is drawn on the GPU with torch, since its FP64 host arrays
+would take tens of GB).
 
 plummer:  r = a / sqrt(u^(-2/3) - 1), isotropic direction, radii >= rmax redrawn.
 clumpy:   a fraction uniform in the unit box, the rest split equally over
@@ -60,3 +62,31 @@ def targets(n, t, seed=4738):
     """Seeded target subset (sorted indices) for the FP64 oracle."""
     rng = np.random.default_rng(seed + 1)
     return np.sort(rng.choice(n, size=min(t, n), replace=False)).astype(np.int64)
+
+
+def clumpy_device(n, device, seed=4738, n_clumps=4096, sigma=0.02, frac_uniform=0.5, sigma_v=0.01, chunk=1 << 26):
+    """The clumpy mixture drawn on the GPU (torch generator, FP32, written in
+    chunks): a fraction uniform in the unit box, the rest split equally over
+    Gaussian clumps with uniform-random centres, wrapped periodically into
+    [0, 1); Maxwellian velocities.  Returns (pos, vel), each (n, 3) FP32 on
+    device.  Same distribution as clumpy(), not the same numbers."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    pos = torch.empty(n, 3, dtype=torch.float32, device=device)
+    vel = torch.empty(n, 3, dtype=torch.float32, device=device)
+    nu = int(round(n * frac_uniform))
+    cen = torch.rand(n_clumps, 3, generator=g, device=device, dtype=torch.float64)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        m = b - a
+        p = torch.rand(m, 3, generator=g, device=device, dtype=torch.float64)
+        lo = max(a, nu)
+        if lo < b:   # the clump part of this chunk
+            idx = torch.arange(lo - nu, b - nu, device=device) % n_clumps
+            p[lo - a:] = cen[idx] + sigma * torch.randn(b - lo, 3, generator=g, device=device, dtype=torch.float64)
+        p = torch.remainder(p, 1.0).to(torch.float32)
+        p = torch.where(p >= 1.0, torch.zeros_like(p), p)
+        pos[a:b] = p
+        vel[a:b] = sigma_v * torch.randn(m, 3, generator=g, device=device, dtype=torch.float32)
+    return pos, vel
