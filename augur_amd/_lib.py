@@ -50,6 +50,8 @@ SIGNATURES = {
     "augur_pm_fields": (c_int, [P, P, c_float, c_int64, c_int64, c_float, c_float, P, P, P, c_size_t, P]),
     "augur_pm_kdk": (c_int, [P, P, P, P, P, c_float, c_int64, c_int64, c_float, c_float, c_float, c_int, P,
                              c_size_t, P]),
+    "augur_pm_kdk_diag": (c_int, [P, P, P, P, P, c_float, c_int64, c_int64, c_float, c_float, c_float, c_int, P, P,
+                                  c_size_t, P]),
     "augur_pm_step_begin": (c_int, [P, P, P, P, P, c_float, c_int64, c_int64, c_float, c_float, c_float, c_int, P,
                                     c_size_t, P]),
     "augur_pm_step_end": (c_int, [P, P, P, c_int64, c_int64, c_float, c_float, c_int, P, c_size_t, P]),

@@ -665,6 +665,27 @@ __device__ __forceinline__ void block_max_to(float v, uint32_t* target) {
   }
 }
 
+// four FP64 sums over the block in a fixed order (lane tree, then waves in
+// order), written by thread 0
+__device__ __forceinline__ void block_sum4_to(double (&v)[4], double* out) {
+  __shared__ double red4[16][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red4[w][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double m = 0.0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m += red4[i][c];
+      out[c] = m;
+    }
+}
+
 constexpr int PM_GU = 1;  // particles in flight per thread in the gather
 
 // CIC gather of the force planes fused with the kicks and the drift (grid
@@ -742,10 +763,10 @@ __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, flo
       if (do_kick) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          float v = vv[u][d] + hdt * a[u][d];
+          float v = __fmaf_rn(hdt, a[u][d], vv[u][d]);
           if (drift) {
-            v = v + hdt * a[u][d];
-            float x = p[u][d] + dt * v;
+            v = __fmaf_rn(hdt, a[u][d], v);
+            float x = __fmaf_rn(dt, v, p[u][d]);
             if (x >= L) x -= L;
             if (x < 0.0f) x += L;
             if (x >= L) x = 0.0f;  // -tiny + L rounds to L
@@ -808,11 +829,15 @@ template <bool P2>
 __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
     float* __restrict__ pos, float* __restrict__ vel, float* __restrict__ acc, const float* __restrict__ force,
     int64_t n, int G, float h, float inv_h, float hdt, float dt, float L, int drift,
-    const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, int npieces, PmDev* __restrict__ st) {
+    const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, int npieces, PmDev* __restrict__ st,
+    const float* __restrict__ mass, float mass_value, double* __restrict__ dpart) {
   __shared__ float fx[TG_REG], fy[TG_REG], fz[TG_REG];
   const int tid = threadIdx.x;
   const int64_t G3 = (int64_t)G * G * G;
   float vmax = 0.0f;
+  // FP64 sums of m v and m v^2 / 2 at the synchronised (full-step) velocity,
+  // when dpart is given: one partial per workgroup, summed in a fixed order
+  double dsum[4] = {0.0, 0.0, 0.0, 0.0};
   const uint32_t total_items = item_start[npieces];
   // XCD-coherent order: workgroups are dealt to the 8 XCDs round-robin, so
   // workgroup b takes item (b % 8) * per + b / 8 and each XCD works through
@@ -908,12 +933,21 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
 #pragma unroll
           for (int d = 0; d < 3; ++d) acc[3 * q + d] = a[d];
         }
+        float vs[3];  // closing half kick: the synchronised velocity of this step
+#pragma unroll
+        for (int d = 0; d < 3; ++d) vs[d] = __fmaf_rn(hdt, a[d], vel0[u][d]);
+        if (dpart) {
+          const double m = mass ? (double)mass[q] : (double)mass_value;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) dsum[d] += m * (double)vs[d];
+          dsum[3] += 0.5 * m * ((double)vs[0] * vs[0] + (double)vs[1] * vs[1] + (double)vs[2] * vs[2]);
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          float vv = vel0[u][d] + hdt * a[d];
+          float vv = vs[d];
           if (drift) {
-            vv = vv + hdt * a[d];
-            float x = p[u][d] + dt * vv;
+            vv = __fmaf_rn(hdt, a[d], vv);
+            float x = __fmaf_rn(dt, vv, p[u][d]);
             if (x >= L) x -= L;
             if (x < 0.0f) x += L;
             if (x >= L) x = 0.0f;  // -tiny + L rounds to L
@@ -927,16 +961,41 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
     __syncthreads();  // LDS is reused by the next item
   }
   if (drift) block_max_to(vmax, &st->vmax_bits);
+  if (dpart) block_sum4_to(dsum, dpart + (int64_t)blockIdx.x * 4);
 }
 
+// sums the gather's per-workgroup FP64 partials in a fixed order: out[0..3]
+constexpr int DIAG_THREADS = 1024;
+__global__ __launch_bounds__(DIAG_THREADS) void k_diag_finish(const double* __restrict__ dpart, int64_t nblk,
+                                                              double* __restrict__ out) {
+  double acc4[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t b = threadIdx.x; b < nblk; b += DIAG_THREADS)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc4[c] += dpart[b * 4 + c];
+  __shared__ double red[DIAG_THREADS][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) red[threadIdx.x][c] = acc4[c];
+  __syncthreads();
+  for (int w = DIAG_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) red[threadIdx.x][c] += red[threadIdx.x + w][c];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = red[0][threadIdx.x];
+}
+
+// Every kick and drift is one explicit FMA (v + (dt/2) a, x + dt v) in all the
+// stepping kernels, so a step gives the same bits whether its kicks are fused
+// into the gather or run here.
 __global__ void k_kick_drift(float* __restrict__ pos, float* __restrict__ vel, const float* __restrict__ acc, int64_t n,
                              float hdt, float dt, float L, PmDev* __restrict__ st) {
   float vmax = 0.0f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      float v = vel[3 * i + d] + hdt * acc[3 * i + d];
-      float x = pos[3 * i + d] + dt * v;
+      float v = __fmaf_rn(hdt, acc[3 * i + d], vel[3 * i + d]);
+      float x = __fmaf_rn(dt, v, pos[3 * i + d]);
       if (x >= L) x -= L;
       if (x < 0.0f) x += L;
       if (x >= L) x = 0.0f;  // -tiny + L rounds to L
@@ -981,6 +1040,7 @@ struct PmBufs {
   // gather plan: work items over the pieces (TG_SUB z layers of a tile)
   uint32_t* gitems;      // npieces + 1
   DepItem* ginfo;        // max_gitems
+  double* dpart;         // 4 * max_gitems: per-workgroup FP64 diagnostics of the gather
 };
 
 static int64_t pm_tiles(int64_t G) { return (G / TD_X) * (G / TD_Y) * (G / TD_Z); }
@@ -1013,6 +1073,7 @@ static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   b.iblk = c.take<uint16_t>(pm_max_items(n, G) * TD_NBLK);
   b.gitems = c.take<uint32_t>(pm_pieces(G) + 1);
   b.ginfo = c.take<DepItem>(pm_max_gitems(n, G));
+  b.dpart = c.take<double>(4 * pm_max_gitems(n, G));
   return c.ok();
 }
 
@@ -1195,6 +1256,45 @@ static int pm_force_grid(const float* pos, const uint32_t* perm, const float* ma
 
 }  // namespace augur
 
+// gather + kicks (+ drift unless last) of one step; with diag, the FP64
+// sums [m vx, m vy, m vz, m v^2 / 2] at the step's synchronised velocity go
+// to diag[0..3] (mass NULL: mass_value each)
+static int pm_step_end(float* pos, float* vel, float* acc, const float* mass, float mass_value, int64_t n, int64_t G,
+                       float L, float dt, int last, double* diag, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_pm(n, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(pos && vel && acc, AUGUR_EINVAL, "pm_step_end: bad arguments");
+  hipStream_t s = as_stream(stream);
+  Carver c(ws, ws_bytes);
+  PmBufs b;
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_step_end: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
+  const float h = (float)((double)L / (double)G);
+  const float hdt = 0.5f * dt;
+  const int64_t npc = pm_pieces(G);
+  const int64_t ng = pm_max_gitems(n, G);
+  double* dpart = diag ? b.dpart : nullptr;
+  {
+    TimedScope ts_("gather_kdk", s);
+    const dim3 gg((unsigned)ng);
+    if (pow2_float(h))
+      hipLaunchKernelGGL(k_gather_tiles<true>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
+                         (int)G, h, 1.0f / h, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
+                         (const DepItem*)b.ginfo, (int)npc, b.st, mass, mass_value, dpart);
+    else
+      hipLaunchKernelGGL(k_gather_tiles<false>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
+                         (int)G, h, 0.0f, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
+                         (const DepItem*)b.ginfo, (int)npc, b.st, mass, mass_value, dpart);
+    AUGUR_LAUNCHED("gather_kdk");
+  }
+  if (diag) {
+    TimedScope ts_("diag_finish", s);
+    hipLaunchKernelGGL(k_diag_finish, dim3(1), dim3(DIAG_THREADS), 0, s, (const double*)dpart, ng, diag);
+    AUGUR_LAUNCHED("diag_finish");
+  }
+  return AUGUR_OK;
+}
+
 extern "C" {
 
 size_t augur_pm_workspace_bytes(int64_t n, int64_t G) {
@@ -1317,29 +1417,7 @@ int augur_pm_step_begin(float* pos, float* vel, float* acc, float* mass, int32_t
 
 int augur_pm_step_end(float* pos, float* vel, float* acc, int64_t n, int64_t G, float L, float dt, int last, void* ws,
                       size_t ws_bytes, void* stream) {
-  int rc = check_pm(n, G, L);
-  if (rc) return rc;
-  AUGUR_REQUIRE(pos && vel && acc, AUGUR_EINVAL, "pm_step_end: bad arguments");
-  hipStream_t s = as_stream(stream);
-  Carver c(ws, ws_bytes);
-  PmBufs b;
-  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_step_end: workspace too small (%zu < %zu)", ws_bytes,
-                c.used);
-  const float h = (float)((double)L / (double)G);
-  const float hdt = 0.5f * dt;
-  const int64_t npc = pm_pieces(G);
-  TimedScope ts_("gather_kdk", s);
-  const dim3 gg((unsigned)pm_max_gitems(n, G));
-  if (pow2_float(h))
-    hipLaunchKernelGGL(k_gather_tiles<true>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
-                       (int)G, h, 1.0f / h, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
-                       (const DepItem*)b.ginfo, (int)npc, b.st);
-  else
-    hipLaunchKernelGGL(k_gather_tiles<false>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
-                       (int)G, h, 0.0f, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
-                       (const DepItem*)b.ginfo, (int)npc, b.st);
-  AUGUR_LAUNCHED("gather_kdk");
-  return AUGUR_OK;
+  return pm_step_end(pos, vel, acc, nullptr, 0.0f, n, G, L, dt, last, nullptr, ws, ws_bytes, stream);
 }
 
 int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* force_offset) {
@@ -1351,6 +1429,23 @@ int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* forc
   carve_pm(c, n, G, b);
   *rho_offset = (size_t)(reinterpret_cast<uintptr_t>(b.rho) - base);
   *force_offset = (size_t)(reinterpret_cast<uintptr_t>(b.force) - base);
+  return AUGUR_OK;
+}
+
+int augur_pm_kdk_diag(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n,
+                      int64_t G, float L, float Gconst, float dt, int nsteps, double* diag, void* ws, size_t ws_bytes,
+                      void* stream) {
+  AUGUR_REQUIRE(nsteps >= 0 && diag, AUGUR_EINVAL, "pm_kdk_diag: bad arguments");
+  int rc = check_pm(n, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(pos && vel && acc, AUGUR_EINVAL, "pm_kdk_diag: bad arguments");
+  for (int st = 0; st < nsteps; ++st) {
+    rc = augur_pm_step_begin(pos, vel, acc, mass, ids, mass_value, n, G, L, Gconst, dt, st == 0, ws, ws_bytes, stream);
+    if (rc) return rc;
+    rc = pm_step_end(pos, vel, acc, mass, mass_value, n, G, L, dt, st == nsteps - 1, diag + 4 * (int64_t)st, ws,
+                     ws_bytes, stream);
+    if (rc) return rc;
+  }
   return AUGUR_OK;
 }
 

@@ -88,8 +88,16 @@ class PeriodicPM:
         _lib.call("augur_pm_prepare", _p(st.pos), _p(st.vel), _p(st.acc), _p(st.mass), _p(st.ids), st.n, self.G,
                   self.L, _p(st.ws), st.ws.numel(), _stream(self.device))
 
-    def kdk(self, st, dt, steps):
-        """`steps` KDK leapfrog steps in place (v += dt/2 a; x += dt v; a = PM(x); v += dt/2 a)."""
+    def kdk(self, st, dt, steps, diag=False):
+        """`steps` KDK leapfrog steps in place (v += dt/2 a; x += dt v; a = PM(x); v += dt/2 a).
+        With diag, returns a float64 (steps, 4) tensor of [sum m v (3), sum m |v|^2 / 2]
+        after each step, summed in the fused gather (the steps are the same)."""
+        if diag:
+            out = torch.empty(int(steps), 4, dtype=torch.float64, device=self.device)
+            _lib.call("augur_pm_kdk_diag", _p(st.pos), _p(st.vel), _p(st.acc), _p(st.mass), _p(st.ids),
+                      st.mass_value, st.n, self.G, self.L, self.Gconst, float(dt), int(steps), _p(out), _p(st.ws),
+                      st.ws.numel(), _stream(self.device))
+            return out
         _lib.call("augur_pm_kdk", _p(st.pos), _p(st.vel), _p(st.acc), _p(st.mass), _p(st.ids), st.mass_value, st.n,
                   self.G, self.L, self.Gconst, float(dt), int(steps), _p(st.ws), st.ws.numel(),
                   _stream(self.device))

@@ -220,6 +220,17 @@ int augur_pm_fields(const float* pos, const float* mass, float mass_value, int64
                     float* rho, float* force, void* ws, size_t ws_bytes, void* stream);
 int augur_pm_kdk(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n, int64_t G,
                  float L, float Gconst, float dt, int nsteps, void* ws, size_t ws_bytes, void* stream);
+/* augur_pm_kdk with per-step FP64 diagnostics fused into the gather:
+ * diag[4 s .. 4 s + 3] = [sum m vx, sum m vy, sum m vz, sum m |v|^2 / 2] at
+ * the synchronised velocity after step s (the closing half kick), summed in
+ * FP64 in a fixed order (deterministic).  The stepping is identical to
+ * augur_pm_kdk, bit for bit.  Replaces the per-tick FP64 momentum and
+ * kinetic-energy sums of the large-N loop (scripts/scatter_bh_1b.py:150-186,
+ * diagnostics(), run each tick by Sim; also scatter_scaling_ops.py:90-103
+ * for the kinetic part), which take a separate pass over the state. */
+int augur_pm_kdk_diag(float* pos, float* vel, float* acc, float* mass, int32_t* ids, float mass_value, int64_t n,
+                      int64_t G, float L, float Gconst, float dt, int nsteps, double* diag, void* ws, size_t ws_bytes,
+                      void* stream);
 /* One KDK step of augur_pm_kdk in two halves, so a caller can change the
  * force field between them (the surrogate's correction):
  * augur_pm_step_begin(): with first != 0 the opening half kick and drift from

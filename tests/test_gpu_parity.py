@@ -392,3 +392,32 @@ def test_energy_momentum_reduction(dev):
     np.testing.assert_allclose(out[3], 0.5 * (md * (vd ** 2).sum(1)).sum(), rtol=1e-12)
     np.testing.assert_allclose(out[4], 0.5 * (md * phi).sum(), rtol=1e-10)
     np.testing.assert_allclose(out[5], md.sum(), rtol=1e-12)
+
+
+def test_pm_kdk_diag_fused_sums(dev):
+    # the per-step FP64 sums fused into the gather equal separate reductions of
+    # the state after each single step (numpy FP64 on the same FP32 values);
+    # the fused call steps the state exactly as augur_pm_kdk does, and single
+    # steps (kicks in the separate kick-drift kernel) give the same bits
+    from augur_amd.pm import PeriodicPM
+    rng = np.random.default_rng(16)
+    n, G, dt, steps = 50000, 64, 2e-3, 4
+    pos = scene3d(rng, n, sigma=0.08)
+    vel = (0.3 * rng.normal(size=(n, 3))).astype(F32)
+    mass = rng.uniform(0.5, 1.5, n).astype(F32) / n
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    a, b, c = (pm.load(T(pos, dev), T(vel, dev), T(mass, dev)) for _ in range(3))
+    d = pm.kdk(a, dt, steps, diag=True).cpu().numpy()
+    pm.kdk(b, dt, steps)
+    for x, y in ((a.pos, b.pos), (a.vel, b.vel), (a.acc, b.acc), (a.ids, b.ids)):
+        assert torch.equal(x, y)
+    for s in range(steps):
+        pm.kdk(c, dt, 1)
+        v = c.vel.cpu().numpy().astype(np.float64)
+        m = c.mass.cpu().numpy().astype(np.float64)
+        p = (m[:, None] * v).sum(0)
+        ke = 0.5 * (m * (v ** 2).sum(1)).sum()
+        scale = (m[:, None] * np.abs(v)).sum(0)
+        assert (np.abs(d[s, :3] - p) <= 1e-12 * scale).all(), (s, d[s], p)
+        assert abs(d[s, 3] - ke) <= 1e-12 * ke, (s, d[s, 3], ke)
+    assert torch.equal(a.vel, c.vel) and torch.equal(a.pos, c.pos)
