@@ -130,9 +130,39 @@ size_t scan_workspace_bytes(int64_t n) {
 }
 
 // out may alias in.
+// one workgroup for a short array, staged in LDS: coalesced load, each
+// thread scans a contiguous chunk, chunk totals scanned across the block,
+// coalesced store (in place allowed)
+constexpr int64_t SC_SMALL = 16384;  // one launch of k_scan_small up to here
+__global__ __launch_bounds__(SC_THREADS) void k_scan_small(const uint32_t* in, uint32_t* out, int64_t n,
+                                                           const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  __shared__ uint32_t buf[SC_SMALL];
+  for (int64_t i = threadIdx.x; i < n; i += SC_THREADS) buf[i] = in[i];
+  __syncthreads();
+  const int64_t chunk = (n + SC_THREADS - 1) / SC_THREADS;
+  const int64_t lo = (int64_t)threadIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  uint32_t local = 0;
+  for (int64_t i = lo; i < hi; ++i) local += buf[i];
+  uint32_t tot;
+  uint32_t pre = block_exclusive_scan_u32(local, &tot);
+  for (int64_t i = lo; i < hi; ++i) {
+    const uint32_t v = buf[i];
+    buf[i] = pre;
+    pre += v;
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += SC_THREADS) out[i] = buf[i];
+}
+
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s, const uint32_t* gate) {
   if (n <= 0) return AUGUR_OK;
   int64_t nb = (n + SC_TILE - 1) / SC_TILE;
+  if (nb > 1 && n <= SC_SMALL) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SC_THREADS), 0, s, in, out, n, gate);
+    AUGUR_LAUNCHED("scan_small");
+    return AUGUR_OK;
+  }
   if (nb == 1) {
     hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SC_THREADS), 0, s, in, out, n, (uint32_t*)nullptr, gate);
     AUGUR_LAUNCHED("scan_block");

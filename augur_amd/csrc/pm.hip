@@ -87,8 +87,9 @@ __global__ void k_pm_set_maxbits(PmDev* st, const float* mass, float mass_value,
 
 // ------------------------------------------------------------- gated sort
 __global__ void k_base_keys(const float* __restrict__ pos, int64_t n, int G, float h, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ vals, const uint32_t* gate) {
+                            uint32_t* __restrict__ vals, uint32_t* __restrict__ ngaps, const uint32_t* gate) {
   if (gate && *gate == 0u) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ngaps = 0u;  // for k_cell_starts, later in the same sort
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     int c[3];
 #pragma unroll
@@ -116,16 +117,70 @@ __global__ void k_copy(const T* __restrict__ src, T* __restrict__ dst, int64_t n
     dst[i] = src[i];
 }
 
-__global__ void k_zero_u32(uint32_t* p, int64_t n, const uint32_t* gate) {
+// The state arrays permuted in one pass into tmp (pos 3n, vel 3n, then ids n
+// and mass n when present), and copied back in one pass.
+__global__ void k_permute_state(const float* __restrict__ pos, const float* __restrict__ vel,
+                                const int32_t* __restrict__ ids, const float* __restrict__ mass,
+                                const uint32_t* __restrict__ perm, int64_t n, float* __restrict__ tmp,
+                                const uint32_t* gate) {
   if (gate && *gate == 0u) return;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0u;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = perm[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      tmp[3 * i + d] = pos[3 * j + d];
+      tmp[3 * n + 3 * i + d] = vel[3 * j + d];
+    }
+    if (ids) reinterpret_cast<int32_t*>(tmp)[6 * n + i] = ids[j];
+    if (mass) tmp[7 * n + i] = mass[j];
+  }
+}
+__global__ void k_restore_state(const float* __restrict__ tmp, int64_t n, float* __restrict__ pos,
+                                float* __restrict__ vel, int32_t* __restrict__ ids, float* __restrict__ mass,
+                                const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pos[3 * i + d] = tmp[3 * i + d];
+      vel[3 * i + d] = tmp[3 * n + 3 * i + d];
+    }
+    if (ids) ids[i] = reinterpret_cast<const int32_t*>(tmp)[6 * n + i];
+    if (mass) mass[i] = tmp[7 * n + i];
+  }
 }
 
-__global__ void k_count_cells(const uint32_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ cnt,
-                              const uint32_t* gate) {
+// cell_start[c] = the first sorted slot whose key is >= c, for c in [0, ncells],
+// straight from the sorted keys: slot i (0..n) owns the cells (key[i-1], key[i]]
+// (key[-1] = -1, key[n] = ncells).  Runs of more than CS_SPAN empty cells go
+// to a list that k_cell_fill writes with whole workgroups.
+constexpr uint32_t CS_SPAN = 64;
+__global__ void k_cell_starts(const uint32_t* __restrict__ ks, int64_t n, uint32_t ncells,
+                              uint32_t* __restrict__ cell_start, uint32_t* __restrict__ gaps,
+                              uint32_t* __restrict__ ngaps, const uint32_t* gate) {
   if (gate && *gate == 0u) return;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&cnt[keys[i]], 1u);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t lo = i == 0 ? 0u : ks[i - 1] + 1u;
+    const uint32_t hi = i == n ? ncells : ks[i];
+    if (hi < lo) continue;  // a repeated key: no cell starts here
+    if (hi - lo < CS_SPAN) {
+      for (uint32_t c = lo; c <= hi; ++c) cell_start[c] = (uint32_t)i;
+    } else {
+      const uint32_t k = atomicAdd(ngaps, 1u);
+      gaps[3 * k] = lo;
+      gaps[3 * k + 1] = hi;
+      gaps[3 * k + 2] = (uint32_t)i;
+    }
+  }
+}
+__global__ void k_cell_fill(const uint32_t* __restrict__ gaps, const uint32_t* __restrict__ ngaps,
+                            uint32_t* __restrict__ cell_start, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  const uint32_t ng = *ngaps;
+  for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x) {
+    const uint32_t lo = gaps[3 * g], hi = gaps[3 * g + 1], v = gaps[3 * g + 2];
+    for (uint32_t c = lo + threadIdx.x; c <= hi; c += blockDim.x) cell_start[c] = v;
+  }
 }
 
 // ---------------------------------------------------------- tile deposit
@@ -1029,7 +1084,9 @@ struct PmBufs {
   uint32_t* vals;        // n
   void* rws;             // radix sort scratch
   void* scan_ws;         // scan scratch (G^3 + 1)
-  float* tmp;            // 3n
+  float* tmp;            // 8n: the permuted state (pos, vel, ids, mass)
+  uint32_t* gaps;        // 3 per run of empty cells longer than CS_SPAN
+  uint32_t* ngaps;
   uint32_t* perm;        // n (for unsorted force evaluation)
   uint32_t* items;       // ntiles + 1: work items per tile, then their exclusive scan
   unsigned long long* fix;  // G^3: int64 tile images of heavy tiles (kept zero between launches)
@@ -1063,7 +1120,9 @@ static bool carve_pm(Carver& c, int64_t n, int64_t G, PmBufs& b) {
   b.vals = c.take<uint32_t>(n);
   b.rws = c.take<char>(radix_workspace_bytes(n));
   b.scan_ws = c.take<char>(scan_workspace_bytes(G3 + 1));
-  b.tmp = c.take<float>(3 * n);
+  b.tmp = c.take<float>(8 * n);
+  b.gaps = c.take<uint32_t>(3 * ((G3 + 1) / CS_SPAN + 2));
+  b.ngaps = c.take<uint32_t>(1);
   b.perm = c.take<uint32_t>(n);
   b.items = c.take<uint32_t>(pm_tiles(G) + 1);
   b.fix = c.take<unsigned long long>(G3);
@@ -1103,18 +1162,18 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   const float h = (float)((double)L / (double)G);
   const int64_t G3 = G * G * G;
   const dim3 g(blocks_for(n, 256, 1024)), bl(256);
-  hipLaunchKernelGGL(k_base_keys, g, bl, 0, s, pos, n, (int)G, h, b.keys, b.vals, gate);
+  hipLaunchKernelGGL(k_base_keys, g, bl, 0, s, pos, n, (int)G, h, b.keys, b.vals, b.ngaps, gate);
   AUGUR_LAUNCHED("base_keys");
   uint32_t *ks, *perm;
   int rc = radix_sort_pairs(b.keys, b.vals, n, 3 * ilog2(G), b.rws, radix_workspace_bytes(n), &ks, &perm, s, gate);
   if (rc) return rc;
-  // cell_start = exclusive scan of per-cell counts
-  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(G3 + 1, 256, 1024)), bl, 0, s, b.cell_start, G3 + 1, gate);
-  AUGUR_LAUNCHED("zero_cells");
-  hipLaunchKernelGGL(k_count_cells, g, bl, 0, s, ks, n, b.cell_start, gate);
-  AUGUR_LAUNCHED("count_cells");
-  rc = exclusive_scan_u32(b.cell_start, b.cell_start, G3 + 1, b.scan_ws, s, gate);
-  if (rc) return rc;
+  // cell_start from the sorted keys
+  hipLaunchKernelGGL(k_cell_starts, dim3(blocks_for(n + 1, 256, 4096)), bl, 0, s, (const uint32_t*)ks, n,
+                     (uint32_t)G3, b.cell_start, b.gaps, b.ngaps, gate);
+  AUGUR_LAUNCHED("cell_starts");
+  hipLaunchKernelGGL(k_cell_fill, dim3(1024), bl, 0, s, (const uint32_t*)b.gaps, (const uint32_t*)b.ngaps,
+                     b.cell_start, gate);
+  AUGUR_LAUNCHED("cell_fill");
   // work items: the deposit's over the tile regions grown for the largest
   // drift bound, the gather's over the tile cores
   const int64_t ntiles = pm_tiles(G);
@@ -1141,25 +1200,17 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
     AUGUR_LAUNCHED("copy_perm");
     return AUGUR_OK;
   }
-  float* arrs3[3] = {pos, vel, acc};
-  for (float* a : arrs3) {
-    if (!a) continue;
-    hipLaunchKernelGGL((k_permute<float, 3>), g, bl, 0, s, a, b.tmp, perm, n, gate);
+  hipLaunchKernelGGL(k_permute_state, g, bl, 0, s, (const float*)pos, (const float*)vel, (const int32_t*)ids,
+                     (const float*)mass, perm, n, b.tmp, gate);
+  AUGUR_LAUNCHED("permute_state");
+  hipLaunchKernelGGL(k_restore_state, g, bl, 0, s, (const float*)b.tmp, n, pos, vel, ids, mass, gate);
+  AUGUR_LAUNCHED("restore_state");
+  if (acc) {  // only when the caller's acc must follow (prepare)
+    hipLaunchKernelGGL((k_permute<float, 3>), g, bl, 0, s, (const float*)acc, b.tmp, perm, n, gate);
     AUGUR_LAUNCHED("permute3");
-    hipLaunchKernelGGL(k_copy<float>, dim3(blocks_for(3 * n, 256, 1024)), bl, 0, s, b.tmp, a, 3 * n, gate);
+    hipLaunchKernelGGL(k_copy<float>, dim3(blocks_for(3 * n, 256, 1024)), bl, 0, s, (const float*)b.tmp, acc, 3 * n,
+                       gate);
     AUGUR_LAUNCHED("copy3");
-  }
-  if (mass) {
-    hipLaunchKernelGGL((k_permute<float, 1>), g, bl, 0, s, mass, b.tmp, perm, n, gate);
-    AUGUR_LAUNCHED("permute1");
-    hipLaunchKernelGGL(k_copy<float>, g, bl, 0, s, b.tmp, mass, n, gate);
-    AUGUR_LAUNCHED("copy1");
-  }
-  if (ids) {
-    hipLaunchKernelGGL((k_permute<int32_t, 1>), g, bl, 0, s, ids, (int32_t*)b.tmp, perm, n, gate);
-    AUGUR_LAUNCHED("permute_ids");
-    hipLaunchKernelGGL(k_copy<int32_t>, g, bl, 0, s, (const int32_t*)b.tmp, ids, n, gate);
-    AUGUR_LAUNCHED("copy_ids");
   }
   return AUGUR_OK;
 }
