@@ -368,20 +368,32 @@ __global__ void k_dep_absmax(const float* __restrict__ vel, const float* __restr
   }
 }
 
+// scene of particle i in a ragged batch: the last b with starts[b] <= i
+__device__ __forceinline__ int64_t scene_of(const int64_t* __restrict__ starts, int64_t B, int64_t i) {
+  int64_t lo = 0, hi = B;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (starts[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// starts (may be NULL): ragged batch, scene b holds particles [starts[b], starts[b+1]);
+// otherwise B scenes of N particles each
 template <int D>
 __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos, const float* __restrict__ vel,
                                                    const float* __restrict__ mass, float mass_value,
-                                                   const float* __restrict__ mask, int64_t B, int64_t N, int G,
+                                                   const float* __restrict__ mask, int64_t B, int64_t N,
+                                                   const int64_t* __restrict__ starts, int64_t tot, int G,
                                                    float off, float h, int periodic, float scale, int C,
                                                    const unsigned* __restrict__ maxbits,
                                                    unsigned long long* __restrict__ acc) {
-  const int64_t tot = B * N;
   int E[4];
   for (int c = 0; c < C; ++c) E[c] = fixed_exponent(maxbits[c], tot);
   int64_t GD = 1;
   for (int d = 0; d < D; ++d) GD *= G;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = i / N;
+    const int64_t b = starts ? scene_of(starts, B, i) : i / N;
     float mm = mass ? mass[i] : mass_value;
     if (mask) mm = __fmul_rn(mm, mask[i]);
     float valE[4];
@@ -683,15 +695,60 @@ int augur_cic_deposit(const float* pos, const float* vel, const float* mass, flo
     TimedScope ts_("dep_fixed", s);
     if (D == 2)
       hipLaunchKernelGGL(k_dep_fixed<2>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, pos, vel, mass, mass_value,
-                         mask, B, N, (int)G, off, h, periodic, scale, C, maxbits, acc);
+                         mask, B, N, (const int64_t*)nullptr, tot, (int)G, off, h, periodic, scale, C, maxbits, acc);
     else
       hipLaunchKernelGGL(k_dep_fixed<3>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, pos, vel, mass, mass_value,
-                         mask, B, N, (int)G, off, h, periodic, scale, C, maxbits, acc);
+                         mask, B, N, (const int64_t*)nullptr, tot, (int)G, off, h, periodic, scale, C, maxbits, acc);
     AUGUR_LAUNCHED("dep_fixed");
   }
   TimedScope ts_("dep_convert", s);
   hipLaunchKernelGGL(k_dep_convert, dim3(blocks_for(B * C * GD, 256)), dim3(256), 0, s, acc, out, B, C, GD,
                      tot > 0 ? tot : 1, maxbits);
+  AUGUR_LAUNCHED("dep_convert");
+  return AUGUR_OK;
+}
+
+int augur_cic_deposit_ragged(const float* pos, const float* vel, const float* mass, float mass_value,
+                             const int64_t* starts, int64_t B, int64_t total, int D, int64_t G, float off, float h,
+                             int periodic, float scale, float* out, void* ws, size_t ws_bytes, void* stream) {
+  AUGUR_REQUIRE(D == 2 || D == 3, AUGUR_EINVAL, "cic_deposit_ragged: D must be 2 or 3");
+  AUGUR_REQUIRE(B >= 1 && total >= 0 && G >= 2 && h > 0.0f && starts, AUGUR_EINVAL, "cic_deposit_ragged: bad sizes");
+  AUGUR_REQUIRE(!periodic || is_pow2(G), AUGUR_EUNSUPPORTED, "cic_deposit_ragged: periodic needs power-of-two G");
+  AUGUR_REQUIRE(G <= 4096, AUGUR_EUNSUPPORTED, "cic_deposit_ragged: G too large");
+  const int C = vel ? 1 + D : 1;
+  int64_t GD = 1;
+  for (int d = 0; d < D; ++d) GD *= G;
+  hipStream_t s = as_stream(stream);
+  Carver c(ws, ws_bytes);
+  unsigned long long* acc = c.take<unsigned long long>(B * C * GD);
+  unsigned* maxbits = c.take<unsigned>(4);
+  AUGUR_REQUIRE(c.ok(), AUGUR_EWORKSPACE, "cic_deposit_ragged: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  AUGUR_REQUIRE(out && (total == 0 || pos), AUGUR_EINVAL, "cic_deposit_ragged: null pointer");
+  int rc = fill_u64(acc, B * C * GD, 0ull, s);
+  if (rc) return rc;
+  rc = fill_u32(maxbits, 4, 0u, s);
+  if (rc) return rc;
+  if (total > 0) {
+    {
+      TimedScope ts_("dep_absmax", s);
+      hipLaunchKernelGGL(k_dep_absmax, dim3(blocks_for(total, 256, 2048)), dim3(256), 0, s, vel, mass, mass_value,
+                         (const float*)nullptr, total, D, C, scale, maxbits);
+      AUGUR_LAUNCHED("dep_absmax");
+    }
+    TimedScope ts_("dep_fixed", s);
+    if (D == 2)
+      hipLaunchKernelGGL(k_dep_fixed<2>, dim3(blocks_for(total, 256)), dim3(256), 0, s, pos, vel, mass, mass_value,
+                         (const float*)nullptr, B, (int64_t)0, starts, total, (int)G, off, h, periodic, scale, C,
+                         maxbits, acc);
+    else
+      hipLaunchKernelGGL(k_dep_fixed<3>, dim3(blocks_for(total, 256)), dim3(256), 0, s, pos, vel, mass, mass_value,
+                         (const float*)nullptr, B, (int64_t)0, starts, total, (int)G, off, h, periodic, scale, C,
+                         maxbits, acc);
+    AUGUR_LAUNCHED("dep_fixed");
+  }
+  TimedScope ts_("dep_convert", s);
+  hipLaunchKernelGGL(k_dep_convert, dim3(blocks_for(B * C * GD, 256)), dim3(256), 0, s, acc, out, B, C, GD,
+                     total > 0 ? total : 1, maxbits);
   AUGUR_LAUNCHED("dep_convert");
   return AUGUR_OK;
 }

@@ -102,6 +102,28 @@ def cic_deposit(pos, mass, G, h, off, periodic, vel=None, mask=None, scale=None,
     return out
 
 
+def cic_deposit_ragged(pos, starts, G, h, off, periodic, mass=None, vel=None, scale=None, mass_value=1.0):
+    """Ragged batch: pos (total, D) with scene b = rows [starts[b], starts[b+1]),
+    starts an int64 tensor of B+1 offsets on the device; mass (total,) or
+    None (mass_value each) -> (B, C, G, .., G) float32, one launch for all scenes."""
+    pos = _c(pos, F32)
+    dev = _dev(pos)
+    total, D = pos.shape
+    starts = _c(starts, torch.int64)
+    B = starts.shape[0] - 1
+    mass = _c(mass, F32)
+    vel = _c(vel, F32)
+    C = 1 if vel is None else 1 + D
+    if scale is None:
+        scale = 1.0 / h ** D
+    out = torch.empty((B, C) + (G,) * D, dtype=F32, device=dev)
+    wsb = _lib.size("augur_cic_deposit_workspace_bytes", B, D, G, C)
+    ws = _ws(wsb, dev)
+    _lib.call("augur_cic_deposit_ragged", _p(pos), _p(vel), _p(mass), float(mass_value), _p(starts), B, total, D,
+              int(G), float(off), float(h), int(bool(periodic)), float(scale), _p(out), _p(ws), wsb, _stream(dev))
+    return out
+
+
 def cic_gather(field, pos, h, off, periodic):
     """field (B, C, G..G), pos (B, N, D) -> (B, N, C)."""
     field = _c(field, F32)

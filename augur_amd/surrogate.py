@@ -17,6 +17,8 @@ voxels, GELU derivative) and Adam on the FP32 masters.
 """
 import math
 
+import numpy as np
+
 import torch
 
 from . import _lib, ops
@@ -139,7 +141,14 @@ class LocalNet3D:
 
     # ----------------------------------------------------------- weights
     def sync_bf16(self):
-        self.wk = [kernel_weights(w, pad16(w.shape[2])) for w in self.w]
+        """BF16 kernel copies of the FP32 masters, rewritten in place once they
+        exist (a captured graph keeps reading the same buffers)."""
+        wk = [kernel_weights(w, pad16(w.shape[2])) for w in self.w]
+        if getattr(self, "wk", None) is not None and all(a.shape == b.shape for a, b in zip(self.wk, wk)):
+            for a, b in zip(self.wk, wk):
+                a.copy_(b)
+        else:
+            self.wk = wk
 
     def parameters(self):
         return self.w + self.b
@@ -149,8 +158,14 @@ class LocalNet3D:
                 "cout": self.cout, "layers": self.layers, "step": self.step_count}
 
     def load_state_dict(self, sd):
-        self.w = [w.to(self.device) for w in sd["w"]]
-        self.b = [b.to(self.device) for b in sd["b"]]
+        # in place when the shapes match, so a captured graph sees the new values
+        for name in ("w", "b"):
+            cur, new = getattr(self, name), [t.to(self.device) for t in sd[name]]
+            if len(cur) == len(new) and all(a.shape == b.shape for a, b in zip(cur, new)):
+                for a, b in zip(cur, new):
+                    a.copy_(b)
+            else:
+                setattr(self, name, new)
         self.step_count = sd.get("step", 0)
         self.sync_bf16()
 
@@ -360,4 +375,81 @@ class PMSurrogate:
         dy = (dfield * self.force_scale)[None].contiguous()
         grads = self.net.backward(dy)
         self.net.adam_step(grads, lr=lr)
+        return loss
+
+
+# ------------------------------------------------------------ batched training
+def ragged_batch(scenes, device):
+    """Concatenate scenes of uneven sizes: (pos (total, 3) FP32, starts (B+1,)
+    int64, mass (total,) with m = 1/N_b in scene b), all on the device."""
+    sizes = [int(p.shape[0]) for p in scenes]
+    starts = torch.tensor([0] + list(np.cumsum(sizes)), dtype=torch.int64, device=device)
+    pos = torch.cat([torch.as_tensor(p, dtype=torch.float32).reshape(-1, 3).to(device) for p in scenes]).contiguous()
+    mass = torch.cat([torch.full((n,), 1.0 / n, dtype=torch.float32, device=device) for n in sizes])
+    return pos, starts, mass
+
+
+def _batch_grids(sur, pos, starts, mass):
+    """PM grids of a ragged batch: every scene's density in one deposit launch,
+    then batched FFT, Green's function and gradient: (rho (B, G, G, G),
+    force (B, 3, G, G, G))."""
+    G = sur.G
+    rho = ops.cic_deposit_ragged(pos, starts, G, sur.h, 0.0, True, mass=mass)[:, 0]
+    spec = ops.rfftn(rho, s=(G, G, G))
+    acc = ops.poisson_green_grad(spec, G, sur.L, sur.pm.Gconst, scale=1.0 / G ** 3)
+    return rho, ops.irfftn(acc, s=(G, G, G), scale=1.0)
+
+
+def batch_grads(sur, pos, starts, mass, tpos, a_true, inv_norm, scene_density):
+    """Loss and parameter gradients of the correction net on a ragged batch
+    with no host synchronisation, so it can be captured in a HIP graph.
+
+    pos/starts/mass: the ragged batch (ragged_batch); tpos (B, T, 3) target
+    positions; a_true (B, T, 3) FP64 target accelerations; inv_norm a 1-element
+    FP64 tensor 1 / sum |a_true|^2; scene_density (B,) mean density of each
+    scene (its mass / L^3).  The loss is sum |a - a_true|^2 * inv_norm over
+    the whole batch (as train_step, summed over scenes).  Returns (loss (1,)
+    FP64 tensor, grads as LocalNet3D.backward)."""
+    assert sur.mode == "correction" and sur.rho_scale is not None, "calibrate the surrogate first"
+    rho, force = _batch_grids(sur, pos, starts, mass)
+    contrast = (rho / scene_density.view(-1, 1, 1, 1) - 1.0) * sur.rho_scale
+    x = torch.cat([contrast[:, None], force / sur.force_scale], 1).contiguous()
+    y = sur.net.forward(x, save=True)
+    f = (force + y * sur.force_scale).contiguous()
+    a = ops.cic_gather(f, tpos, sur.h, 0.0, True)
+    r = a.double() - a_true
+    g = (2.0 * inv_norm * r).float().contiguous()
+    # loss by the library's FP64 reduction (sum m |v|^2 / 2 with m = 1)
+    loss = 2.0 * inv_norm * ops.energy_momentum_f64(r.float().reshape(-1, 3), None, 1.0)[3:4]
+    dfield = ops.cic_deposit(tpos, None, sur.G, sur.h, 0.0, True, vel=g, scale=1.0, mass_value=1.0)[:, 1:4]
+    grads = sur.net.backward((dfield * sur.force_scale).contiguous())
+    return loss, grads
+
+
+class BatchGraph:
+    """batch_grads captured once in a HIP graph for fixed shapes and replayed
+    with each new batch copied into its static inputs, the pattern of the
+    reference's captured force evaluation (scripts/scatter_field.py:567-589)
+    applied to the training step (scripts/train_scatter_field.py:158-255).
+    The Adam update runs after each replay (its step count lives on the host)."""
+
+    def __init__(self, sur, pos, starts, mass, tpos, a_true, inv_norm, scene_density):
+        self.sur = sur
+        self.inp = [t.clone() for t in (pos, starts, mass, tpos, a_true, inv_norm, scene_density)]
+        batch_grads(sur, *self.inp)          # eager warm-up: allocations and kernel loads
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.loss, self.grads = batch_grads(sur, *self.inp)
+        torch.cuda.synchronize()
+
+    def grads_for(self, pos, starts, mass, tpos, a_true, inv_norm, scene_density):
+        for dst, src in zip(self.inp, (pos, starts, mass, tpos, a_true, inv_norm, scene_density)):
+            dst.copy_(src)
+        self.graph.replay()
+        return self.loss, self.grads
+
+    def step(self, *batch, lr=3e-4):
+        loss, grads = self.grads_for(*batch)
+        self.sur.net.adam_step(grads, lr=lr)
         return loss

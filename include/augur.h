@@ -79,6 +79,17 @@ size_t augur_cic_deposit_workspace_bytes(int64_t B, int D, int64_t G, int C);
 int augur_cic_deposit(const float* pos, const float* vel, const float* mass, float mass_value, const float* mask,
                       int64_t B, int64_t N, int D, int64_t G, float off, float h, int periodic, float scale, float* out,
                       void* ws, size_t ws_bytes, void* stream);
+/* The same fixed-point deposit for a ragged batch: scene b holds particles
+ * [starts[b], starts[b+1]) of pos (total = starts[B] particles, starts an
+ * int64 device array of B+1 entries), one launch for all scenes; out is
+ * (B, C, G^D) as above, workspace as augur_cic_deposit_workspace_bytes(B, D,
+ * G, C).  Replaces the padded, masked batch of ScatterField.scatter
+ * (scripts/scatter_field.py:237-250 with mask) when the scenes' particle
+ * counts differ (the training batches of scripts/train_scatter_field.py:
+ * 158-255). */
+int augur_cic_deposit_ragged(const float* pos, const float* vel, const float* mass, float mass_value,
+                             const int64_t* starts, int64_t B, int64_t total, int D, int64_t G, float off, float h,
+                             int periodic, float scale, float* out, void* ws, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------ a6 gather */
 
