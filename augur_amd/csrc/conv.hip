@@ -396,22 +396,38 @@ template <int CIN>
 __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __restrict__ x,
                                                               const uint16_t* __restrict__ dz, int B, int G,
                                                               int64_t ntiles, float* __restrict__ part) {
+  constexpr int NB = CIN / 32 > 0 ? CIN / 32 : 1;
+  constexpr int XCH = HZ * HY * HX * (CIN / 8);           // 16-byte chunks of the x tile (with halo)
+  constexpr int XPER = (XCH + CONV_THREADS - 1) / CONV_THREADS;
+  constexpr int DCH = CT_Z * CT_Y * CT_X * 4;              // 16-byte chunks of the dz tile
+  constexpr int DPER = DCH / CONV_THREADS;
   __shared__ __attribute__((aligned(16))) uint16_t xl[CIN * HZ * HY * XS];
   __shared__ __attribute__((aligned(16))) uint16_t dl[32 * CT_Z * CT_Y * DS];
+  __shared__ float bred[4][32];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i = lane & 31, kh = lane >> 5;
   const int ntx = G / CT_X, nty = G / CT_Y, ntz = G / CT_Z;
   const int64_t G3 = (int64_t)G * G * G;
-  f32x16 acc[DW_TAPS_PER_WAVE][CIN / 32 > 0 ? CIN / 32 : 1];
+  f32x16 acc[DW_TAPS_PER_WAVE][NB];
 #pragma unroll
   for (int k = 0; k < DW_TAPS_PER_WAVE; ++k)
 #pragma unroll
-    for (int n = 0; n < (CIN / 32 > 0 ? CIN / 32 : 1); ++n)
+    for (int n = 0; n < NB; ++n)
 #pragma unroll
       for (int q = 0; q < 16; ++q) acc[k][n][q] = 0.0f;
-  float dbias = 0.0f;  // lanes 0..31 of wave 0 own co = lane
+  // bias: db[co] = sum_v dz[v][co], as MFMAs of dz^T with a ones operand,
+  // a quarter of the K steps of each tile per wave
+  f32x16 accb;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) accb[q] = 0.0f;
+  bf16x8 ones;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
 
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // the next tile's chunks travel global -> registers (in flight during the
+  // current tile's MFMAs) -> LDS, transposed to channel-major
+  u32x4 rx[XPER], rd[DPER];
+  auto load_tile = [&](int64_t tile) {
     int64_t t = tile;
     const int bx = (int)(t % ntx);
     t /= ntx;
@@ -420,23 +436,23 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
     const int bz = (int)(t % ntz);
     const int b = (int)(t / ntz);
     const int x0 = bx * CT_X, y0 = by * CT_Y, z0 = bz * CT_Z;
-    __syncthreads();
-    // x tile (with halo) -> channel-major LDS
-    for (int c = tid; c < HZ * HY * HX * (CIN / 8); c += CONV_THREADS) {
-      const int ch = c % (CIN / 8);
-      int v = c / (CIN / 8);
-      const int tx = v % HX;
-      v /= HX;
-      const int ty = v % HY;
-      const int tz = v / HY;
-      const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1), gz = (z0 + tz - 1) & (G - 1);
-      const uint4 raw = *reinterpret_cast<const uint4*>(&x[((int64_t)b * G3 + ((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
-      const uint16_t* e = reinterpret_cast<const uint16_t*>(&raw);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) xl[(((ch * 8 + k) * HZ + tz) * HY + ty) * XS + tx] = e[k];
+    for (int k = 0; k < XPER; ++k) {
+      const int c = tid + k * CONV_THREADS;
+      if (c < XCH) {
+        const int ch = c % (CIN / 8);
+        int v = c / (CIN / 8);
+        const int tx = v % HX;
+        v /= HX;
+        const int ty = v % HY;
+        const int tz = v / HY;
+        const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1), gz = (z0 + tz - 1) & (G - 1);
+        rx[k] = *reinterpret_cast<const u32x4*>(&x[((int64_t)b * G3 + ((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
+      }
     }
-    // dz tile -> channel-major LDS
-    for (int c = tid; c < CT_Z * CT_Y * CT_X * 4; c += CONV_THREADS) {
+#pragma unroll
+    for (int k = 0; k < DPER; ++k) {
+      const int c = tid + k * CONV_THREADS;
       const int ch = c % 4;
       int v = c / 4;
       const int tx = v % CT_X;
@@ -444,34 +460,67 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
       const int ty = v % CT_Y;
       const int tz = v / CT_Y;
       const int64_t gv = ((int64_t)(z0 + tz) * G + (y0 + ty)) * G + x0 + tx;
-      const uint4 raw = *reinterpret_cast<const uint4*>(&dz[((int64_t)b * G3 + gv) * 32 + ch * 8]);
-      const uint16_t* e = reinterpret_cast<const uint16_t*>(&raw);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) dl[(((ch * 8 + k) * CT_Z + tz) * CT_Y + ty) * DS + tx] = e[k];
+      rd[k] = *reinterpret_cast<const u32x4*>(&dz[((int64_t)b * G3 + gv) * 32 + ch * 8]);
     }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int k = 0; k < XPER; ++k) {
+      const int c = tid + k * CONV_THREADS;
+      if (c < XCH) {
+        const int ch = c % (CIN / 8);
+        int v = c / (CIN / 8);
+        const int tx = v % HX;
+        v /= HX;
+        const int ty = v % HY;
+        const int tz = v / HY;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          xl[(((ch * 8 + 2 * e) * HZ + tz) * HY + ty) * XS + tx] = (uint16_t)(rx[k][e] & 0xffffu);
+          xl[(((ch * 8 + 2 * e + 1) * HZ + tz) * HY + ty) * XS + tx] = (uint16_t)(rx[k][e] >> 16);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < DPER; ++k) {
+      const int c = tid + k * CONV_THREADS;
+      const int ch = c % 4;
+      int v = c / 4;
+      const int tx = v % CT_X;
+      v /= CT_X;
+      const int ty = v % CT_Y;
+      const int tz = v / CT_Y;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        dl[(((ch * 8 + 2 * e) * CT_Z + tz) * CT_Y + ty) * DS + tx] = (uint16_t)(rd[k][e] & 0xffffu);
+        dl[(((ch * 8 + 2 * e + 1) * CT_Z + tz) * CT_Y + ty) * DS + tx] = (uint16_t)(rd[k][e] >> 16);
+      }
+    }
+  };
+
+  int64_t tile = blockIdx.x;
+  if (tile < ntiles) load_tile(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile's MFMAs are done with LDS
+    store_tile();
     __syncthreads();
-    if (wv == 0 && lane < 32) {
-      for (int r = 0; r < CT_Z * CT_Y; ++r)
-        for (int vx = 0; vx < CT_X; ++vx) dbias += bf2f(dl[((lane * CT_Z + r / CT_Y) * CT_Y + r % CT_Y) * DS + vx]);
-    }
-    // K loop: 8 rows x 32 voxels = 16 steps of 16 voxels (two per row)
+    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
+    // K loop: 8 rows x 32 voxels = 16 steps of 16 voxels (two per row); each
+    // step's dz fragment feeds the wave's 7 taps, 7 independent MFMA chains
+#pragma unroll 2
+    for (int st = 0; st < 2 * CT_Z * CT_Y; ++st) {
+      const int r = st >> 1, half = st & 1;
+      const int rz = r / CT_Y, ry = r % CT_Y;
+      const int vb = half * 16 + kh * 8;  // first of this lane's 8 voxels
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&dl[((i * CT_Z + rz) * CT_Y + ry) * DS + vb]);
+      if ((st & 3) == wv) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ones, accb, 0, 0, 0);
 #pragma unroll
-    for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
-      const int tap = wv + 4 * k;
-      if (tap >= 27) break;
-      const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
-#pragma unroll 1
-      for (int r = 0; r < CT_Z * CT_Y; ++r) {
-        const int rz = r / CT_Y, ry = r % CT_Y;
+      for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+        const int tap = wv + 4 * k;
+        if (tap < 27) {
+          const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          const int vb = half * 16 + kh * 8;  // first of this lane's 8 voxels
-          bf16x8 av;
-          const uint16_t* ap = &dl[((i * CT_Z + rz) * CT_Y + ry) * DS + vb];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) av[e] = __builtin_bit_cast(__bf16, ap[e]);
-#pragma unroll
-          for (int n = 0; n < (CIN / 32 > 0 ? CIN / 32 : 1); ++n) {
+          for (int n = 0; n < NB; ++n) {
             const int ci = n * 32 + i;
             bf16x8 bv;
             if (ci < CIN) {
@@ -488,6 +537,11 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
       }
     }
   }
+  // bias partial of the wave: accb[q] holds (co = 8 (q/4) + 4 kh + q%4, any column): lanes 0 and 32
+  if (i == 0)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) bred[wv][8 * (q / 4) + 4 * kh + (q % 4)] = accb[q];
+  __syncthreads();
   // partial layout: [wg][27][32 co][CIN] then [wg] bias block of 32
   float* pw = part + (int64_t)blockIdx.x * (27 * 32 * CIN + 32);
 #pragma unroll
@@ -495,7 +549,7 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
     const int tap = wv + 4 * k;
     if (tap >= 27) break;
 #pragma unroll
-    for (int n = 0; n < (CIN / 32 > 0 ? CIN / 32 : 1); ++n) {
+    for (int n = 0; n < NB; ++n) {
       const int ci = n * 32 + i;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
@@ -504,7 +558,7 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
       }
     }
   }
-  if (wv == 0 && lane < 32) pw[27 * 32 * CIN + lane] = dbias;
+  if (tid < 32) pw[27 * 32 * CIN + tid] = ((bred[0][tid] + bred[1][tid]) + bred[2][tid]) + bred[3][tid];
 }
 
 // out_w[j] (j < mw) and out_b[j - mw] = sum over partials in workgroup order

@@ -72,9 +72,9 @@ def conv3_wgrad(x, dz, nwg=None):
     """-> (dw (27, 32, cin) FP32, db (32,) FP32)."""
     dev = _dev(x, dz)
     B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
-    if nwg is None:
+    if nwg is None:   # one persistent workgroup per CU (the LDS tile allows one)
         ntiles = B * (G // 32) * (G // 4) * (G // 2)
-        nwg = max(1, min(1024, ntiles))
+        nwg = max(1, min(256, ntiles))
     dw = torch.empty(27, 32, cin, dtype=torch.float32, device=dev)
     db = torch.empty(32, dtype=torch.float32, device=dev)
     wsb = _lib.size("augur_conv3_wgrad_workspace_bytes", cin, B, G, nwg)

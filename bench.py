@@ -245,10 +245,98 @@ def large_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def train_bench(a):
+    """Batched surrogate training: 64 scenes per batch with particle counts
+    drawn log-uniformly in [2^14, 2^20] (Plummer with a in [0.02, 0.1], or the
+    clumpy mix), 128^3 grids, FP64 Ewald targets on 8192 particles per scene,
+    the gradient step captured in a HIP graph and replayed, Adam lr 3e-4 on
+    FP32 masters.  Reports training iterations/s and the network's MFMA rate."""
+    import numpy as np
+    import torch
+    from augur_amd import _lib, ops, scenes
+    from augur_amd.surrogate import PMSurrogate, BatchGraph, ragged_batch, batch_grads, _batch_grids
+
+    dev = torch.device("cuda:0")
+    B, G, T = a.train_scenes, 128, a.train_targets
+    rng = np.random.default_rng(4738)
+    sizes = np.exp(rng.uniform(np.log(2 ** 14), np.log(2 ** 20), B)).astype(np.int64)
+    sur = PMSurrogate(G, 1.0, 1.0, dev, mode="correction")
+    batches = []
+    t0 = time.perf_counter()
+    for k in range(2):
+        sc = []
+        for i, n in enumerate(sizes):
+            r = np.random.default_rng(1000 * k + i)
+            p = (scenes.plummer(int(n), float(r.uniform(0.02, 0.1)), r) if i % 2 == 0
+                 else scenes.clumpy(int(n), r, n_clumps=int(r.integers(4, 64)), sigma=float(r.uniform(0.01, 0.05))))
+            sc.append(scenes.to_box_f32(p))
+        pos, starts, mass = ragged_batch(sc, dev)
+        tp, at = [], []
+        for i, p in enumerate(sc):
+            tg = torch.from_numpy(scenes.targets(len(p), T, seed=1000 * k + i)).to(dev)
+            pb = pos[int(starts[i]): int(starts[i + 1])]
+            at.append(ops.ewald_f64(pb, 1.0 / len(p), tg, L=1.0, eps=1.0 / G))
+            tp.append(pb[tg])
+        tpos, a_true = torch.stack(tp).contiguous(), torch.stack(at).contiguous()
+        inv_norm = (1.0 / (a_true ** 2).sum()).reshape(1)
+        dens = torch.ones(B, dtype=torch.float32, device=dev)
+        batches.append((pos, starts, mass, tpos, a_true, inv_norm, dens))
+    torch.cuda.synchronize()
+    total = int(sizes.sum())
+    print(f"[bench] {B} scenes x 2 batches ({total} particles per batch, {T} FP64 targets per scene) in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    rho, force = _batch_grids(sur, *batches[0][:3])
+    sur.calibrate(rho[0], force[0])
+    del rho, force
+    # one eager step with the kernel timers on (they are not recorded inside a replay)
+    _lib.timing_enable(True)
+    loss0, grads = batch_grads(sur, *batches[0])
+    sur.net.adam_step(grads, lr=3e-4)
+    torch.cuda.synchronize()
+    timing = _lib.timing_collect()
+    _lib.timing_enable(False)
+    del grads
+    torch.cuda.empty_cache()
+    graph = BatchGraph(sur, *batches[0])
+    for k in range(a.warmup):
+        graph.step(*batches[k % 2], lr=3e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = []
+    for k in range(a.steps):
+        losses.append(graph.step(*batches[k % 2], lr=3e-4).clone())
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    it_s = a.steps / sec
+    # network flop per voxel: forward 2*27*(4*32 + 4*32*32 + 32*3); backward data and weight
+    # gradients twice that (the first layer's data gradient is not needed)
+    fwd = 2 * 27 * (4 * 32 + 4 * 32 * 32 + 32 * 3)
+    flop = 3 * fwd * B * G ** 3
+    conv_ms = sum(v[0] for k, v in timing.items() if k.startswith("conv") or k == "gelu_bwd")
+    out = {
+        "metric": "batched surrogate training iterations/s (64 ragged scenes, 128^3)", "value": it_s,
+        "unit": "iterations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": sec * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (FP32 accumulate, FP32 masters)", "data": "synthetic",
+        "config": {"workload": f"{B} scenes per batch, N log-uniform in [2^14, 2^20] (Plummer a~U[0.02,0.1] / "
+                               f"clumpy), {G}^3 grids, {T} FP64 Ewald targets per scene, gradient step in a "
+                               "HIP graph, Adam lr 3e-4", "scenes": B, "grid": G, "particles_per_batch": total},
+        "scene_steps_per_s": it_s * B, "particles_per_s": it_s * total,
+        "network_tflops": flop / (sec / a.steps) / 1e12, "mfma_frac": flop / (sec / a.steps) / 1e12 / 2500.0,
+        "eager_step_kernels_ms": {k: v[0] for k, v in timing.items()}, "eager_step_conv_ms": conv_ms,
+        "loss_first": float(loss0), "loss_last": float(losses[-1]),
+        "peak_device_bytes": torch.cuda.max_memory_allocated(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["default", "large"], default="default",
-                    help="large: N=2^30 on a 512^3 grid (4096 clumps), PM steps with FP64 reductions")
+    ap.add_argument("--workload", choices=["default", "large", "train"], default="default",
+                    help="large: N=2^30 on a 512^3 grid (4096 clumps), PM steps with FP64 reductions; "
+                         "train: batched surrogate training on 64 ragged scenes at 128^3")
+    ap.add_argument("--train-scenes", type=int, default=64)
+    ap.add_argument("--train-targets", type=int, default=8192)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -271,6 +359,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "train":
+        train_bench(a)
+        return
     if a.workload == "large":
         if a.steps == 10 and "--steps" not in sys.argv:
             a.steps = 5
