@@ -64,6 +64,7 @@ SIGNATURES = {
     "augur_conv3_flip_weights": (c_int, [P, c_int, c_int, c_int, P, P]),
     "augur_gelu_bwd": (c_int, [P, P, c_int64, P, P]),
     "augur_pack_cl_bf16": (c_int, [P, c_int, c_int64, c_int64, c_int, c_float, P, P]),
+    "augur_pack_features_bf16": (c_int, [P, P, c_int64, c_int64, c_float, c_float, c_float, c_int, P, P]),
     "augur_adam_update": (c_int, [P, P, P, P, c_int64, c_float, c_float, c_float, c_float, c_int, c_float, P, P]),
     "augur_timing_enable": (c_int, [c_int]),
     "augur_timing_collect": (c_int, [P, c_size_t, P, P, c_int, P]),

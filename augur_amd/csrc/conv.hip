@@ -594,6 +594,50 @@ __global__ void k_flip_weights(const uint16_t* __restrict__ w, int cin_fwd, int 
 }
 
 // FP32 channels-first [B][C][G^3] * scale -> BF16 channels-last [B][G^3][CP] (zero padded)
+// one voxel per thread: C strided FP32 channel reads (coalesced across the
+// voxels), CP BF16 channels written as 16-byte chunks (CP a multiple of 8)
+__global__ void k_pack_cl8(const float* __restrict__ in, int C, int64_t B, int64_t G3, int CP, float scale,
+                           uint16_t* __restrict__ out) {
+  const int64_t nv = B * G3;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nv; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / G3, v = t - b * G3;
+    const float* src = in + b * C * G3 + v;
+    for (int g = 0; g < CP / 8; ++g) {
+      float f[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int c = g * 8 + k;
+        f[k] = c < C ? src[(int64_t)c * G3] * scale : 0.0f;
+      }
+      u32x4 w;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k] = pack_bf16x2(f[2 * k], f[2 * k + 1]);
+      *reinterpret_cast<u32x4*>(&out[t * CP + g * 8]) = w;
+    }
+  }
+}
+
+// the correction net's input straight from the PM grids: channel 0 =
+// rho * rho_mul + rho_add (the scaled density contrast), channels 1..3 =
+// force planes * force_mul, channels 4..CP-1 zero; BF16 channels-last
+__global__ void k_pack_features(const float* __restrict__ rho, const float* __restrict__ force, int64_t B,
+                                int64_t G3, float rho_mul, float rho_add, float force_mul, int CP,
+                                uint16_t* __restrict__ out) {
+  const int64_t nv = B * G3;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nv; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / G3, v = t - b * G3;
+    const float* fb = force + b * 3 * G3 + v;
+    u32x4 w;
+    w[0] = pack_bf16x2(fmaf(rho[t], rho_mul, rho_add), fb[0] * force_mul);
+    w[1] = pack_bf16x2(fb[G3] * force_mul, fb[2 * G3] * force_mul);
+    w[2] = 0u;
+    w[3] = 0u;
+    *reinterpret_cast<u32x4*>(&out[t * CP]) = w;
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    for (int g = 1; g < CP / 8; ++g) *reinterpret_cast<u32x4*>(&out[t * CP + g * 8]) = z;
+  }
+}
+
 __global__ void k_pack_cl(const float* __restrict__ in, int C, int64_t B, int64_t G3, int CP, float scale,
                           uint16_t* __restrict__ out) {
   const int64_t n = B * G3 * CP;
@@ -739,9 +783,26 @@ int augur_pack_cl_bf16(const float* in, int C, int64_t B, int64_t G, int CP, flo
                        void* stream) {
   AUGUR_REQUIRE(in && out && C >= 1 && C <= CP && B >= 1 && G >= 1, AUGUR_EINVAL, "pack_cl: bad arguments");
   const int64_t G3 = G * G * G;
-  hipLaunchKernelGGL(k_pack_cl, dim3(blocks_for(B * G3 * CP, 256)), dim3(256), 0, as_stream(stream), in, C, B, G3, CP,
-                     scale, out);
+  TimedScope ts_("pack_cl", as_stream(stream));
+  if (CP % 8 == 0)
+    hipLaunchKernelGGL(k_pack_cl8, dim3(blocks_for(B * G3, 256)), dim3(256), 0, as_stream(stream), in, C, B, G3, CP,
+                       scale, out);
+  else
+    hipLaunchKernelGGL(k_pack_cl, dim3(blocks_for(B * G3 * CP, 256)), dim3(256), 0, as_stream(stream), in, C, B, G3,
+                       CP, scale, out);
   AUGUR_LAUNCHED("pack_cl");
+  return AUGUR_OK;
+}
+
+int augur_pack_features_bf16(const float* rho, const float* force, int64_t B, int64_t G, float rho_mul, float rho_add,
+                             float force_mul, int CP, uint16_t* out, void* stream) {
+  AUGUR_REQUIRE(rho && force && out && B >= 1 && G >= 1 && CP >= 8 && CP % 8 == 0, AUGUR_EINVAL,
+                "pack_features: bad arguments");
+  const int64_t G3 = G * G * G;
+  TimedScope ts_("pack_features", as_stream(stream));
+  hipLaunchKernelGGL(k_pack_features, dim3(blocks_for(B * G3, 256)), dim3(256), 0, as_stream(stream), rho, force, B,
+                     G3, rho_mul, rho_add, force_mul, CP, out);
+  AUGUR_LAUNCHED("pack_features");
   return AUGUR_OK;
 }
 

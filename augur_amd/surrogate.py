@@ -179,7 +179,10 @@ class LocalNet3D:
 
     def forward(self, x, save=False):
         """x: FP32 (B, cin, G, G, G) -> FP32 (B, cout, G, G, G)."""
-        xc = pack_cl(x, self.cinp)
+        return self.forward_packed(pack_cl(x, self.cinp), save=save)
+
+    def forward_packed(self, xc, save=False):
+        """xc: BF16 channels-last (B, G, G, G, cinp) input -> FP32 (B, cout, G, G, G)."""
         zs, hs = [], [xc]
         need_z = save
         z = torch.empty(xc.shape[:-1] + (32,), dtype=BF16, device=xc.device) if need_z else None
@@ -294,6 +297,17 @@ class PMSurrogate:
         rho, force = self.pm.fields(pos, mass, mass_value)
         return self.features_from_grids(rho, force), force
 
+    def packed_features(self, rho, force, mean_density):
+        """The correction net's BF16 input straight from the PM grids in one
+        pass (same values as pack_cl(features_from_grids(...)) up to the
+        rounding of the contrast, computed here as one FMA)."""
+        G = self.G
+        out = torch.empty(1, G, G, G, self.net.cinp, dtype=BF16, device=self.device)
+        _lib.call("augur_pack_features_bf16", _p(rho), _p(force), 1, G, float(self.rho_scale / mean_density),
+                  float(-self.rho_scale), float(1.0 / self.force_scale), self.net.cinp, _p(out),
+                  _stream(self.device))
+        return out
+
     def features_from_grids(self, rho, force):
         if self.rho_scale is None:
             self.calibrate(rho, force)
@@ -337,20 +351,30 @@ class PMSurrogate:
         PM at those particles and, past the threshold, the whole call runs
         plain PM.  Returns (fell_back, residual or None)."""
         fell_back, res = False, None
+        mean_density = None
         for s in range(steps):
             self.pm.step_begin(st, dt, s == 0)
             rho, force = self.pm.grids(st)
             if not fell_back:
-                y = self.net.forward(self.features_from_grids(rho, force))[0] * self.force_scale
+                if self.mode == "correction" and self.rho_scale is not None:
+                    if mean_density is None:   # total mass / L^3, once per call
+                        mtot = float(st.mass.double().sum()) if st.mass is not None else st.mass_value * st.n
+                        mean_density = mtot / self.L ** 3
+                    y = self.net.forward_packed(self.packed_features(rho, force, mean_density))[0]
+                else:
+                    y = self.net.forward(self.features_from_grids(rho, force))[0] * self.force_scale
+                    if self.mode == "correction":
+                        y = y / self.force_scale
                 if s == 0 and probe is not None:
                     pp = st.pos[probe].contiguous()
                     a_pm = self.gather(force, pp)
-                    a_sur = self.gather(force + y if self.mode == "correction" else y, pp)
+                    a_sur = self.gather(torch.add(force, y, alpha=self.force_scale) if self.mode == "correction"
+                                        else y, pp)
                     res = rel_rms(a_sur, a_pm)
                     fell_back = res > self.threshold
                 if not fell_back:
                     if self.mode == "correction":
-                        force.add_(y)
+                        force.add_(y, alpha=self.force_scale)
                     else:
                         force.copy_(y)
             self.pm.step_end(st, dt, s == steps - 1)

@@ -300,6 +300,14 @@ int augur_gelu_bwd(const float* dh, const uint16_t* z, int64_t n, uint16_t* dz, 
  * channels C..CP-1 zero. */
 int augur_pack_cl_bf16(const float* in, int C, int64_t B, int64_t G, int CP, float scale, uint16_t* out,
                        void* stream);
+/* The correction net's input from the PM grids in one pass: BF16
+ * channels-last [B][G^3][CP] with channel 0 = rho * rho_mul + rho_add (the
+ * scaled density contrast), channels 1..3 = force * force_mul (force: B x 3
+ * planes of G^3), the rest zero; CP a multiple of 8.  The input stage of
+ * _grid_force (scripts/scatter_field.py:620-666: the density grid and the
+ * field handed to the net), for the 3D correction net. */
+int augur_pack_features_bf16(const float* rho, const float* force, int64_t B, int64_t G, float rho_mul, float rho_add,
+                             float force_mul, int CP, uint16_t* out, void* stream);
 
 /* Adam step on FP32 master weights (bias-corrected, step >= 1), gradient
  * times gscale; p_bf16 (may be NULL) receives the BF16 copy.  Replaces
