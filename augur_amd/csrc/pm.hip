@@ -348,25 +348,27 @@ __global__ __launch_bounds__(256) void k_tile_items(const uint32_t* __restrict__
 }
 
 // adds a lane's run sums (base cell at tile offsets in key) into the LDS tile.
-// Branch-free: a corner outside the tile is redirected to the lane's own trash
-// cell past the tile (never read), so every lane issues the same 8 atomics.
+// Branch-free: a corner outside the tile, and every corner when key < 0 (no
+// run), is redirected to the lane's own trash cell past the tile (never
+// read), so every lane issues the same 8 atomics.  Byte offsets: an axis
+// term outside the tile is TD_OFF, which lifts the sum past every trash
+// cell, and min(sum, trash) then picks the trash cell.
 constexpr int TD_TRASH = 64;  // one trash cell per lane of a wave
+constexpr int TD_OFF = 1 << 20;
 __device__ __forceinline__ void td_flush(double* acc, const double* a, int key, int G, int lane) {
-  if (key < 0) return;
-  const int ax = key & 1023, ay = (key >> 10) & 1023, az = key >> 20;
+  const int ax = key & 1023, ay = (key >> 10) & 1023, az = (key >> 20) & 1023;
   const int x1 = (ax + 1) & (G - 1), y1 = (ay + 1) & (G - 1), z1 = (az + 1) & (G - 1);
-  const bool vx[2] = {ax < TD_X, x1 < TD_X};
-  const int cx[2] = {ax, x1};
-  const bool vy[2] = {ay < TD_Y, y1 < TD_Y};
-  const int cy[2] = {ay * TD_X, y1 * TD_X};
-  const bool vz[2] = {az < TD_Z, z1 < TD_Z};
-  const int cz[2] = {az * (TD_X * TD_Y), z1 * (TD_X * TD_Y)};
-  const int trash = TD_CELLS + lane;
+  const int cx[2] = {ax < TD_X ? ax * 8 : TD_OFF, x1 < TD_X ? x1 * 8 : TD_OFF};
+  const int cy[2] = {ay < TD_Y ? ay * (TD_X * 8) : TD_OFF, y1 < TD_Y ? y1 * (TD_X * 8) : TD_OFF};
+  const bool ok = key >= 0;
+  const int cz[2] = {ok && az < TD_Z ? az * (TD_X * TD_Y * 8) : TD_OFF,
+                     ok && z1 < TD_Z ? z1 * (TD_X * TD_Y * 8) : TD_OFF};
+  const int trash = (TD_CELLS + lane) * 8;
+  char* base = reinterpret_cast<char*>(acc);
 #pragma unroll
   for (int o = 0; o < 8; ++o) {
     const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-    const bool in = vx[ox] && vy[oy] && vz[oz];
-    atomicAdd(&acc[in ? cz[oz] + cy[oy] + cx[ox] : trash], a[o]);
+    atomicAdd(reinterpret_cast<double*>(base + min(cz[oz] + cy[oy] + cx[ox], trash)), a[o]);
   }
 }
 
@@ -421,33 +423,54 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
         while (seg_pre[seg + 1] <= l0) ++seg;
         uint32_t snext = seg_pre[seg + 1];
         int64_t sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
-        float px[TD_K][3], pm[TD_K];
+        // slots first (LDS walk), then each kind of load for all visits
+        // together, so the position loads are in flight at the same time
+        // (a load that depends on another, the perm lookup, is its own pass)
+        int64_t sl[TD_K];
 #pragma unroll
         for (int u = 0; u < TD_K; ++u) {
           const uint32_t ww = l0 + u;
-          pm[u] = mass_value;
-          px[u][0] = px[u][1] = px[u][2] = 0.0f;
+          sl[u] = -1;
           if (ww < l1) {
             while (ww >= snext) {
               ++seg;
               snext = seg_pre[seg + 1];
               sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
             }
-            const int64_t slot = sbase + ww;
-            const int64_t p = perm ? (int64_t)perm[slot] : slot;
-            px[u][0] = pos[3 * p];
-            px[u][1] = pos[3 * p + 1];
-            px[u][2] = pos[3 * p + 2];
-            if (mass) pm[u] = mass[p];
+            sl[u] = sbase + ww;
           }
+        }
+        if (perm) {
+#pragma unroll
+          for (int u = 0; u < TD_K; ++u)
+            if (sl[u] >= 0) sl[u] = (int64_t)perm[sl[u]];
+        }
+        float px[TD_K][3], pm[TD_K];
+#pragma unroll
+        for (int u = 0; u < TD_K; ++u) {
+          px[u][0] = px[u][1] = px[u][2] = 0.0f;
+          if (sl[u] >= 0) {
+            px[u][0] = pos[3 * sl[u]];
+            px[u][1] = pos[3 * sl[u] + 1];
+            px[u][2] = pos[3 * sl[u] + 2];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < TD_K; ++u) pm[u] = mass_value;
+        if (mass) {
+#pragma unroll
+          for (int u = 0; u < TD_K; ++u)
+            if (sl[u] >= 0) pm[u] = mass[sl[u]];
         }
         int rkey = -1;
         double run[8];
 #pragma unroll
         for (int o = 0; o < 8; ++o) run[o] = 0.0;
+        // straight-line per visit: a visit outside the grown tile (or past
+        // the lane's range) keeps the current run and adds zero units, so
+        // only the flush on a change of base cell is a branch
 #pragma unroll
         for (int u = 0; u < TD_K; ++u) {
-          if (l0 + u >= l1) break;
           int i0[3];
           float wv[3];
 #pragma unroll
@@ -460,17 +483,17 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
           // base cell as offsets from the tile origin (wrapped into [0, G));
           // a corner is inside when its offset is below the tile size
           const int ax = (i0[0] - t.x0) & (G - 1), ay = (i0[1] - t.y0) & (G - 1), az = (i0[2] - t.z0) & (G - 1);
-          const bool in = (ax < TD_X || ax == G - 1) && (ay < TD_Y || ay == G - 1) && (az < TD_Z || az == G - 1);
-          if (!in) continue;
-          const int key = (az << 20) | (ay << 10) | ax;
+          const bool in = (sl[u] >= 0) & ((ax < TD_X) | (ax == G - 1)) & ((ay < TD_Y) | (ay == G - 1)) &
+                          ((az < TD_Z) | (az == G - 1));
+          const int key = in ? (az << 20) | (ay << 10) | ax : rkey;
           if (key != rkey) {
-            td_flush(acc, run, rkey, G, lane);
+            if (u > 0) td_flush(acc, run, rkey, G, lane);  // nothing to add before the first visit
 #pragma unroll
             for (int o = 0; o < 8; ++o) run[o] = 0.0;
             rkey = key;
           }
           // units rint(m * scale * w * 2^E), the power-of-two scaling applied to m * scale
-          const float valE = mass ? ldexpf(__fmul_rn(pm[u], scale), E) : valE0;
+          const float valE = in ? (mass ? ldexpf(__fmul_rn(pm[u], scale), E) : valE0) : 0.0f;
 #pragma unroll
           for (int o = 0; o < 8; ++o) {
             const float wx = (o & 1) ? wv[0] : __fsub_rn(1.0f, wv[0]);
@@ -880,18 +903,30 @@ __global__ void k_gather_items(const uint32_t* __restrict__ cell_start, const ui
     info[i0 + c] = DepItem{(uint32_t)pc, min(hi, lo + c * per), min(hi, lo + (c + 1) * per), 0u};
 }
 
-template <bool P2>
+// Raw buffer views of an item's particle arrays: a lane past the item's end
+// gets zeros from a load and its store is dropped by the range check, so the
+// particle loop has no per-lane branches and the loads of the next batch are
+// in flight while the stores of this one drain.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t item_view(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+
+// DRIFT: also the next opening half kick and the drift (pos, vel written);
+// otherwise the closing half kick only (acc, vel written).  DIAG: per-
+// workgroup FP64 sums of m v and m v^2 / 2 at the closing-kick velocity.
+template <bool P2, bool DRIFT, bool DIAG>
 __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
     float* __restrict__ pos, float* __restrict__ vel, float* __restrict__ acc, const float* __restrict__ force,
-    int64_t n, int G, float h, float inv_h, float hdt, float dt, float L, int drift,
+    int64_t n, int G, float h, float inv_h, float hdt, float dt, float L,
     const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, int npieces, PmDev* __restrict__ st,
     const float* __restrict__ mass, float mass_value, double* __restrict__ dpart) {
   __shared__ float fx[TG_REG], fy[TG_REG], fz[TG_REG];
   const int tid = threadIdx.x;
   const int64_t G3 = (int64_t)G * G * G;
   float vmax = 0.0f;
-  // FP64 sums of m v and m v^2 / 2 at the synchronised (full-step) velocity,
-  // when dpart is given: one partial per workgroup, summed in a fixed order
+  // FP64 sums of m v and m v^2 / 2 at the synchronised (full-step) velocity:
+  // one partial per workgroup, summed in a fixed order
   double dsum[4] = {0.0, 0.0, 0.0, 0.0};
   const uint32_t total_items = item_start[npieces];
   // XCD-coherent order: workgroups are dealt to the 8 XCDs round-robin, so
@@ -905,6 +940,11 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
     const DepItem it = info[item];
     const TileOrigin to = tile_origin((int)it.tile / TG_NSUB, G);
     const int x0 = to.x0, y0 = to.y0, z0 = to.z0 + ((int)it.tile % TG_NSUB) * TG_SUB;
+    const uint32_t cnt = it.vhi - it.vlo;  // at most TG_CHUNK
+    const __amdgpu_buffer_rsrc_t rpos = item_view(pos + 3 * (int64_t)it.vlo, 12u * cnt);
+    const __amdgpu_buffer_rsrc_t rvel = item_view(vel + 3 * (int64_t)it.vlo, 12u * cnt);
+    const __amdgpu_buffer_rsrc_t racc = item_view(acc + 3 * (int64_t)it.vlo, 12u * cnt);
+    const __amdgpu_buffer_rsrc_t rmass = item_view(mass ? mass + it.vlo : pos, mass ? 4u * cnt : 0u);
     // force planes over the grown piece, origin (x0, y0, z0) - DMAX wrapped:
     // every load of the region issued at once
     constexpr int PER = (TG_REG + TG_THREADS - 1) / TG_THREADS;
@@ -921,22 +961,23 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
         r[k][2] = force[2 * G3 + cell];
       }
     }
-    // the item's first particles load alongside
-    float p[TG_U][3], vel0[TG_U][3];
-    auto load_particles = [&](uint32_t v0) {
+    // particle j of the item (byte offset 12 j) is lane tid + TG_THREADS u of batch j0
+    float p[TG_U][3], vel0[TG_U][3], pm[TG_U];
+    auto load_particles = [&](uint32_t j0, float (&pp)[TG_U][3], float (&vv)[TG_U][3], float (&mm)[TG_U]) {
 #pragma unroll
       for (int u = 0; u < TG_U; ++u) {
-        const uint32_t v = v0 + u * TG_THREADS;
-        if (v < it.vhi) {
+        const uint32_t off = 12u * (j0 + tid + u * TG_THREADS);
+        const u32x3 a = __builtin_amdgcn_raw_buffer_load_b96(rpos, off, 0, 0);
+        const u32x3 b = __builtin_amdgcn_raw_buffer_load_b96(rvel, off, 0, 0);
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            p[u][d] = pos[3 * (int64_t)v + d];
-            vel0[u][d] = vel[3 * (int64_t)v + d];
-          }
+        for (int d = 0; d < 3; ++d) {
+          pp[u][d] = __uint_as_float(a[d]);
+          vv[u][d] = __uint_as_float(b[d]);
         }
+        mm[u] = DIAG ? __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rmass, off / 3u, 0, 0)) : 0.0f;
       }
     };
-    load_particles(it.vlo + tid);
+    load_particles(0, p, vel0, pm);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int c = tid + k * TG_THREADS;
@@ -947,13 +988,26 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
       }
     }
     __syncthreads();
-    // the piece's particles: contiguous slots, TG_U per thread in flight
-    for (uint32_t v0 = it.vlo + tid; v0 < it.vhi; v0 += TG_THREADS * TG_U) {
-      if (v0 != it.vlo + tid) load_particles(v0);
+    // the first batch has landed by now: waiting here (vmcnt 0) leaves no load
+    // pending on the loop-carried registers at the loop head, so the compiler
+    // puts no wait there (which would also stall on the last batch's stores)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // the piece's particles in batches of TG_THREADS * TG_U, the whole
+    // workgroup on every batch (lanes past the end masked by the views)
+    for (uint32_t j0 = 0; j0 < cnt; j0 += TG_THREADS * TG_U) {
+      // the next batch's loads go out first and land during this batch's
+      // arithmetic (and ahead of its stores); with the diagnostics' FP64 sums
+      // that does not fit in four waves' registers per SIMD, and they go out
+      // after the arithmetic, still ahead of the stores
+      float pn[TG_U][3], vn[TG_U][3], pmn[TG_U];
+      if (!DIAG) {
+        load_particles(j0 + TG_THREADS * TG_U, pn, vn, pmn);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      float o1[TG_U][3], o2[TG_U][3];  // DRIFT: pos, vel; otherwise acc, vel
 #pragma unroll
       for (int u = 0; u < TG_U; ++u) {
-        const uint32_t v = v0 + u * TG_THREADS;
-        if (v >= it.vhi) continue;
+        const bool live = j0 + tid + u * TG_THREADS < cnt;
         float w[3];
         int lb[3];
         const int t0[3] = {x0, y0, z0};
@@ -964,7 +1018,7 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
           const float f = floorf(uu);
           w[d] = __fsub_rn(uu, f);
           // base cell in region coordinates; the drift bound keeps it in range
-          // (the clamp only guards the LDS bounds)
+          // (the clamp only guards the LDS bounds, also for masked lanes)
           lb[d] = min(((int)f - t0[d] + DMAX) & (G - 1), tmax[d]);
         }
         float a[3] = {0.f, 0.f, 0.f};
@@ -983,40 +1037,59 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
               a[1] += fy[c] * wt;
               a[2] += fz[c] * wt;
             }
-        const int64_t q = v;
-        if (!drift) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) acc[3 * q + d] = a[d];
-        }
         float vs[3];  // closing half kick: the synchronised velocity of this step
 #pragma unroll
         for (int d = 0; d < 3; ++d) vs[d] = __fmaf_rn(hdt, a[d], vel0[u][d]);
-        if (dpart) {
-          const double m = mass ? (double)mass[q] : (double)mass_value;
+        if (DIAG) {
+          const double m = live ? (mass ? (double)pm[u] : (double)mass_value) : 0.0;
 #pragma unroll
           for (int d = 0; d < 3; ++d) dsum[d] += m * (double)vs[d];
           dsum[3] += 0.5 * m * ((double)vs[0] * vs[0] + (double)vs[1] * vs[1] + (double)vs[2] * vs[2]);
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          float vv = vs[d];
-          if (drift) {
-            vv = __fmaf_rn(hdt, a[d], vv);
+          if (DRIFT) {
+            const float vv = __fmaf_rn(hdt, a[d], vs[d]);
             float x = __fmaf_rn(dt, vv, p[u][d]);
             if (x >= L) x -= L;
             if (x < 0.0f) x += L;
             if (x >= L) x = 0.0f;  // -tiny + L rounds to L
-            pos[3 * q + d] = x;
-            vmax = fmaxf(vmax, fabsf(vv));
+            o1[u][d] = x;
+            o2[u][d] = vv;
+            vmax = fmaxf(vmax, live ? fabsf(vv) : 0.0f);
+          } else {
+            o1[u][d] = a[d];
+            o2[u][d] = vs[d];
           }
-          vel[3 * q + d] = vv;
+        }
+      }
+      if (DIAG) load_particles(j0 + TG_THREADS * TG_U, pn, vn, pmn);
+#pragma unroll
+      for (int u = 0; u < TG_U; ++u) {
+        const uint32_t off = 12u * (j0 + tid + u * TG_THREADS);
+        u32x3 a, b;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          a[d] = __float_as_uint(o1[u][d]);
+          b[d] = __float_as_uint(o2[u][d]);
+        }
+        __builtin_amdgcn_raw_buffer_store_b96(a, DRIFT ? rpos : racc, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b96(b, rvel, off, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < TG_U; ++u) {
+        pm[u] = pmn[u];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          p[u][d] = pn[u][d];
+          vel0[u][d] = vn[u][d];
         }
       }
     }
     __syncthreads();  // LDS is reused by the next item
   }
-  if (drift) block_max_to(vmax, &st->vmax_bits);
-  if (dpart) block_sum4_to(dsum, dpart + (int64_t)blockIdx.x * 4);
+  if (DRIFT) block_max_to(vmax, &st->vmax_bits);
+  if (DIAG) block_sum4_to(dsum, dpart + (int64_t)blockIdx.x * 4);
 }
 
 // sums the gather's per-workgroup FP64 partials in a fixed order: out[0..3]
@@ -1328,14 +1401,24 @@ static int pm_step_end(float* pos, float* vel, float* acc, const float* mass, fl
   {
     TimedScope ts_("gather_kdk", s);
     const dim3 gg((unsigned)ng);
-    if (pow2_float(h))
-      hipLaunchKernelGGL(k_gather_tiles<true>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
-                         (int)G, h, 1.0f / h, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
-                         (const DepItem*)b.ginfo, (int)npc, b.st, mass, mass_value, dpart);
-    else
-      hipLaunchKernelGGL(k_gather_tiles<false>, gg, dim3(TG_THREADS), 0, s, pos, vel, acc, (const float*)b.force, n,
-                         (int)G, h, 0.0f, hdt, dt, L, last ? 0 : 1, (const uint32_t*)b.gitems,
-                         (const DepItem*)b.ginfo, (int)npc, b.st, mass, mass_value, dpart);
+    const bool p2 = pow2_float(h);
+    const float inv_h = p2 ? 1.0f / h : 0.0f;
+#define AUGUR_GATHER(P2, DRIFT, DIAG)                                                                             \
+  hipLaunchKernelGGL((k_gather_tiles<P2, DRIFT, DIAG>), gg, dim3(TG_THREADS), 0, s, pos, vel, acc,                  \
+                     (const float*)b.force, n, (int)G, h, inv_h, hdt, dt, L, (const uint32_t*)b.gitems,            \
+                     (const DepItem*)b.ginfo, (int)npc, b.st, mass, mass_value, dpart)
+    const int variant = (p2 ? 4 : 0) | (last ? 0 : 2) | (dpart ? 1 : 0);
+    switch (variant) {
+      case 0: AUGUR_GATHER(false, false, false); break;
+      case 1: AUGUR_GATHER(false, false, true); break;
+      case 2: AUGUR_GATHER(false, true, false); break;
+      case 3: AUGUR_GATHER(false, true, true); break;
+      case 4: AUGUR_GATHER(true, false, false); break;
+      case 5: AUGUR_GATHER(true, false, true); break;
+      case 6: AUGUR_GATHER(true, true, false); break;
+      default: AUGUR_GATHER(true, true, true); break;
+    }
+#undef AUGUR_GATHER
     AUGUR_LAUNCHED("gather_kdk");
   }
   if (diag) {
