@@ -207,7 +207,10 @@ constexpr int TD_NW = TD_THREADS / 64;
 constexpr int TD_REGY = TD_Y + TD_LO + TD_HI, TD_REGZ = TD_Z + TD_LO + TD_HI;
 constexpr int TD_MAXSEG = 2 * TD_REGY * TD_REGZ + 2 * TD_REGZ + 3;
 constexpr uint32_t TD_CHUNK = 16384;  // most particle visits per work item
-constexpr int TD_K = 8;               // consecutive visits per lane in a wave batch
+#ifndef AUGUR_TD_K
+#define AUGUR_TD_K 8
+#endif
+constexpr int TD_K = AUGUR_TD_K;      // consecutive visits per lane in a wave batch
 constexpr int TD_BLK = 64;            // visits per entry of the visit -> segment table
 constexpr int TD_NBLK = TD_CHUNK / TD_BLK;  // table entries per item
 
