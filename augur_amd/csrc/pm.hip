@@ -876,6 +876,15 @@ constexpr uint32_t TG_CHUNK = 8192;  // most particles per work item
 constexpr int TG_U = 4;              // particles per thread in flight
 constexpr int TG_RX = TD_X + 1 + 2 * DMAX, TG_RY = TD_Y + 1 + 2 * DMAX, TG_RZ = TG_SUB + 1 + 2 * DMAX;
 constexpr int TG_REG = TG_RX * TG_RY * TG_RZ;
+// LDS image of the region: rows of TG_P floats holding columns x0 - 4 ..
+// x0 + TD_X + 3 (16-byte chunks of the force planes, loaded straight into LDS)
+constexpr int TG_XOFF = 4;                    // LDS column of x0
+constexpr int TG_P = TD_X + 2 * TG_XOFF;      // row pitch (floats)
+constexpr int TG_QR = TG_P / 4;               // 16-byte chunks per row
+constexpr int TG_NROW = TG_RY * TG_RZ;        // rows per plane
+constexpr int TG_NQ = TG_NROW * TG_QR;        // chunks per plane
+constexpr int TG_PLANE = TG_NROW * TG_P;      // floats per plane
+static_assert(DMAX < TG_XOFF && TD_X + 1 + DMAX <= TD_X + TG_XOFF, "region columns inside the LDS rows");
 
 // sort-time plan: items per piece (one thread each); items[npieces] = 0
 __global__ void k_gather_plan(const uint32_t* __restrict__ cell_start, int npieces, uint32_t* __restrict__ items,
@@ -924,8 +933,11 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
     int64_t n, int G, float h, float inv_h, float hdt, float dt, float L,
     const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, int npieces, PmDev* __restrict__ st,
     const float* __restrict__ mass, float mass_value, double* __restrict__ dpart) {
-  __shared__ float fx[TG_REG], fy[TG_REG], fz[TG_REG];
-  const int tid = threadIdx.x;
+  __shared__ __attribute__((aligned(16))) float fr[3 * TG_PLANE];
+  const float* fx = fr;
+  const float* fy = fr + TG_PLANE;
+  const float* fz = fr + 2 * TG_PLANE;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t G3 = (int64_t)G * G * G;
   float vmax = 0.0f;
   // FP64 sums of m v and m v^2 / 2 at the synchronised (full-step) velocity:
@@ -948,20 +960,21 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
     const __amdgpu_buffer_rsrc_t rvel = item_view(vel + 3 * (int64_t)it.vlo, 12u * cnt);
     const __amdgpu_buffer_rsrc_t racc = item_view(acc + 3 * (int64_t)it.vlo, 12u * cnt);
     const __amdgpu_buffer_rsrc_t rmass = item_view(mass ? mass + it.vlo : pos, mass ? 4u * cnt : 0u);
-    // force planes over the grown piece, origin (x0, y0, z0) - DMAX wrapped:
-    // every load of the region issued at once
-    constexpr int PER = (TG_REG + TG_THREADS - 1) / TG_THREADS;
-    float r[PER][3];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int c = tid + k * TG_THREADS;
-      if (c < TG_REG) {
-        const int xx = c % TG_RX, yy = (c / TG_RX) % TG_RY, zz = c / (TG_RX * TG_RY);
-        const int gx = (x0 - DMAX + xx) & (G - 1), gy = (y0 - DMAX + yy) & (G - 1), gz = (z0 - DMAX + zz) & (G - 1);
-        const int64_t cell = ((int64_t)gz * G + gy) * G + gx;
-        r[k][0] = force[cell];
-        r[k][1] = force[G3 + cell];
-        r[k][2] = force[2 * G3 + cell];
+    // force planes over the grown piece, origin (x0, y0, z0) - DMAX wrapped,
+    // loaded straight into LDS in 16-byte chunks (one wave instruction fills
+    // 64 consecutive chunks of the image; chunk (plane, row, j) holds columns
+    // x0 - 4 + 4j .. + 3): every load of the region issued at once
+    constexpr int NWQ = (3 * TG_NQ + 63) / 64;
+    for (int wq = wave; wq < NWQ; wq += TG_THREADS / 64) {
+      const int Q = wq * 64 + lane;
+      if (Q < 3 * TG_NQ) {
+        const int pl = Q / TG_NQ, q = Q - pl * TG_NQ;
+        const int row = q / TG_QR, j = q - row * TG_QR;
+        const int yy = row % TG_RY, zz = row / TG_RY;
+        const int gx = (x0 - TG_XOFF + 4 * j) & (G - 1), gy = (y0 - DMAX + yy) & (G - 1),
+                  gz = (z0 - DMAX + zz) & (G - 1);
+        const float* src = force + pl * G3 + ((int64_t)gz * G + gy) * G + gx;
+        __builtin_amdgcn_global_load_lds(src, fr + wq * 256, 16, 0, 0);
       }
     }
     // particle j of the item (byte offset 12 j) is lane tid + TG_THREADS u of batch j0
@@ -981,20 +994,12 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
       }
     };
     load_particles(0, p, vel0, pm);
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int c = tid + k * TG_THREADS;
-      if (c < TG_REG) {
-        fx[c] = r[k][0];
-        fy[c] = r[k][1];
-        fz[c] = r[k][2];
-      }
-    }
-    __syncthreads();
-    // the first batch has landed by now: waiting here (vmcnt 0) leaves no load
-    // pending on the loop-carried registers at the loop head, so the compiler
-    // puts no wait there (which would also stall on the last batch's stores)
+    // the region (LDS) and the first batch (registers) landed: waiting here
+    // (vmcnt 0) also leaves no load pending on the loop-carried registers at
+    // the loop head, so the compiler puts no wait there (which would stall on
+    // the last batch's stores)
     __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
     // the piece's particles in batches of TG_THREADS * TG_U, the whole
     // workgroup on every batch (lanes past the end masked by the views)
     for (uint32_t j0 = 0; j0 < cnt; j0 += TG_THREADS * TG_U) {
@@ -1035,7 +1040,7 @@ __global__ __launch_bounds__(TG_THREADS) void k_gather_tiles(
               const float wy = oy ? w[1] : __fsub_rn(1.0f, w[1]);
               const float wz = oz ? w[2] : __fsub_rn(1.0f, w[2]);
               const float wt = __fmul_rn(__fmul_rn(wx, wy), wz);
-              const int c = ((lb[2] + oz) * TG_RY + lb[1] + oy) * TG_RX + lb[0] + ox;
+              const int c = ((lb[2] + oz) * TG_RY + lb[1] + oy) * TG_P + lb[0] + (TG_XOFF - DMAX) + ox;
               a[0] += fx[c] * wt;
               a[1] += fy[c] * wt;
               a[2] += fz[c] * wt;
