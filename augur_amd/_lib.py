@@ -67,12 +67,18 @@ SIGNATURES = {
     "augur_pack_features_bf16": (c_int, [P, P, c_int64, c_int64, c_float, c_float, c_float, c_int, P, P]),
     "augur_adam_update": (c_int, [P, P, P, P, c_int64, c_float, c_float, c_float, c_float, c_int, c_float, P, P]),
     "augur_timing_enable": (c_int, [c_int]),
+    "augur_timing_filter": (c_int, [ctypes.c_char_p]),
     "augur_timing_collect": (c_int, [P, c_size_t, P, P, c_int, P]),
 }
 
 
 def timing_enable(on=True):
     call("augur_timing_enable", int(bool(on)))
+
+
+def timing_filter(name=None):
+    """Record only launches named `name` (None: all) while timing is enabled."""
+    call("augur_timing_filter", None if name is None else name.encode())
 
 
 def timing_collect():

@@ -22,6 +22,7 @@ bool enabled = false;
 std::vector<hipEvent_t> pool;
 std::vector<Rec> recs;
 size_t next_event = 0;
+std::string only;  // when set, only launches of this name are recorded
 }  // namespace
 
 bool timing_on() { return enabled; }
@@ -30,6 +31,7 @@ int timing_start(const char* name, hipStream_t s, int* slot) {
   *slot = -1;
   if (!enabled) return AUGUR_OK;
   std::lock_guard<std::mutex> g(mu);
+  if (!only.empty() && only != name) return AUGUR_OK;
   if (next_event + 2 > pool.size()) return AUGUR_OK;  // pool exhausted: stop recording
   Rec r{name, pool[next_event], pool[next_event + 1]};
   next_event += 2;
@@ -61,6 +63,12 @@ int augur_timing_enable(int on) {
   enabled = on != 0;
   recs.clear();
   next_event = 0;
+  return AUGUR_OK;
+}
+
+int augur_timing_filter(const char* name) {
+  std::lock_guard<std::mutex> g(mu);
+  only = name ? name : "";
   return AUGUR_OK;
 }
 

@@ -324,6 +324,10 @@ int augur_adam_update(float* p, const float* g, float* m, float* v, int64_t n, f
  * done while disabled.  Replaces the CUDA-event component timers of
  * scripts/scatter_field_2d_cost_blowup_bench.py:64-80. */
 int augur_timing_enable(int on);
+/* Restricts recording to launches of one name (NULL or "": all), so a timed
+ * region carries the events of a single kernel only.  No counterpart in the
+ * reference; for timing one kernel inside an otherwise undisturbed run. */
+int augur_timing_filter(const char* name);
 int augur_timing_collect(char* names, size_t names_cap, double* ms, int64_t* counts, int max_entries, int* n_entries);
 
 #ifdef __cplusplus
