@@ -134,9 +134,7 @@ size_t scan_workspace_bytes(int64_t n) {
 // thread scans a contiguous chunk, chunk totals scanned across the block,
 // coalesced store (in place allowed)
 constexpr int64_t SC_SMALL = 16384;  // one launch of k_scan_small up to here
-__global__ __launch_bounds__(SC_THREADS) void k_scan_small(const uint32_t* in, uint32_t* out, int64_t n,
-                                                           const uint32_t* gate) {
-  if (gate && *gate == 0u) return;
+__device__ __forceinline__ void scan_small_body(const uint32_t* in, uint32_t* out, int64_t n) {
   __shared__ uint32_t buf[SC_SMALL];
   for (int64_t i = threadIdx.x; i < n; i += SC_THREADS) buf[i] = in[i];
   __syncthreads();
@@ -153,6 +151,33 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_small(const uint32_t* in, u
   }
   __syncthreads();
   for (int64_t i = threadIdx.x; i < n; i += SC_THREADS) out[i] = buf[i];
+}
+__global__ __launch_bounds__(SC_THREADS) void k_scan_small(const uint32_t* in, uint32_t* out, int64_t n,
+                                                           const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  scan_small_body(in, out, n);
+}
+// two short arrays in one launch, one workgroup each
+__global__ __launch_bounds__(SC_THREADS) void k_scan_small2(const uint32_t* in0, uint32_t* out0, int64_t n0,
+                                                            const uint32_t* in1, uint32_t* out1, int64_t n1,
+                                                            const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  if (blockIdx.x == 0)
+    scan_small_body(in0, out0, n0);
+  else
+    scan_small_body(in1, out1, n1);
+}
+
+// two exclusive scans; one launch when both arrays are short
+int exclusive_scan2_u32(const uint32_t* in0, uint32_t* out0, int64_t n0, const uint32_t* in1, uint32_t* out1,
+                        int64_t n1, void* ws, hipStream_t s, const uint32_t* gate) {
+  if (n0 > 0 && n1 > 0 && n0 <= SC_SMALL && n1 <= SC_SMALL) {
+    hipLaunchKernelGGL(k_scan_small2, dim3(2), dim3(SC_THREADS), 0, s, in0, out0, n0, in1, out1, n1, gate);
+    AUGUR_LAUNCHED("scan_small2");
+    return AUGUR_OK;
+  }
+  int rc = exclusive_scan_u32(in0, out0, n0, ws, s, gate);
+  return rc ? rc : exclusive_scan_u32(in1, out1, n1, ws, s, gate);
 }
 
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s, const uint32_t* gate) {

@@ -9,6 +9,9 @@ int fill_u32(void* p, int64_t n, uint32_t v, hipStream_t s);
 size_t scan_workspace_bytes(int64_t n);
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s,
                        const uint32_t* gate = nullptr);
+// two exclusive scans (one launch when both have at most 16384 elements)
+int exclusive_scan2_u32(const uint32_t* in0, uint32_t* out0, int64_t n0, const uint32_t* in1, uint32_t* out1,
+                        int64_t n1, void* ws, hipStream_t s, const uint32_t* gate);
 size_t radix_workspace_bytes(int64_t n);
 int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* ws, size_t ws_bytes,
                      uint32_t** keys_out, uint32_t** vals_out, hipStream_t s, const uint32_t* gate = nullptr);

@@ -248,11 +248,10 @@ __device__ __forceinline__ int box_runs(const Piece& px, const Piece& py, const 
 // slot starts and an inclusive prefix of their lengths (TD_MAXSEG each; the
 // unused tail has length 0), and the tile's number of work items of at most
 // TD_CHUNK visits; items[ntiles] = 0 so an exclusive scan leaves the total there.
-__global__ __launch_bounds__(256) void k_dep_plan(const uint32_t* __restrict__ cell_start, int G, int ntiles,
-                                                  uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
-                                                  uint32_t* __restrict__ seg_pre, const uint32_t* gate) {
-  if (gate && *gate == 0u) return;
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__device__ __forceinline__ void dep_plan_body(int bid, const uint32_t* __restrict__ cell_start, int G, int ntiles,
+                                              uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
+                                              uint32_t* __restrict__ seg_pre) {
+  const int tile = bid * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (tile > ntiles) return;
   if (tile == ntiles) {
     if (lane == 0) items[ntiles] = 0u;
@@ -321,12 +320,10 @@ struct DepItem {
 };
 
 template <int MAXSEG, uint32_t CHUNK>
-__global__ __launch_bounds__(256) void k_tile_items(const uint32_t* __restrict__ item_start, int ntiles,
-                                                    const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
-                                                    uint16_t* __restrict__ blk, const uint32_t* gate) {
+__device__ __forceinline__ void tile_items_body(uint32_t item, const uint32_t* __restrict__ item_start, int ntiles,
+                                                const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
+                                                uint16_t* __restrict__ blk) {
   constexpr int NBLK = CHUNK / TD_BLK;
-  if (gate && *gate == 0u) return;
-  const uint32_t item = blockIdx.x;
   if (item >= item_start[ntiles]) return;
   int a = 0, b = ntiles;  // last tile with item_start[tile] <= item
   while (b - a > 1) {
@@ -887,10 +884,9 @@ constexpr int TG_PLANE = TG_NROW * TG_P;      // floats per plane
 static_assert(DMAX < TG_XOFF && TD_X + 1 + DMAX <= TD_X + TG_XOFF, "region columns inside the LDS rows");
 
 // sort-time plan: items per piece (one thread each); items[npieces] = 0
-__global__ void k_gather_plan(const uint32_t* __restrict__ cell_start, int npieces, uint32_t* __restrict__ items,
-                              const uint32_t* gate) {
-  if (gate && *gate == 0u) return;
-  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gather_plan_body(int bid, const uint32_t* __restrict__ cell_start, int npieces,
+                                                 uint32_t* __restrict__ items) {
+  const int pc = bid * (int)blockDim.x + threadIdx.x;
   if (pc > npieces) return;
   if (pc == npieces) {
     items[npieces] = 0u;
@@ -902,10 +898,10 @@ __global__ void k_gather_plan(const uint32_t* __restrict__ cell_start, int npiec
 }
 
 // item -> (piece, slot range): the piece's particles cut into near-equal parts
-__global__ void k_gather_items(const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ item_start,
-                               int npieces, DepItem* __restrict__ info, const uint32_t* gate) {
-  if (gate && *gate == 0u) return;
-  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gather_items_body(int bid, const uint32_t* __restrict__ cell_start,
+                                                  const uint32_t* __restrict__ item_start, int npieces,
+                                                  DepItem* __restrict__ info) {
+  const int pc = bid * (int)blockDim.x + threadIdx.x;
   if (pc >= npieces) return;
   constexpr uint32_t PC = TG_SUB * TD_Y * TD_X;
   const uint32_t lo = cell_start[(uint32_t)pc * PC], hi = cell_start[(uint32_t)(pc + 1) * PC];
@@ -913,6 +909,36 @@ __global__ void k_gather_items(const uint32_t* __restrict__ cell_start, const ui
   const uint32_t per = (hi - lo + nch - 1) / nch;
   for (uint32_t c = 0; c < nch; ++c)
     info[i0 + c] = DepItem{(uint32_t)pc, min(hi, lo + c * per), min(hi, lo + (c + 1) * per), 0u};
+}
+
+// The deposit's and the gather's sort-time plans share launches (256
+// threads): blocks [0, dep_blocks) plan the tiles (a wave per tile), the
+// rest the gather pieces (a thread per piece); then, after the scans of both
+// item counts, blocks [0, item_blocks) cut the tiles' work items and the
+// rest the pieces'.
+__global__ __launch_bounds__(256) void k_plans(const uint32_t* __restrict__ cell_start, int G, int ntiles,
+                                               uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
+                                               uint32_t* __restrict__ seg_pre, int dep_blocks, int npieces,
+                                               uint32_t* __restrict__ gitems, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  if ((int)blockIdx.x < dep_blocks)
+    dep_plan_body((int)blockIdx.x, cell_start, G, ntiles, items, seg_lo, seg_pre);
+  else
+    gather_plan_body((int)blockIdx.x - dep_blocks, cell_start, npieces, gitems);
+}
+
+template <int MAXSEG, uint32_t CHUNK>
+__global__ __launch_bounds__(256) void k_plan_items(const uint32_t* __restrict__ item_start, int ntiles,
+                                                    const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
+                                                    uint16_t* __restrict__ blk, int item_blocks,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    const uint32_t* __restrict__ gitem_start, int npieces,
+                                                    DepItem* __restrict__ ginfo, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  if ((int)blockIdx.x < item_blocks)
+    tile_items_body<MAXSEG, CHUNK>(blockIdx.x, item_start, ntiles, tseg_pre, info, blk);
+  else
+    gather_items_body((int)blockIdx.x - item_blocks, cell_start, gitem_start, npieces, ginfo);
 }
 
 // Raw buffer views of an item's particle arrays: a lane past the item's end
@@ -1258,24 +1284,20 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   // work items: the deposit's over the tile regions grown for the largest
   // drift bound, the gather's over the tile cores
   const int64_t ntiles = pm_tiles(G);
-  const dim3 pg((unsigned)((ntiles + 1 + 3) / 4));
-  hipLaunchKernelGGL(k_dep_plan, pg, dim3(256), 0, s, (const uint32_t*)b.cell_start, (int)G, (int)ntiles, b.items,
-                     b.tseg_lo, b.tseg_pre, gate);
-  AUGUR_LAUNCHED("dep_plan");
-  rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, gate);
-  if (rc) return rc;
-  hipLaunchKernelGGL((k_tile_items<TD_MAXSEG, TD_CHUNK>), dim3((unsigned)pm_max_items(n, G)), dim3(256), 0, s,
-                     (const uint32_t*)b.items, (int)ntiles, (const uint32_t*)b.tseg_pre, b.info, b.iblk, gate);
-  AUGUR_LAUNCHED("dep_items");
   const int64_t npc = pm_pieces(G);
-  hipLaunchKernelGGL(k_gather_plan, dim3(blocks_for(npc + 1, 256)), dim3(256), 0, s, (const uint32_t*)b.cell_start,
+  const int dep_blocks = (int)((ntiles + 1 + 3) / 4);
+  hipLaunchKernelGGL(k_plans, dim3((unsigned)(dep_blocks + blocks_for(npc + 1, 256))), dim3(256), 0, s,
+                     (const uint32_t*)b.cell_start, (int)G, (int)ntiles, b.items, b.tseg_lo, b.tseg_pre, dep_blocks,
                      (int)npc, b.gitems, gate);
-  AUGUR_LAUNCHED("gather_plan");
-  rc = exclusive_scan_u32(b.gitems, b.gitems, npc + 1, b.scan_ws, s, gate);
+  AUGUR_LAUNCHED("plans");
+  rc = exclusive_scan2_u32(b.items, b.items, ntiles + 1, b.gitems, b.gitems, npc + 1, b.scan_ws, s, gate);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_gather_items, dim3(blocks_for(npc, 256)), dim3(256), 0, s, (const uint32_t*)b.cell_start,
-                     (const uint32_t*)b.gitems, (int)npc, b.ginfo, gate);
-  AUGUR_LAUNCHED("gather_items");
+  const int item_blocks = (int)pm_max_items(n, G);
+  hipLaunchKernelGGL((k_plan_items<TD_MAXSEG, TD_CHUNK>), dim3((unsigned)(item_blocks + blocks_for(npc, 256))),
+                     dim3(256), 0, s, (const uint32_t*)b.items, (int)ntiles, (const uint32_t*)b.tseg_pre, b.info,
+                     b.iblk, item_blocks, (const uint32_t*)b.cell_start, (const uint32_t*)b.gitems, (int)npc,
+                     b.ginfo, gate);
+  AUGUR_LAUNCHED("plan_items");
   if (!physical) {
     hipLaunchKernelGGL(k_copy<uint32_t>, g, bl, 0, s, perm, b.perm, n, gate);
     AUGUR_LAUNCHED("copy_perm");
