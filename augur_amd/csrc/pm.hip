@@ -572,12 +572,16 @@ struct PmCol {
 
 // z columns at fixed ky: forward z FFT, Green's function g = -4 pi G / k^2 / G^3,
 // outputs A = IFFTz(g rho) (in place of the input) and B = IFFTz(-i kz g rho)
-// (to outB; Nyquist kz zeroed).
+// (to outB; Nyquist kz zeroed).  One LDS buffer: the Green-scaled spectrum P
+// is formed in place, the B transform runs in place too, and each thread
+// keeps the P values its first B pass reads, to put P back for the A
+// transform (four workgroups per CU instead of two).
 template <int N>
 __global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ spec, float2* __restrict__ outB,
                                                          float kf, float green) {
   using C = PmCol<N>;
-  __shared__ float2 A[C::NL * C::LS], B[C::NL * C::LS];
+  constexpr int R0 = FftPlan<N>::R0, Q = N / R0;
+  __shared__ float2 X[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
   __syncthreads();
@@ -588,37 +592,44 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ s
   auto gsrc = [&](int c, int k) -> float2 {
     return (c0 + c < C::nxc) ? spec[base + k * es + c] : make_float2(0.f, 0.f);
   };
-  // forward transform straight into B, scaled by the Green's function
+  // forward transform, scaled by the Green's function, into X (in place)
   auto green_dst = [&](int c, int kz, float2 v) {
     const int kx = c0 + c;
     const int nz = kz <= N / 2 ? kz : kz - N;
     const float fx = kf * kx, fy = kf * ny, fz = kf * nz;
     const float k2 = fx * fx + fy * fy + fz * fz;
     const float g = (kx == 0 && ky == 0 && kz == 0) ? 0.0f : green / k2;
-    B[c * C::LS + fft_pad(kz)] = make_float2(v.x * g, v.y * g);
+    X[c * C::LS + fft_pad(kz)] = make_float2(v.x * g, v.y * g);
   };
-  fft_lines<N, false, true>(A, tw, C::NL, gsrc, green_dst);
-  // A = p, B = -i kz p
-  for (int e = threadIdx.x; e < C::NL * N; e += C::T) {
-    const int c = e % C::NL, kz = e / C::NL;
-    const int idx = c * C::LS + fft_pad(kz);
-    const float2 p = B[idx];
+  fft_lines<N, false, true, false, true>(X, tw, C::NL, gsrc, green_dst);
+  // B = -i kz P, transformed in place
+  float2 keep[R0];
+  int kc = 0;
+  auto bsrc = [&](int c, int kz) -> float2 {
+    const float2 p = X[c * C::LS + fft_pad(kz)];
+    keep[kc++] = p;
     const int nz = kz <= N / 2 ? kz : kz - N;
     const float az = (kz == N / 2) ? 0.0f : kf * nz;
-    A[idx] = p;
-    B[idx] = make_float2(az * p.y, -az * p.x);
-  }
-  __syncthreads();
-  auto asrc = [&](int c, int k) { return A[c * C::LS + fft_pad(k)]; };
-  auto bsrc = [&](int c, int k) { return B[c * C::LS + fft_pad(k)]; };
-  auto adst = [&](int c, int k, float2 v) {
-    if (c0 + c < C::nxc) spec[base + k * es + c] = v;
+    return make_float2(az * p.y, -az * p.x);
   };
   auto bdst = [&](int c, int k, float2 v) {
     if (c0 + c < C::nxc) outB[base + k * es + c] = v;
   };
-  fft_lines<N, true, true, true>(A, tw, C::NL, asrc, adst);
-  fft_lines<N, true, true, true>(B, tw, C::NL, bsrc, bdst);
+  fft_lines<N, true, true, true, false>(X, tw, C::NL, bsrc, bdst);
+  // P back into X: the first pass of a line-fastest transform gives thread t
+  // line t % NL and points t / NL + r Q (r = 0 .. R0 - 1)
+  const int t = threadIdx.x;
+  if (t < C::NL * Q) {
+    const int line = t % C::NL, j = t / C::NL;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) X[line * C::LS + fft_pad(j + r * Q)] = keep[r];
+  }
+  __syncthreads();
+  auto asrc = [&](int c, int k) { return X[c * C::LS + fft_pad(k)]; };
+  auto adst = [&](int c, int k, float2 v) {
+    if (c0 + c < C::nxc) spec[base + k * es + c] = v;
+  };
+  fft_lines<N, true, true, true, false>(X, tw, C::NL, asrc, adst);
 }
 
 // y columns at fixed z: reads A (s0) and B (s2); writes
