@@ -653,13 +653,21 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0
   auto st = [&](float2* p, int c, int k, float2 v) {
     if (c0 + c < C::nxc) p[base + k * es + c] = v;
   };
+  // s0 is read once: the first pass of the -i ky transform keeps the values
+  // it reads, and the first pass of the plain transform (same thread, same
+  // points, same order) takes them back
+  constexpr int R0 = FftPlan<N>::R0;
+  float2 keep[R0];
+  int kc = 0;
   auto src_ky = [&](int c, int ky) -> float2 {
     const float2 v = ld(s0, c, ky);
+    keep[kc++] = v;
     const float f = (ky == N / 2) ? 0.0f : kf * (float)(ky <= N / 2 ? ky : ky - N);
     return make_float2(f * v.y, -f * v.x);
   };
   fft_lines<N, true, true>(X, tw, C::NL, src_ky, [&](int c, int k, float2 v) { st(s1, c, k, v); });
-  fft_lines<N, true, true>(X, tw, C::NL, [&](int c, int k) { return ld(s0, c, k); },
+  int kr = 0;
+  fft_lines<N, true, true>(X, tw, C::NL, [&](int, int) { return keep[kr++]; },
                            [&](int c, int k, float2 v) { st(s0, c, k, v); });
   fft_lines<N, true, true>(X, tw, C::NL, [&](int c, int k) { return ld(s2, c, k); },
                            [&](int c, int k, float2 v) { st(s2, c, k, v); });
