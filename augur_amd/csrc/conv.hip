@@ -381,6 +381,154 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   }
 }
 
+// --------------------------------------- 32 -> 32 GELU layers, split outputs
+// The same z-marching column with eight waves: waves 0-3 compute output
+// channels 0-15 and waves 4-7 channels 16-31 of the same 16 rows, each with
+// v_mfma_f32_16x16x32_bf16 (K = all 32 input channels per instruction) and
+// only its half of the weights in registers (27 taps x 4 VGPRs).  Two waves
+// per SIMD: one wave's epilogue (bias, GELU, residual, BF16 stores) runs
+// while the other's MFMAs do.  Input fragments are read by both halves.
+constexpr int ZS_THREADS = 512;
+
+template <int MODE>
+__global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* __restrict__ x,
+                                                               const uint16_t* __restrict__ w,
+                                                               const float* __restrict__ bias, int G,
+                                                               uint16_t* __restrict__ z_out,
+                                                               uint16_t* __restrict__ h_out) {
+  constexpr int CIN = 32, CP = CIN + 8;
+  constexpr int PLANE = ZM_PY * ZM_PX * CP;
+  constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
+  constexpr int PER = (CHUNKS + ZS_THREADS - 1) / ZS_THREADS;
+  __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
+  __shared__ __attribute__((aligned(16))) float sbias[32];
+  const int tid = threadIdx.x;
+  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / ZM_ZC;
+  int t = blockIdx.x;
+  const int bx = t % ntx;
+  t /= ntx;
+  const int by = t % nty;
+  t /= nty;
+  const int bz = t % ntz;
+  const int b = t / ntz;
+  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * ZM_ZC;
+  const int64_t G3 = (int64_t)G * G * G;
+  const uint16_t* xb = x + (int64_t)b * G3 * CIN;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int half = wv >> 2, wrow = wv & 3;  // output channels 16 half .. + 15, rows 4 wrow .. + 3
+  const int n16 = lane & 15, kq = lane >> 4;
+  if (tid < 32) sbias[tid] = bias[tid];
+
+  // A operand: w[tap][co = 16 half + n16][ci = 8 kq .. 8 kq + 7]
+  bf16x8 wr[27];
+#pragma unroll
+  for (int tap = 0; tap < 27; ++tap)
+    wr[tap] = *reinterpret_cast<const bf16x8*>(&w[((int64_t)tap * 32 + 16 * half + n16) * CIN + kq * 8]);
+
+  auto load_plane = [&](int zz, u32x4 (&r)[PER]) {
+    const int gz = zz & (G - 1);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * ZS_THREADS;
+      if (c < CHUNKS) {
+        const int ch = c % (CIN / 8), v = c / (CIN / 8);
+        const int tx = v % ZM_PX, ty = v / ZM_PX;
+        const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1);
+        r[k] = *reinterpret_cast<const u32x4*>(&xb[(((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
+      }
+    }
+  };
+  auto store_plane = [&](int slot, const u32x4 (&r)[PER]) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = tid + k * ZS_THREADS;
+      if (c < CHUNKS) {
+        const int ch = c % (CIN / 8), v = c / (CIN / 8);
+        *reinterpret_cast<u32x4*>(&ring[slot * PLANE + v * CP + ch * 8]) = r[k];
+      }
+    }
+  };
+  {
+    u32x4 r[PER];
+#pragma unroll 1
+    for (int s = 0; s < 3; ++s) {
+      load_plane(zs - 1 + s, r);
+      store_plane(s, r);
+    }
+  }
+  __syncthreads();
+
+#pragma unroll 1
+  for (int zi = 0; zi < ZM_ZC; ++zi) {
+    const int z = zs + zi;
+    u32x4 pre[PER];
+    if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);
+    f32x4 acc[ZM_RPW][2];
+#pragma unroll
+    for (int r = 0; r < ZM_RPW; ++r)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) acc[r][hf] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz) {
+      const uint16_t* P = ring + ((zi + dz) % 3) * PLANE;
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int yr = 0; yr < ZM_RPW + 2; ++yr)
+#pragma unroll
+          for (int hf = 0; hf < 2; ++hf) {
+            // B operand: voxel 16 hf + n16 + dx - 1 of the row, ci = 8 kq .. 8 kq + 7
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(
+                &P[((wrow * ZM_RPW + yr) * ZM_PX + hf * 16 + n16 + dx) * CP + kq * 8]);
+#pragma unroll
+            for (int r = 0; r < ZM_RPW; ++r) {
+              const int dy = yr - r;
+              if (dy < 0 || dy > 2) continue;
+              acc[r][hf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[(dz * 3 + dy) * 3 + dx], av, acc[r][hf], 0, 0, 0);
+            }
+          }
+    }
+    // acc[r][hf][q] holds output channel 16 half + 4 kq + q of voxel (x0 + 16 hf + n16, y0 + row, z)
+    const uint16_t* C = ring + ((zi + 1) % 3) * PLANE;  // input plane z (residual centre)
+    const int co0 = 16 * half + 4 * kq;
+    const float4 bj = *reinterpret_cast<const float4*>(&sbias[co0]);
+#pragma unroll
+    for (int r = 0; r < ZM_RPW; ++r) {
+      const int ly = wrow * ZM_RPW + r;
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const int lx = hf * 16 + n16;
+        const int64_t o = ((int64_t)b * G3 + ((int64_t)z * G + y0 + ly) * G + x0 + lx) * 32 + co0;
+        const float v0 = acc[r][hf][0] + bj.x, v1 = acc[r][hf][1] + bj.y, v2 = acc[r][hf][2] + bj.z,
+                    v3 = acc[r][hf][3] + bj.w;
+        if (z_out) {
+          uint2 zp;
+          zp.x = pack_bf16x2(v0, v1);
+          zp.y = pack_bf16x2(v2, v3);
+          *reinterpret_cast<uint2*>(&z_out[o]) = zp;
+        }
+        float h0 = gelu(v0), h1 = gelu(v1), h2 = gelu(v2), h3 = gelu(v3);
+        if (MODE == CONV_RES_GELU) {
+          const uint2 xr = *reinterpret_cast<const uint2*>(&C[((ly + 1) * ZM_PX + 1 + lx) * CP + co0]);
+          h0 += __uint_as_float(xr.x << 16);
+          h1 += __uint_as_float(xr.x & 0xffff0000u);
+          h2 += __uint_as_float(xr.y << 16);
+          h3 += __uint_as_float(xr.y & 0xffff0000u);
+        }
+        uint2 hp;
+        hp.x = pack_bf16x2(h0, h1);
+        hp.y = pack_bf16x2(h2, h3);
+        *reinterpret_cast<uint2*>(&h_out[o]) = hp;
+      }
+    }
+    __syncthreads();
+    if (zi + 1 < ZM_ZC) {
+      store_plane(zi % 3, pre);
+      __syncthreads();
+    }
+  }
+}
+
 // ------------------------------------------------------------ weight grad
 // dW[tap][co][ci] = sum_v dz[v][co] * x[v + o_tap][ci]; db[co] = sum_v dz[v][co].
 // Persistent workgroups walk tiles in a fixed order and write one partial per
@@ -666,12 +814,21 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 int conv_tiles_ok(int64_t G) { return is_pow2(G) && G >= 32 && G <= 1024; }
 
+#ifndef AUGUR_CONV_SPLIT
+#define AUGUR_CONV_SPLIT 1
+#endif
 template <int CIN, int MODE>
 int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t B, int64_t G, int cout, uint16_t* z,
                uint16_t* h, float* y, const float* addend, hipStream_t s, const char* name) {
   const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / ZM_ZC);
   AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
   TimedScope ts_(name, s);
+  if constexpr (AUGUR_CONV_SPLIT && CIN == 32 && (MODE == CONV_GELU || MODE == CONV_RES_GELU)) {
+    hipLaunchKernelGGL((k_conv3_zm_split<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias, (int)G,
+                       z, h);
+    AUGUR_LAUNCHED(name);
+    return AUGUR_OK;
+  }
   hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
                      cout, z, h, y, addend);
   AUGUR_LAUNCHED(name);
