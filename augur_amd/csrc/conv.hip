@@ -387,7 +387,8 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
 // v_mfma_f32_16x16x32_bf16 (K = all 32 input channels per instruction) and
 // only its half of the weights in registers (27 taps x 4 VGPRs).  Two waves
 // per SIMD: one wave's epilogue (bias, GELU, residual, BF16 stores) runs
-// while the other's MFMAs do.  Input fragments are read by both halves.
+// while the other's MFMAs do.  Input fragments are read by both halves.  Also
+// the FP32 channels-last 32 -> 32 layer of the backward pass.
 constexpr int ZS_THREADS = 512;
 
 template <int MODE>
@@ -395,7 +396,9 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
                                                                const uint16_t* __restrict__ w,
                                                                const float* __restrict__ bias, int G,
                                                                uint16_t* __restrict__ z_out,
-                                                               uint16_t* __restrict__ h_out) {
+                                                               uint16_t* __restrict__ h_out,
+                                                               float* __restrict__ y_out,
+                                                               const float* __restrict__ addend) {
   constexpr int CIN = 32, CP = CIN + 8;
   constexpr int PLANE = ZM_PY * ZM_PX * CP;
   constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
   const int lane = tid & 63, wv = tid >> 6;
   const int half = wv >> 2, wrow = wv & 3;  // output channels 16 half .. + 15, rows 4 wrow .. + 3
   const int n16 = lane & 15, kq = lane >> 4;
-  if (tid < 32) sbias[tid] = bias[tid];
+  if (tid < 32) sbias[tid] = MODE == CONV_OUT_F32_CL ? 0.0f : bias[tid];
 
   // A operand: w[tap][co = 16 half + n16][ci = 8 kq .. 8 kq + 7]
   bf16x8 wr[27];
@@ -501,6 +504,18 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
         const int64_t o = ((int64_t)b * G3 + ((int64_t)z * G + y0 + ly) * G + x0 + lx) * 32 + co0;
         const float v0 = acc[r][hf][0] + bj.x, v1 = acc[r][hf][1] + bj.y, v2 = acc[r][hf][2] + bj.z,
                     v3 = acc[r][hf][3] + bj.w;
+        if (MODE == CONV_OUT_F32_CL) {  // FP32 channels-last, 32 channels (+ addend)
+          float4 val = {v0, v1, v2, v3};
+          if (addend) {
+            const float4 ad = *reinterpret_cast<const float4*>(&addend[o]);
+            val.x += ad.x;
+            val.y += ad.y;
+            val.z += ad.z;
+            val.w += ad.w;
+          }
+          *reinterpret_cast<float4*>(&y_out[o]) = val;
+          continue;
+        }
         if (z_out) {
           uint2 zp;
           zp.x = pack_bf16x2(v0, v1);
@@ -825,9 +840,17 @@ int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t 
   TimedScope ts_(name, s);
   if constexpr (AUGUR_CONV_SPLIT && CIN == 32 && (MODE == CONV_GELU || MODE == CONV_RES_GELU)) {
     hipLaunchKernelGGL((k_conv3_zm_split<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias, (int)G,
-                       z, h);
+                       z, h, y, addend);
     AUGUR_LAUNCHED(name);
     return AUGUR_OK;
+  }
+  if constexpr (AUGUR_CONV_SPLIT && CIN == 32 && MODE == CONV_OUT_F32_CL) {
+    if (cout == 32) {
+      hipLaunchKernelGGL((k_conv3_zm_split<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias,
+                         (int)G, z, h, y, addend);
+      AUGUR_LAUNCHED(name);
+      return AUGUR_OK;
+    }
   }
   hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
                      cout, z, h, y, addend);

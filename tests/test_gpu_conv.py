@@ -61,6 +61,13 @@ def test_conv_forward_modes(dev, cin):
         y5 = S.conv3_fwd(xc, wk, None, S.MODE_OUT_CL, 5, addend=add.to(dev)).cpu().double()
         exp5 = ref_nb[..., :5] + add.double()
         assert (y5 - exp5).abs().max() < 1e-4 * exp5.abs().max()
+        # all 32 channels channels-last (the backward data-gradient layout), with and without addend
+        add32 = torch.randn(B, G, G, G, 32, generator=g)
+        y32 = S.conv3_fwd(xc, wk, None, S.MODE_OUT_CL, 32, addend=add32.to(dev)).cpu().double()
+        exp32 = ref_nb + add32.double()
+        assert (y32 - exp32).abs().max() < 1e-4 * exp32.abs().max()
+        y32n = S.conv3_fwd(xc, wk, None, S.MODE_OUT_CL, 32).cpu().double()
+        assert (y32n - ref_nb).abs().max() < 1e-4 * ref_nb.abs().max()
 
 
 def test_conv_data_gradient_is_transposed_conv(dev):
