@@ -886,7 +886,10 @@ __global__ __launch_bounds__(256) void k_gather_kdk(float* __restrict__ pos, flo
 // up to DMAX cells) in LDS, then streams the piece's particles through the
 // same arithmetic as k_gather_kdk.  Two workgroups per CU.
 constexpr int TG_THREADS = 512;
-constexpr int TG_SUB = 4;  // z layers per piece
+#ifndef AUGUR_TG_SUB
+#define AUGUR_TG_SUB 4
+#endif
+constexpr int TG_SUB = AUGUR_TG_SUB;  // z layers per piece
 constexpr int TG_NSUB = TD_Z / TG_SUB;
 constexpr uint32_t TG_CHUNK = 8192;  // most particles per work item
 constexpr int TG_U = 4;              // particles per thread in flight
