@@ -410,11 +410,8 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     __syncthreads();
     {
       // wave batches of 64 * TD_K consecutive visits, TD_K per lane: loads are
-      // coalesced, a lane sums a cell's particles in registers and touches LDS
-      // when its base cell changes, and rows outside the tile skip as a whole
-      // wave batches of 64 * TD_K consecutive visits, TD_K per lane: loads are
-      // coalesced, a lane sums a cell's particles in registers and touches LDS
-      // when its base cell changes, and rows outside the tile skip as a whole
+      // coalesced, and a lane sums a cell's particles in registers and
+      // touches LDS when its base cell changes
       for (uint32_t bb = plo + (uint32_t)wave * 64 * TD_K; bb < phi; bb += TD_NW * 64 * TD_K) {
         const uint32_t l0 = bb + (uint32_t)lane * TD_K;
         if (l0 >= phi) continue;
