@@ -70,6 +70,7 @@ enum ConvMode : int {
 // plane that is no longer needed.
 constexpr int ZM_X = 32, ZM_Y = 16, ZM_ZC = 16, ZM_THREADS = 256, ZM_RPW = 4;
 constexpr int ZM_PX = ZM_X + 2, ZM_PY = ZM_Y + 2;
+constexpr int ZS_THREADS = 512;  // the eight-wave kernels (split outputs, narrow outputs)
 
 template <int CIN, int MODE>
 __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restrict__ x,
@@ -250,12 +251,13 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
 // CIN = 32 (the network's output convolution): v_mfma_f32_16x16x32_bf16 with
 // A = weights (16 output channels x 32 input channels of one tap) and B = 16
 // voxels of an input row, so no matrix work is spent on padded channels.
-// Each wave keeps the 27 weight fragments in registers and owns 4 output
-// rows, each as two 16-voxel halves.
+// Eight waves: each keeps the 27 weight fragments in registers and owns one
+// 16-voxel half of 4 output rows (waves 0-3 the first halves, 4-7 the second),
+// so two waves share a SIMD.
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
 template <int MODE>
-__global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* __restrict__ x,
+__global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* __restrict__ x,
                                                                 const uint16_t* __restrict__ w,
                                                                 const float* __restrict__ bias, int G, int cout,
                                                                 float* __restrict__ y_out,
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   constexpr int CIN = 32, CP = CIN + 8;
   constexpr int PLANE = ZM_PY * ZM_PX * CP;
   constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
-  constexpr int PER = (CHUNKS + ZM_THREADS - 1) / ZM_THREADS;
+  constexpr int PER = (CHUNKS + ZS_THREADS - 1) / ZS_THREADS;
   __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
   __shared__ __attribute__((aligned(16))) float sbias[32];
   const int tid = threadIdx.x;
@@ -279,6 +281,7 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   const int64_t G3 = (int64_t)G * G * G;
   const uint16_t* xb = x + (int64_t)b * G3 * CIN;
   const int lane = tid & 63, wv = tid >> 6;
+  const int hf = wv >> 2, wrow = wv & 3;  // voxels 16 hf .. + 15 of rows 4 wrow .. + 3
   const int n16 = lane & 15, kq = lane >> 4;
   if (tid < 32) sbias[tid] = (MODE == CONV_OUT_F32_CL || tid >= cout) ? 0.0f : bias[tid];
 
@@ -292,7 +295,7 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
     const int gz = zz & (G - 1);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int c = tid + k * ZM_THREADS;
+      const int c = tid + k * ZS_THREADS;
       if (c < CHUNKS) {
         const int ch = c % (CIN / 8), v = c / (CIN / 8);
         const int tx = v % ZM_PX, ty = v / ZM_PX;
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   auto store_plane = [&](int slot, const u32x4 (&r)[PER]) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int c = tid + k * ZM_THREADS;
+      const int c = tid + k * ZS_THREADS;
       if (c < CHUNKS) {
         const int ch = c % (CIN / 8), v = c / (CIN / 8);
         *reinterpret_cast<u32x4*>(&ring[slot * PLANE + v * CP + ch * 8]) = r[k];
@@ -326,49 +329,42 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
     const int z = zs + zi;
     u32x4 pre[PER];
     if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);
-    f32x4 acc[ZM_RPW][2];
+    f32x4 acc[ZM_RPW];
 #pragma unroll
-    for (int r = 0; r < ZM_RPW; ++r)
-#pragma unroll
-      for (int hf = 0; hf < 2; ++hf) acc[r][hf] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int r = 0; r < ZM_RPW; ++r) acc[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int dz = 0; dz < 3; ++dz) {
       const uint16_t* P = ring + ((zi + dz) % 3) * PLANE;
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-        for (int yr = 0; yr < ZM_RPW + 2; ++yr)
+        for (int yr = 0; yr < ZM_RPW + 2; ++yr) {
+          // B operand: voxel 16 hf + n16 + dx - 1 of the row, ci = 8 kq .. 8 kq + 7
+          const bf16x8 av =
+              *reinterpret_cast<const bf16x8*>(&P[((wrow * ZM_RPW + yr) * ZM_PX + hf * 16 + n16 + dx) * CP + kq * 8]);
 #pragma unroll
-          for (int hf = 0; hf < 2; ++hf) {
-            // B operand: voxel 16 hf + n16 + dx - 1 of the row, ci = 8 kq .. 8 kq + 7
-            const bf16x8 av =
-                *reinterpret_cast<const bf16x8*>(&P[((wv * ZM_RPW + yr) * ZM_PX + hf * 16 + n16 + dx) * CP + kq * 8]);
-#pragma unroll
-            for (int r = 0; r < ZM_RPW; ++r) {
-              const int dy = yr - r;
-              if (dy < 0 || dy > 2) continue;
-              acc[r][hf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[(dz * 3 + dy) * 3 + dx], av, acc[r][hf], 0, 0, 0);
-            }
+          for (int r = 0; r < ZM_RPW; ++r) {
+            const int dy = yr - r;
+            if (dy < 0 || dy > 2) continue;
+            acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[(dz * 3 + dy) * 3 + dx], av, acc[r], 0, 0, 0);
           }
+        }
     }
-    // acc[r][hf][q] holds output channel 4 kq + q of voxel (x0 + 16 hf + n16, y0 + row, z)
+    // acc[r][q] holds output channel 4 kq + q of voxel (x0 + 16 hf + n16, y0 + row, z)
 #pragma unroll
     for (int r = 0; r < ZM_RPW; ++r) {
-      const int ly = wv * ZM_RPW + r;
+      const int ly = wrow * ZM_RPW + r;
+      const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + hf * 16 + n16;
 #pragma unroll
-      for (int hf = 0; hf < 2; ++hf) {
-        const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + hf * 16 + n16;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int co = 4 * kq + q;
-          if (co < cout) {
-            const float val = acc[r][hf][q] + sbias[co];
-            if (MODE == CONV_OUT_F32_CF) {
-              y_out[((int64_t)b * cout + co) * G3 + sv] = val;
-            } else {
-              const int64_t o = ((int64_t)b * G3 + sv) * cout + co;
-              y_out[o] = addend ? val + addend[o] : val;
-            }
+      for (int q = 0; q < 4; ++q) {
+        const int co = 4 * kq + q;
+        if (co < cout) {
+          const float val = acc[r][q] + sbias[co];
+          if (MODE == CONV_OUT_F32_CF) {
+            y_out[((int64_t)b * cout + co) * G3 + sv] = val;
+          } else {
+            const int64_t o = ((int64_t)b * G3 + sv) * cout + co;
+            y_out[o] = addend ? val + addend[o] : val;
           }
         }
       }
@@ -389,8 +385,6 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm_narrow(const uint16_t* 
 // per SIMD: one wave's epilogue (bias, GELU, residual, BF16 stores) runs
 // while the other's MFMAs do.  Input fragments are read by both halves.  Also
 // the FP32 channels-last 32 -> 32 layer of the backward pass.
-constexpr int ZS_THREADS = 512;
-
 template <int MODE>
 __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* __restrict__ x,
                                                                const uint16_t* __restrict__ w,
@@ -864,7 +858,7 @@ int launch_fwd_narrow(const uint16_t* x, const uint16_t* w, const float* bias, i
   const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / ZM_ZC);
   AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
   TimedScope ts_(name, s);
-  hipLaunchKernelGGL((k_conv3_zm_narrow<MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
+  hipLaunchKernelGGL((k_conv3_zm_narrow<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias, (int)G,
                      cout, y, addend);
   AUGUR_LAUNCHED(name);
   return AUGUR_OK;
