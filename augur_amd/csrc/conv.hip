@@ -59,10 +59,11 @@ enum ConvMode : int {
 };
 
 // ------------------------------------------------------- z-marching forward
-// One workgroup (4 waves) computes a 32 x 16 (x, y) column of output voxels
-// over ZM_ZC consecutive z planes, all 32 output channels.  Each wave keeps
-// the whole weight tensor in registers (27 taps x CIN/16 MFMA B fragments)
-// and owns 4 output rows; the input planes z-1, z, z+1 (with a one-voxel
+// One workgroup computes a 32 x 16 (x, y) column of output voxels over ZM_ZC
+// consecutive z planes, all 32 output channels.  Each wave keeps the whole
+// weight tensor in registers (27 taps x CIN/16 MFMA A fragments) and owns RPW
+// output rows (4 rows, four waves; for CIN = 16, 2 rows and eight waves, so two
+// waves share a SIMD); the input planes z-1, z, z+1 (with a one-voxel
 // periodic halo) sit in a 3-plane LDS ring.  An input row fragment is loaded
 // once per (dz, dx, K-step) and feeds every output row it touches (dy = 0..2),
 // two MFMAs per LDS read on average.  The next plane is fetched into
@@ -72,8 +73,8 @@ constexpr int ZM_X = 32, ZM_Y = 16, ZM_ZC = 16, ZM_THREADS = 256, ZM_RPW = 4;
 constexpr int ZM_PX = ZM_X + 2, ZM_PY = ZM_Y + 2;
 constexpr int ZS_THREADS = 512;  // the eight-wave kernels (split outputs, narrow outputs)
 
-template <int CIN, int MODE>
-__global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restrict__ x,
+template <int CIN, int MODE, int RPW = ZM_RPW>
+__global__ __launch_bounds__(64 * (ZM_Y / RPW)) void k_conv3_zm(const uint16_t* __restrict__ x,
                                                          const uint16_t* __restrict__ w,
                                                          const float* __restrict__ bias, int G, int cout,
                                                          uint16_t* __restrict__ z_out, uint16_t* __restrict__ h_out,
@@ -82,7 +83,8 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
   constexpr int NH = CIN / 16;
   constexpr int PLANE = ZM_PY * ZM_PX * CP;
   constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
-  constexpr int PER = (CHUNKS + ZM_THREADS - 1) / ZM_THREADS;
+  constexpr int NT = 64 * (ZM_Y / RPW);  // threads: one wave per RPW rows
+  constexpr int PER = (CHUNKS + NT - 1) / NT;
   __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
   __shared__ __attribute__((aligned(16))) float sbias[32];
   const int tid = threadIdx.x;
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
     const int gz = zz & (G - 1);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int c = tid + k * ZM_THREADS;
+      const int c = tid + k * NT;
       if (c < CHUNKS) {
         const int ch = c % (CIN / 8), v = c / (CIN / 8);
         const int tx = v % ZM_PX, ty = v / ZM_PX;
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
   auto store_plane = [&](int slot, const u32x4 (&r)[PER]) __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int c = tid + k * ZM_THREADS;
+      const int c = tid + k * NT;
       if (c < CHUNKS) {
         const int ch = c % (CIN / 8), v = c / (CIN / 8);
         *reinterpret_cast<u32x4*>(&ring[slot * PLANE + v * CP + ch * 8]) = r[k];
@@ -150,9 +152,9 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
     const int z = zs + zi;
     u32x4 pre[PER];
     if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);  // in flight during the MFMAs below
-    f32x16 acc[ZM_RPW];
+    f32x16 acc[RPW];
 #pragma unroll
-    for (int r = 0; r < ZM_RPW; ++r)
+    for (int r = 0; r < RPW; ++r)
 #pragma unroll
       for (int q = 0; q < 16; ++q) acc[r][q] = 0.0f;
 #pragma unroll
@@ -161,14 +163,14 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-        for (int yr = 0; yr < ZM_RPW + 2; ++yr)
+        for (int yr = 0; yr < RPW + 2; ++yr)
 #pragma unroll
           for (int hh = 0; hh < NH; ++hh) {
             // B operand: input row voxel i + dx - 1, k-slice of lane
             const bf16x8 av =
-                *reinterpret_cast<const bf16x8*>(&P[((wv * ZM_RPW + yr) * ZM_PX + i + dx) * CP + hh * 16 + kh * 8]);
+                *reinterpret_cast<const bf16x8*>(&P[((wv * RPW + yr) * ZM_PX + i + dx) * CP + hh * 16 + kh * 8]);
 #pragma unroll
-            for (int r = 0; r < ZM_RPW; ++r) {
+            for (int r = 0; r < RPW; ++r) {
               const int dy = yr - r;
               if (dy < 0 || dy > 2) continue;
               acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[(dz * 3 + dy) * 3 + dx][hh], av, acc[r], 0, 0, 0);
@@ -179,8 +181,8 @@ __global__ __launch_bounds__(ZM_THREADS) void k_conv3_zm(const uint16_t* __restr
     // voxel (x0 + i, y0 + row, z): four consecutive channels per j
     const uint16_t* C = ring + ((zi + 1) % 3) * PLANE;  // input plane z (residual centre)
 #pragma unroll
-    for (int r = 0; r < ZM_RPW; ++r) {
-      const int ly = wv * ZM_RPW + r;
+    for (int r = 0; r < RPW; ++r) {
+      const int ly = wv * RPW + r;
       const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + i;  // voxel index within the scene
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -846,8 +848,13 @@ int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t 
       return AUGUR_OK;
     }
   }
-  hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
-                     cout, z, h, y, addend);
+  if constexpr (CIN == 16) {  // 108 weight VGPRs: eight waves of 2 rows fit two waves per SIMD
+    hipLaunchKernelGGL((k_conv3_zm<CIN, MODE, 2>), dim3((unsigned)blocks), dim3(64 * (ZM_Y / 2)), 0, s, x, w, bias,
+                       (int)G, cout, z, h, y, addend);
+  } else {
+    hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
+                       cout, z, h, y, addend);
+  }
   AUGUR_LAUNCHED(name);
   return AUGUR_OK;
 }
