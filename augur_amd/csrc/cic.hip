@@ -120,6 +120,27 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_add(uint32_t* out, int64_t 
     if (base + j < n) out[base + j] += o;
 }
 
+// For a short list of block sums: workgroup b adds sum(sums[0 .. b)) to its
+// tile itself (a block reduction of at most SC_FUSE values), which saves the
+// separate scan of the block sums.
+constexpr int64_t SC_FUSE = 1024;
+__global__ __launch_bounds__(SC_THREADS) void k_scan_add_prefix(uint32_t* out, int64_t n, const uint32_t* sums,
+                                                                const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
+  __shared__ uint32_t part[SC_THREADS / 64];
+  const int b = blockIdx.x;
+  uint32_t v = 0;
+  for (int j = threadIdx.x; j < b; j += SC_THREADS) v += sums[j];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t o = 0;
+  for (int w = 0; w < SC_THREADS / 64; ++w) o += part[w];
+  const int64_t base = (int64_t)b * SC_TILE;
+  for (int j = threadIdx.x; j < SC_TILE; j += SC_THREADS)
+    if (base + j < n) out[base + j] += o;
+}
+
 size_t scan_workspace_bytes(int64_t n) {
   size_t b = 0;
   while (n > SC_TILE) {
@@ -198,6 +219,12 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, h
   void* rest = (char*)offs + align_up(nb * sizeof(uint32_t));
   hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, in, out, n, sums, gate);
   AUGUR_LAUNCHED("scan_block");
+  if (nb <= SC_FUSE) {
+    hipLaunchKernelGGL(k_scan_add_prefix, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, out, n, (const uint32_t*)sums,
+                       gate);
+    AUGUR_LAUNCHED("scan_add_prefix");
+    return AUGUR_OK;
+  }
   int rc = exclusive_scan_u32(sums, offs, nb, rest, s, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, out, n, offs, gate);
