@@ -76,7 +76,7 @@ constexpr int ZS_THREADS = 512;  // the eight-wave kernels (split outputs, narro
 template <int CIN, int MODE, int RPW = ZM_RPW>
 __global__ __launch_bounds__(64 * (ZM_Y / RPW)) void k_conv3_zm(const uint16_t* __restrict__ x,
                                                          const uint16_t* __restrict__ w,
-                                                         const float* __restrict__ bias, int G, int cout,
+                                                         const float* __restrict__ bias, int G, int zc, int cout,
                                                          uint16_t* __restrict__ z_out, uint16_t* __restrict__ h_out,
                                                          float* __restrict__ y_out, const float* __restrict__ addend) {
   constexpr int CP = CIN + 8;  // padded channel stride (conflict-free 16-byte LDS reads)
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(64 * (ZM_Y / RPW)) void k_conv3_zm(const uint16_t* 
   __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
   __shared__ __attribute__((aligned(16))) float sbias[32];
   const int tid = threadIdx.x;
-  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / ZM_ZC;
+  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / zc;
   int t = blockIdx.x;
   const int bx = t % ntx;
   t /= ntx;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(64 * (ZM_Y / RPW)) void k_conv3_zm(const uint16_t* 
   t /= nty;
   const int bz = t % ntz;
   const int b = t / ntz;
-  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * ZM_ZC;
+  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * zc;
   const int64_t G3 = (int64_t)G * G * G;
   const uint16_t* xb = x + (int64_t)b * G3 * CIN;
   const int lane = tid & 63, wv = tid >> 6;
@@ -148,10 +148,10 @@ __global__ __launch_bounds__(64 * (ZM_Y / RPW)) void k_conv3_zm(const uint16_t* 
   __syncthreads();
 
 #pragma unroll 1
-  for (int zi = 0; zi < ZM_ZC; ++zi) {
+  for (int zi = 0; zi < zc; ++zi) {
     const int z = zs + zi;
     u32x4 pre[PER];
-    if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);  // in flight during the MFMAs below
+    if (zi + 1 < zc) load_plane(z + 2, pre);  // in flight during the MFMAs below
     f32x16 acc[RPW];
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(64 * (ZM_Y / RPW)) void k_conv3_zm(const uint16_t* 
       }
     }
     __syncthreads();  // every wave is done with the slot of plane z - 1
-    if (zi + 1 < ZM_ZC) {
+    if (zi + 1 < zc) {
       store_plane(zi % 3, pre);  // plane z + 2 replaces plane z - 1
       __syncthreads();
     }
@@ -261,7 +261,7 @@ typedef __attribute__((ext_vector_type(4))) float f32x4;
 template <int MODE>
 __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* __restrict__ x,
                                                                 const uint16_t* __restrict__ w,
-                                                                const float* __restrict__ bias, int G, int cout,
+                                                                const float* __restrict__ bias, int G, int zc, int cout,
                                                                 float* __restrict__ y_out,
                                                                 const float* __restrict__ addend) {
   constexpr int CIN = 32, CP = CIN + 8;
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
   __shared__ __attribute__((aligned(16))) float sbias[32];
   const int tid = threadIdx.x;
-  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / ZM_ZC;
+  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / zc;
   int t = blockIdx.x;
   const int bx = t % ntx;
   t /= ntx;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   t /= nty;
   const int bz = t % ntz;
   const int b = t / ntz;
-  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * ZM_ZC;
+  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * zc;
   const int64_t G3 = (int64_t)G * G * G;
   const uint16_t* xb = x + (int64_t)b * G3 * CIN;
   const int lane = tid & 63, wv = tid >> 6;
@@ -327,10 +327,10 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   __syncthreads();
 
 #pragma unroll 1
-  for (int zi = 0; zi < ZM_ZC; ++zi) {
+  for (int zi = 0; zi < zc; ++zi) {
     const int z = zs + zi;
     u32x4 pre[PER];
-    if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);
+    if (zi + 1 < zc) load_plane(z + 2, pre);
     f32x4 acc[ZM_RPW];
 #pragma unroll
     for (int r = 0; r < ZM_RPW; ++r) acc[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
       }
     }
     __syncthreads();
-    if (zi + 1 < ZM_ZC) {
+    if (zi + 1 < zc) {
       store_plane(zi % 3, pre);
       __syncthreads();
     }
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
 template <int MODE>
 __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* __restrict__ x,
                                                                const uint16_t* __restrict__ w,
-                                                               const float* __restrict__ bias, int G,
+                                                               const float* __restrict__ bias, int G, int zc,
                                                                uint16_t* __restrict__ z_out,
                                                                uint16_t* __restrict__ h_out,
                                                                float* __restrict__ y_out,
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
   __shared__ __attribute__((aligned(16))) uint16_t ring[3 * PLANE];
   __shared__ __attribute__((aligned(16))) float sbias[32];
   const int tid = threadIdx.x;
-  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / ZM_ZC;
+  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / zc;
   int t = blockIdx.x;
   const int bx = t % ntx;
   t /= ntx;
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
   t /= nty;
   const int bz = t % ntz;
   const int b = t / ntz;
-  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * ZM_ZC;
+  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * zc;
   const int64_t G3 = (int64_t)G * G * G;
   const uint16_t* xb = x + (int64_t)b * G3 * CIN;
   const int lane = tid & 63, wv = tid >> 6;
@@ -458,10 +458,10 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
   __syncthreads();
 
 #pragma unroll 1
-  for (int zi = 0; zi < ZM_ZC; ++zi) {
+  for (int zi = 0; zi < zc; ++zi) {
     const int z = zs + zi;
     u32x4 pre[PER];
-    if (zi + 1 < ZM_ZC) load_plane(z + 2, pre);
+    if (zi + 1 < zc) load_plane(z + 2, pre);
     f32x4 acc[ZM_RPW][2];
 #pragma unroll
     for (int r = 0; r < ZM_RPW; ++r)
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
       }
     }
     __syncthreads();
-    if (zi + 1 < ZM_ZC) {
+    if (zi + 1 < zc) {
       store_plane(zi % 3, pre);
       __syncthreads();
     }
@@ -828,85 +828,121 @@ int conv_tiles_ok(int64_t G) { return is_pow2(G) && G >= 32 && G <= 1024; }
 #ifndef AUGUR_CONV_SPLIT
 #define AUGUR_CONV_SPLIT 1
 #endif
+// Kernel families of the forward convolution (the tile variants a caller or
+// an autotuner may pick; AUGUR_CONV_TILE_* in augur.h):
+//   ZM     k_conv3_zm: 32x32x16 MFMAs, four waves of 4 rows (CIN 16: eight waves of 2 rows)
+//   SPLIT  k_conv3_zm_split: eight waves, each half the output channels (CIN 32, 32 outputs)
+//   NARROW k_conv3_zm_narrow: 16x16x32 MFMAs over at most 16 output channels (CIN 32)
+// Every family computes each output voxel with the same per-voxel MFMA chain
+// for any z planes per workgroup, so results do not depend on zc.
+enum ConvTile : int { TILE_DEFAULT = 0, TILE_ZM = 1, TILE_SPLIT = 2, TILE_NARROW = 3 };
+
+int conv_variant_mask(int cin, int mode, int cout) {
+  int m = 1 << TILE_ZM;
+  if (cin == 32 && (mode == CONV_GELU || mode == CONV_RES_GELU || (mode == CONV_OUT_F32_CL && cout == 32)))
+    m |= 1 << TILE_SPLIT;
+  if (cin == 32 && cout <= 16 && (mode == CONV_OUT_F32_CF || mode == CONV_OUT_F32_CL)) m |= 1 << TILE_NARROW;
+  return m;
+}
+
+int conv_default_variant(int mask) {
+  if (mask & (1 << TILE_NARROW)) return TILE_NARROW;
+  if (AUGUR_CONV_SPLIT && (mask & (1 << TILE_SPLIT))) return TILE_SPLIT;
+  return TILE_ZM;
+}
+
 template <int CIN, int MODE>
 int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t B, int64_t G, int cout, uint16_t* z,
-               uint16_t* h, float* y, const float* addend, hipStream_t s, const char* name) {
-  const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / ZM_ZC);
+               uint16_t* h, float* y, const float* addend, int variant, int zc, hipStream_t s, const char* name) {
+  const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / zc);
   AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
   TimedScope ts_(name, s);
-  if constexpr (AUGUR_CONV_SPLIT && CIN == 32 && (MODE == CONV_GELU || MODE == CONV_RES_GELU)) {
-    hipLaunchKernelGGL((k_conv3_zm_split<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias, (int)G,
-                       z, h, y, addend);
-    AUGUR_LAUNCHED(name);
-    return AUGUR_OK;
-  }
-  if constexpr (AUGUR_CONV_SPLIT && CIN == 32 && MODE == CONV_OUT_F32_CL) {
-    if (cout == 32) {
+  if (variant == TILE_SPLIT) {
+    if constexpr (CIN == 32 && MODE != CONV_OUT_F32_CF) {
       hipLaunchKernelGGL((k_conv3_zm_split<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias,
-                         (int)G, z, h, y, addend);
+                         (int)G, zc, z, h, y, addend);
       AUGUR_LAUNCHED(name);
       return AUGUR_OK;
     }
+  } else if (variant == TILE_NARROW) {
+    if constexpr (CIN == 32 && (MODE == CONV_OUT_F32_CF || MODE == CONV_OUT_F32_CL)) {
+      hipLaunchKernelGGL((k_conv3_zm_narrow<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias,
+                         (int)G, zc, cout, y, addend);
+      AUGUR_LAUNCHED(name);
+      return AUGUR_OK;
+    }
+  } else if (variant == TILE_ZM) {
+    if constexpr (CIN == 16) {  // 108 weight VGPRs: eight waves of 2 rows fit two waves per SIMD
+      hipLaunchKernelGGL((k_conv3_zm<CIN, MODE, 2>), dim3((unsigned)blocks), dim3(64 * (ZM_Y / 2)), 0, s, x, w,
+                         bias, (int)G, zc, cout, z, h, y, addend);
+    } else {
+      hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias,
+                         (int)G, zc, cout, z, h, y, addend);
+    }
+    AUGUR_LAUNCHED(name);
+    return AUGUR_OK;
   }
-  if constexpr (CIN == 16) {  // 108 weight VGPRs: eight waves of 2 rows fit two waves per SIMD
-    hipLaunchKernelGGL((k_conv3_zm<CIN, MODE, 2>), dim3((unsigned)blocks), dim3(64 * (ZM_Y / 2)), 0, s, x, w, bias,
-                       (int)G, cout, z, h, y, addend);
-  } else {
-    hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias, (int)G,
-                       cout, z, h, y, addend);
-  }
-  AUGUR_LAUNCHED(name);
-  return AUGUR_OK;
-}
-
-template <int MODE>
-int launch_fwd_narrow(const uint16_t* x, const uint16_t* w, const float* bias, int64_t B, int64_t G, int cout,
-                      float* y, const float* addend, hipStream_t s, const char* name) {
-  const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / ZM_ZC);
-  AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
-  TimedScope ts_(name, s);
-  hipLaunchKernelGGL((k_conv3_zm_narrow<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias, (int)G,
-                     cout, y, addend);
-  AUGUR_LAUNCHED(name);
-  return AUGUR_OK;
+  set_error("conv: tile variant %d has no kernel for cin %d, mode %d", variant, CIN, MODE);
+  return AUGUR_EUNSUPPORTED;
 }
 
 }  // namespace augur
 
 extern "C" {
 
-int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
-                         int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
-                         void* stream) {
+int augur_conv3_tile_variants(int cin, int mode, int cout) {
+  if ((cin != 16 && cin != 32) || mode < CONV_OUT_F32_CF || mode > CONV_OUT_F32_CL || cout < 1 || cout > 32) return 0;
+  if (mode == CONV_RES_GELU && cin != 32) return 0;
+  return conv_variant_mask(cin, mode, cout);
+}
+
+int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
+                              int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
+                              int variant, int zc, void* stream) {
   AUGUR_REQUIRE(conv_tiles_ok(G), AUGUR_EUNSUPPORTED, "conv: G must be a power of two in [32, 1024]");
   AUGUR_REQUIRE(B >= 1 && x && w, AUGUR_EINVAL, "conv: bad arguments");
   AUGUR_REQUIRE(cin == 16 || cin == 32, AUGUR_EUNSUPPORTED, "conv: cin must be 16 or 32 (pad channels)");
   AUGUR_REQUIRE(cout >= 1 && cout <= 32, AUGUR_EINVAL, "conv: cout must be 1..32");
+  AUGUR_REQUIRE(is_pow2(zc) && zc >= 2 && zc <= G, AUGUR_EINVAL, "conv: zc must be a power of two in [2, G]");
+  const int mask = conv_variant_mask(cin, mode, cout);
+  if (variant == TILE_DEFAULT) variant = conv_default_variant(mask);
+  AUGUR_REQUIRE(variant >= TILE_ZM && variant <= TILE_NARROW && ((mask >> variant) & 1), AUGUR_EUNSUPPORTED,
+                "conv: tile variant cannot run this layer");
   hipStream_t s = as_stream(stream);
   switch (mode) {
     case CONV_OUT_F32_CF:
       AUGUR_REQUIRE(y_out && bias, AUGUR_EINVAL, "conv: mode 0 needs y_out and bias");
-      if (cin == 32 && cout <= 16)
-        return launch_fwd_narrow<CONV_OUT_F32_CF>(x, w, bias, B, G, cout, y_out, nullptr, s, "conv_fwd_out");
-      return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr, s, "conv_fwd_out")
-                       : launch_fwd<32, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr, s, "conv_fwd_out");
+      return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr,
+                                                         variant, zc, s, "conv_fwd_out")
+                       : launch_fwd<32, CONV_OUT_F32_CF>(x, w, bias, B, G, cout, nullptr, nullptr, y_out, nullptr,
+                                                         variant, zc, s, "conv_fwd_out");
     case CONV_GELU:
       AUGUR_REQUIRE(h_out && bias && cout == 32, AUGUR_EINVAL, "conv: mode 1 needs h_out, bias, cout 32");
-      return cin == 16 ? launch_fwd<16, CONV_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, s, "conv_fwd_gelu")
-                       : launch_fwd<32, CONV_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, s, "conv_fwd_gelu");
+      return cin == 16 ? launch_fwd<16, CONV_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, variant, zc,
+                                                   s, "conv_fwd_gelu")
+                       : launch_fwd<32, CONV_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, variant, zc,
+                                                   s, "conv_fwd_gelu");
     case CONV_RES_GELU:
       AUGUR_REQUIRE(h_out && bias && cout == 32 && cin == 32, AUGUR_EINVAL, "conv: mode 2 needs cin = cout = 32");
-      return launch_fwd<32, CONV_RES_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, s, "conv_fwd_res");
+      return launch_fwd<32, CONV_RES_GELU>(x, w, bias, B, G, 32, z_out, h_out, nullptr, nullptr, variant, zc, s,
+                                           "conv_fwd_res");
     case CONV_OUT_F32_CL:
       AUGUR_REQUIRE(y_out, AUGUR_EINVAL, "conv: mode 3 needs y_out");
-      if (cin == 32 && cout <= 16)
-        return launch_fwd_narrow<CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, y_out, addend, s, "conv_bwd_data");
-      return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend, s, "conv_bwd_data")
-                       : launch_fwd<32, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend, s, "conv_bwd_data");
+      return cin == 16 ? launch_fwd<16, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend,
+                                                         variant, zc, s, "conv_bwd_data")
+                       : launch_fwd<32, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend,
+                                                         variant, zc, s, "conv_bwd_data");
     default:
       set_error("conv: unknown mode %d", mode);
       return AUGUR_EINVAL;
   }
+}
+
+int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
+                         int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
+                         void* stream) {
+  return augur_conv3_bf16_fwd_tile(x, cin, w, bias, B, G, mode, cout, z_out, h_out, y_out, addend, TILE_DEFAULT,
+                                   ZM_ZC, stream);
 }
 
 size_t augur_conv3_wgrad_workspace_bytes(int cin, int64_t B, int64_t G, int nwg) {

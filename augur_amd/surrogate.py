@@ -52,8 +52,10 @@ def kernel_weights(w_taps, cinp):
     return out
 
 
-def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None):
-    """x: BF16 channels-last (B, G, G, G, cin); w: BF16 (27, 32, cin); bias FP32 (32,)."""
+def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None):
+    """x: BF16 channels-last (B, G, G, G, cin); w: BF16 (27, 32, cin); bias FP32 (32,).
+    tile: (kernel family, z planes per workgroup) from the autotuner, or None
+    for the library's default tile."""
     dev = _dev(x)
     B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
     y = h = None
@@ -63,8 +65,12 @@ def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None):
         y = torch.empty(B, G, G, G, cout, dtype=torch.float32, device=dev)
     else:
         h = torch.empty(B, G, G, G, 32, dtype=BF16, device=dev)
-    _lib.call("augur_conv3_bf16_fwd", _p(x), cin, _p(w), _p(bias), B, G, mode, cout, _p(z_out), _p(h), _p(y),
-              _p(addend), _stream(dev))
+    if tile is None:
+        _lib.call("augur_conv3_bf16_fwd", _p(x), cin, _p(w), _p(bias), B, G, mode, cout, _p(z_out), _p(h), _p(y),
+                  _p(addend), _stream(dev))
+    else:
+        _lib.call("augur_conv3_bf16_fwd_tile", _p(x), cin, _p(w), _p(bias), B, G, mode, cout, _p(z_out), _p(h),
+                  _p(y), _p(addend), int(tile[0]), int(tile[1]), _stream(dev))
     return h if h is not None else y
 
 
@@ -137,6 +143,8 @@ class LocalNet3D:
                                                                           b.to(self.device)))
         self.adam = None
         self.step_count = 0
+        self.tuner = None      # augur_amd.autotune.ConvTuner with the tuned tiles (None: library defaults)
+        self._tuning = False
         self.sync_bf16()
 
     # ----------------------------------------------------------- weights
@@ -173,6 +181,45 @@ class LocalNet3D:
         """[(weight (co, ci, 3, 3, 3), bias (co,))] for a torch reference."""
         return [(from_taps(w), b[: w.shape[1]]) for w, b in zip(self.w, self.b)]
 
+    # ------------------------------------------------------ tuned tiles
+    def _conv(self, x, w, b, mode, cout, z_out=None, addend=None):
+        tile = None
+        if self.tuner is not None:
+            if self._tuning:
+                tile = self.tuner.tune_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend)
+            else:
+                tile = self.tuner.fwd_tile(x.shape[-1], mode, cout, x.shape[0], x.shape[1], addend is not None)
+        return conv3_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, tile=tile)
+
+    def _wgrad(self, x, dz):
+        nwg = None
+        if self.tuner is not None:
+            nwg = (self.tuner.tune_wgrad(x, dz) if self._tuning
+                   else self.tuner.wgrad_nwg(x.shape[-1], x.shape[0], x.shape[1]))
+        return conv3_wgrad(x, dz, nwg=nwg)
+
+    def autotune(self, B, G, tuner=None, reps=5, backward=True, seed=0):
+        """Choose every conv call's tile for batch B on a G^3 grid by timing the
+        candidates (augur_amd.autotune.ConvTuner) on one forward (and backward)
+        pass of random activations; later passes of that shape use the winners.
+        Returns the tuner (its ``report()`` lists the candidates' times)."""
+        from .autotune import ConvTuner
+        self.tuner = tuner if tuner is not None else (self.tuner or ConvTuner(reps=reps))
+        dev = self.w[0].device
+        g = torch.Generator(device=dev).manual_seed(seed)
+        x = torch.randn(B, G, G, G, self.cinp, device=dev, generator=g).to(BF16)
+        self._tuning = True
+        try:
+            self.forward_packed(x, save=backward)
+            if backward:
+                dy = 1e-3 * torch.randn(B, self.cout, G, G, G, device=dev, generator=g)
+                self.backward(dy)
+        finally:
+            self._tuning = False
+            self._saved = None
+        torch.cuda.synchronize(dev)
+        return self.tuner
+
     # ----------------------------------------------------------- forward
     def __call__(self, x):
         return self.forward(x)
@@ -186,15 +233,15 @@ class LocalNet3D:
         zs, hs = [], [xc]
         need_z = save
         z = torch.empty(xc.shape[:-1] + (32,), dtype=BF16, device=xc.device) if need_z else None
-        h = conv3_fwd(xc, self.wk[0], self.b[0], MODE_GELU, 32, z_out=z)
+        h = self._conv(xc, self.wk[0], self.b[0], MODE_GELU, 32, z_out=z)
         zs.append(z)
         hs.append(h)
         for k in range(1, self.layers):
             z = torch.empty_like(h) if need_z else None
-            h = conv3_fwd(h, self.wk[k], self.b[k], MODE_RES_GELU, 32, z_out=z)
+            h = self._conv(h, self.wk[k], self.b[k], MODE_RES_GELU, 32, z_out=z)
             zs.append(z)
             hs.append(h)
-        y = conv3_fwd(h, self.wk[-1], self.b[-1], MODE_OUT_CF, self.cout)
+        y = self._conv(h, self.wk[-1], self.b[-1], MODE_OUT_CF, self.cout)
         if save:
             self._saved = (zs, hs)
         return y
@@ -206,18 +253,18 @@ class LocalNet3D:
         grads = [None] * (self.layers + 1)
         dyc = pack_cl(dy, 32)
         # output layer
-        dw, db = conv3_wgrad(hs[-1], dyc)
+        dw, db = self._wgrad(hs[-1], dyc)
         grads[-1] = (dw[:, : self.cout, :], db[: self.cout])
         wt = flip_weights(self.wk[-1], 32, self.cout, 32)
-        dh = conv3_fwd(dyc, wt, None, MODE_OUT_CL, 32)            # dL/dh_L, FP32 channels-last
+        dh = self._conv(dyc, wt, None, MODE_OUT_CL, 32)            # dL/dh_L, FP32 channels-last
         for k in range(self.layers - 1, 0, -1):
             dz = gelu_bwd(dh, zs[k])
-            dw, db = conv3_wgrad(hs[k], dz)
+            dw, db = self._wgrad(hs[k], dz)
             grads[k] = (dw, db)
             wt = flip_weights(self.wk[k], 32, 32, 32)
-            dh = conv3_fwd(dz, wt, None, MODE_OUT_CL, 32, addend=dh)  # residual path adds dh
+            dh = self._conv(dz, wt, None, MODE_OUT_CL, 32, addend=dh)  # residual path adds dh
         dz = gelu_bwd(dh, zs[0])
-        dw, db = conv3_wgrad(hs[0], dz)
+        dw, db = self._wgrad(hs[0], dz)
         grads[0] = (dw[:, :, : self.cin], db)
         self._saved = None
         return grads

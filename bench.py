@@ -61,6 +61,12 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
 
     sur = PMSurrogate(G, 1.0, 1.0, dev, mode="correction")
     t0 = time.perf_counter()
+    tuner = sur.net.autotune(1, G)   # conv tiles of this grid (forward and backward calls)
+    t_tune = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    print(f"[bench] conv tiles autotuned in {t_tune:.2f}s: "
+          f"{ {k: v['best'] for k, v in tuner.report().items()} }", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
     train = []
     for k in range(a.sur_scenes):
         p, _ = scenes.cfg2(seed=1000 + k, n=N)
@@ -138,6 +144,7 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
                      "achieved": res_k["tflops"] if res_k else None, "peak": MFMA_BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": res_k["frac"] if res_k else None, "traffic": None},
         "conv_kernels": conv,
+        "conv_tiles": {"autotune_seconds": t_tune, "chosen": {k: v["best"] for k, v in tuner.report().items()}},
         "kernels": {k: {"us_avg": v[0] / v[1] * 1e3, "launches": v[1]} for k, v in timing.items()},
     }
 
@@ -305,6 +312,11 @@ def train_bench(a):
     rho, force = _batch_grids(sur, *batches[0][:3])
     sur.calibrate(rho[0], force[0])
     del rho, force
+    t0 = time.perf_counter()
+    tuner = sur.net.autotune(B, G)   # conv tiles of the batch (forward and backward calls)
+    t_tune = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    print(f"[bench] conv tiles autotuned in {t_tune:.2f}s", file=sys.stderr, flush=True)
     # one eager step with the kernel timers on (they are not recorded inside a replay)
     _lib.timing_enable(True)
     loss0, grads = batch_grads(sur, *batches[0])
@@ -341,6 +353,7 @@ def train_bench(a):
         "scene_steps_per_s": it_s * B, "particles_per_s": it_s * total,
         "network_tflops": flop / (sec / a.steps) / 1e12, "mfma_frac": flop / (sec / a.steps) / 1e12 / 2500.0,
         "eager_step_kernels_ms": {k: v[0] for k, v in timing.items()}, "eager_step_conv_ms": conv_ms,
+        "conv_tiles": {"autotune_seconds": t_tune, "chosen": {k: v["best"] for k, v in tuner.report().items()}},
         "loss_first": float(loss0), "loss_last": float(losses[-1]),
         "peak_device_bytes": torch.cuda.max_memory_allocated(),
     }

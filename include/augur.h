@@ -273,11 +273,34 @@ int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* forc
  *   2: as 1 with a residual: h_out = x + gelu(z) (cin = cout = 32);
  *   3: y_out FP32 channels-last [B][G^3][cout] = conv (+ addend, same layout),
  *      no bias (the data gradient, with augur_conv3_flip_weights()).
- * gelu is the erf form.  Replaces the nn.Conv2d + F.gelu layers of LocalNet
- * (scripts/scatter_field.py:15-26), here in 3D with periodic padding. */
+ * gelu is the tanh form (torch's approximate='tanh').  Replaces the
+ * nn.Conv2d + F.gelu layers of LocalNet (scripts/scatter_field.py:15-26),
+ * here in 3D with periodic padding.  Runs the library's default tile. */
 int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
                          int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
                          void* stream);
+
+/* The same convolution on an explicit tile: a kernel family (variant) and the
+ * number of z planes each workgroup marches through (zc, a power of two in
+ * [2, G]; the default is 16).  Families:
+ *   AUGUR_CONV_TILE_ZM     32x32x16 MFMAs, every wave all 32 output channels;
+ *   AUGUR_CONV_TILE_SPLIT  eight waves, each half the output channels
+ *                          (cin 32 with 32 outputs: modes 1, 2, and 3 with cout 32);
+ *   AUGUR_CONV_TILE_NARROW 16x16x32 MFMAs over at most 16 outputs (cin 32, modes 0, 3);
+ *   AUGUR_CONV_TILE_DEFAULT the library's choice (NARROW, else SPLIT, else ZM).
+ * A family computes each voxel the same way for every zc, so zc changes the
+ * time and not the bits.  augur_conv3_tile_variants() returns the bit mask
+ * (1 << family) of the families that can run a layer (0: unsupported layer).
+ * No counterpart in the reference (cuDNN picks its own algorithms); the
+ * autotuner in augur_amd/autotune.py times the candidates. */
+#define AUGUR_CONV_TILE_DEFAULT 0
+#define AUGUR_CONV_TILE_ZM 1
+#define AUGUR_CONV_TILE_SPLIT 2
+#define AUGUR_CONV_TILE_NARROW 3
+int augur_conv3_tile_variants(int cin, int mode, int cout);
+int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
+                              int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
+                              int variant, int zc, void* stream);
 
 /* Weight gradient dw[27][32][cin] = sum_v dz[v][co] x[v + o_tap][ci] and
  * db[32] = sum_v dz[v][co] (FP32), dz BF16 channels-last [B][G^3][32].

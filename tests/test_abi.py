@@ -55,3 +55,18 @@ def test_ops_refuse_cpu_tensors():
         ops.cic_deposit(pos, None, 8, 0.125, 0.0, True)
     with pytest.raises(RuntimeError, match="ROCm GPU only"):
         ops.direct_sum_f64(pos[0], 0.1)
+
+
+def test_conv_tile_candidates_host_only(lib):
+    """The autotuner's candidate tiles come from the library's own rule (no GPU needed)."""
+    from augur_amd import autotune as A
+    assert A.variants(16, 1, 32) == [A.TILE_ZM]
+    assert A.variants(32, 2, 32) == [A.TILE_ZM, A.TILE_SPLIT]
+    assert A.variants(32, 0, 3) == [A.TILE_ZM, A.TILE_NARROW]
+    assert A.variants(32, 3, 32) == [A.TILE_ZM, A.TILE_SPLIT]
+    assert A.variants(32, 3, 16) == [A.TILE_ZM, A.TILE_NARROW]
+    assert A.variants(16, 2, 32) == [] and A.variants(8, 1, 32) == [] and A.variants(32, 1, 33) == []
+    assert A.candidates(32, 2, 32, 32) == [(1, 8), (1, 16), (1, 32), (2, 8), (2, 16), (2, 32)]
+    assert len(A.candidates(32, 0, 3, 256)) == 10
+    t = A.ConvTuner()
+    assert t.fwd_tile(32, 2, 32, 1, 256) is None and t.wgrad_nwg(32, 1, 256) is None

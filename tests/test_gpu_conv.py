@@ -148,3 +148,98 @@ def test_adam_matches_torch(dev):
         tp.grad = gr.double()
         opt.step()
     assert (p.cpu().double() - tp.detach()).abs().max() < 1e-6
+
+
+# layer signatures of the surrogate's conv calls: (cin, mode, cout, with addend)
+TILE_LAYERS = [(16, 1, 32, False), (32, 1, 32, False), (32, 2, 32, False), (32, 0, 3, False),
+               (32, 0, 32, False), (32, 3, 32, True), (32, 3, 16, False), (16, 3, 16, True)]
+
+
+@pytest.mark.parametrize("cin,mode,cout,with_add", TILE_LAYERS)
+def test_every_conv_tile_matches_reference(dev, cin, mode, cout, with_add):
+    """Every (kernel family, z planes per workgroup) tile the library offers for a
+    layer agrees with the float64 conv; within a family every zc gives the same bits."""
+    from augur_amd import autotune as A
+    from augur_amd import surrogate as S
+    g = torch.Generator().manual_seed(100 * cin + 10 * mode + cout)
+    B, G = 2, 64
+    x = bf16_round(torch.randn(B, cin, G, G, G, generator=g))
+    w = bf16_round(torch.randn(27, 32, cin, generator=g) * 0.1)
+    w[:, cout:] = 0
+    b = torch.randn(32, generator=g) * 0.1
+    xc = S.pack_cl(x.to(dev), cin)
+    wk = S.kernel_weights(w.to(dev), cin)
+    ref = ref_conv(x, S.from_taps(w), None if mode == 3 else b)[:, :cout]   # (B, cout, G, G, G)
+    add = torch.randn(B, G, G, G, cout, generator=g) if with_add else None
+    if mode == 3:
+        ref = ref.permute(0, 2, 3, 4, 1) + (add.double() if with_add else 0)
+    elif mode in (1, 2):
+        z = ref.permute(0, 2, 3, 4, 1)
+        ref = F.gelu(z, approximate='tanh') + (x.double().permute(0, 2, 3, 4, 1) if mode == 2 else 0)
+    cands = A.candidates(cin, mode, cout, G)
+    fams = {v for v, _ in cands}
+    assert A.TILE_ZM in fams and len(cands) == 4 * len(fams)
+    if cin == 32 and cout <= 16 and mode in (0, 3):
+        assert A.TILE_NARROW in fams
+    if cin == 32 and cout == 32 and mode in (1, 2, 3):
+        assert A.TILE_SPLIT in fams
+    tol = (1e-4 if mode in (0, 3) else 1.5e-2) * ref.abs().max()
+    first = {}
+    for tile in cands:
+        y = S.conv3_fwd(xc, wk, b.to(dev), mode, cout, addend=None if add is None else add.to(dev), tile=tile)
+        err = (y.cpu().double() - ref).abs().max()
+        assert err < tol, (tile, float(err))
+        if tile[0] in first:
+            assert torch.equal(y, first[tile[0]]), tile
+        else:
+            first[tile[0]] = y
+    # the default entry point is the library's default family at zc 16
+    y0 = S.conv3_fwd(xc, wk, b.to(dev), mode, cout, addend=None if add is None else add.to(dev))
+    assert any(torch.equal(y0, v) for v in first.values())
+
+
+def test_conv_tile_rejects_unavailable_family(dev):
+    from augur_amd import _lib
+    from augur_amd import autotune as A
+    from augur_amd import surrogate as S
+    x = torch.zeros(1, 32, 32, 32, 16, dtype=torch.bfloat16, device=dev)
+    wk = torch.zeros(27, 32, 16, dtype=torch.bfloat16, device=dev)
+    b = torch.zeros(32, device=dev)
+    with pytest.raises(_lib.AugurError, match="tile variant"):
+        S.conv3_fwd(x, wk, b, S.MODE_GELU, 32, tile=(A.TILE_SPLIT, 16))   # split needs 32 input channels
+    with pytest.raises(_lib.AugurError, match="zc"):
+        S.conv3_fwd(x, wk, b, S.MODE_GELU, 32, tile=(A.TILE_ZM, 24))
+
+
+def test_autotuned_localnet_matches_default_tiles(dev, tmp_path):
+    """autotune() times every candidate of every conv call of a forward and
+    backward pass; the tuned net's outputs and gradients agree with the
+    default tiles', and the table round-trips through its JSON file."""
+    from augur_amd import autotune as A
+    from augur_amd.surrogate import LocalNet3D
+    B, G = 2, 64
+    net = LocalNet3D(cin=4, cout=3, device=dev, init_std=0.05)
+    g = torch.Generator(device=dev).manual_seed(3)
+    x = torch.randn(B, 4, G, G, G, device=dev, generator=g)
+    dy = torch.randn(B, 3, G, G, G, device=dev, generator=g)
+    y_def = net.forward(x, save=True)
+    g_def = net.backward(dy)
+    tuner = net.autotune(B, G, reps=2)
+    rep = tuner.report()
+    # 6 forward calls: conv_in, 4 residual layers with one shape, conv_out; backward: the output
+    # layer's data gradient, the residual layers' (with addend); three weight-gradient shapes
+    fwd = [k for k in rep if k.startswith("fwd:")]
+    wg = [k for k in rep if k.startswith("wgrad:")]
+    assert len(fwd) == 5 and len(wg) == 2, sorted(rep)
+    for k, v in rep.items():
+        assert v["best"] in v["ms"] and min(v["ms"].values()) == v["ms"][v["best"]], (k, v)
+    y_t = net.forward(x, save=True)
+    g_t = net.backward(dy)
+    assert (y_t - y_def).abs().max() <= 1e-2 * y_def.abs().max() + 1e-6
+    for (dw0, db0), (dw1, db1) in zip(g_def, g_t):
+        assert (dw1 - dw0).abs().max() <= 2e-2 * dw0.abs().max() + 1e-6
+        assert (db1 - db0).abs().max() <= 2e-2 * db0.abs().max() + 1e-6
+    path = str(tmp_path / "tiles.json")
+    tuner.save(path)
+    t2 = A.ConvTuner(path=path)
+    assert t2.table == tuner.table
