@@ -337,6 +337,7 @@ def train_bench(a):
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     it_s = a.steps / sec
+    stepping = batched_stepping(a, sur, batches[0][:3], G, total, dev)
     # network flop per voxel: forward 2*27*(4*32 + 4*32*32 + 32*3); backward data and weight
     # gradients twice that (the first layer's data gradient is not needed)
     fwd = 2 * 27 * (4 * 32 + 4 * 32 * 32 + 32 * 3)
@@ -355,9 +356,51 @@ def train_bench(a):
         "eager_step_kernels_ms": {k: v[0] for k, v in timing.items()}, "eager_step_conv_ms": conv_ms,
         "conv_tiles": {"autotune_seconds": t_tune, "chosen": {k: v["best"] for k, v in tuner.report().items()}},
         "loss_first": float(loss0), "loss_last": float(losses[-1]),
+        "batched_stepping": stepping,
         "peak_device_bytes": torch.cuda.max_memory_allocated(),
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_stepping(a, sur, batch, G, total, dev):
+    """The batch's scenes stepped together: `a.steps` KDK steps of all of them
+    (ragged deposit, batched FFT solve, optionally the trained net at batch B,
+    one persistent gather/kick/drift launch per step) captured in one HIP
+    graph, replayed once to warm up and once timed; plain PM and surrogate."""
+    import torch
+    from augur_amd import _lib
+    from augur_amd.batch import RaggedStepper
+    pos, starts, mass = batch
+    out = {}
+    for name, s in (("plain_pm", None), ("surrogate", sur)):
+        st = RaggedStepper(G, device=dev, surrogate=s)
+        _, P, M = st.sort(pos, starts, mass)   # each scene in cell order (set-up, not timed)
+        V = torch.zeros_like(P)
+        mass = M
+        A = st.accel(P, starts, mass)
+        graph = st.capture(P, V, A, starts, mass, a.dt, a.steps)
+        graph.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graph.replay()
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        # per-kernel breakdown from one eager call of the same steps
+        _lib.timing_enable(True)
+        st.kdk(P, V, A, starts, mass, a.dt, a.steps)
+        torch.cuda.synchronize()
+        tm = _lib.timing_collect()
+        _lib.timing_enable(False)
+        out[name] = {"particle_steps_per_s": total * a.steps / sec, "ms_per_step": sec * 1e3 / a.steps,
+                     "steps": a.steps, "dt": a.dt,
+                     "step_ragged_us_avg": tm["step_ragged"][0] / tm["step_ragged"][1] * 1e3
+                     if "step_ragged" in tm else None,
+                     "eager_kernels_ms": {k: v[0] for k, v in tm.items()}}
+        print(f"[bench] batched {name}: {a.steps} steps of {total} particles in {sec * 1e3:.1f} ms -> "
+              f"{out[name]['particle_steps_per_s']:.3e} particle-steps/s", file=sys.stderr, flush=True)
+        del graph, P, V, A
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():

@@ -261,6 +261,25 @@ int augur_pm_step_end(float* pos, float* vel, float* acc, int64_t n, int64_t G, 
                       size_t ws_bytes, void* stream);
 int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* force_offset);
 
+/* One launch over a ragged batch of B scenes (B <= 1024): scene b holds
+ * particles [starts[b], starts[b+1]) of pos/vel/acc (FP32 (total, 3), starts
+ * an int64 device array of B+1 nondecreasing offsets, starts[B] = total), and
+ * its force field is force[b], three FP32 planes a_x, a_y, a_z of G^3 cells
+ * ((B, 3, G, G, G), the batched PM solve's layout).  Periodic box [0, L)^3,
+ * G a power of two.  The KDK step splits around the force evaluation:
+ *   phase 0 (GATHER):     acc = F(x)
+ *   phase 1 (OPEN):       v += dt/2 acc; x += dt v            (force unused)
+ *   phase 2 (CLOSE):      a = F(x); v += dt/2 a; acc = a
+ *   phase 3 (CLOSE_OPEN): a = F(x); v += 2 x dt/2 a; x += dt v (acc not stored)
+ * so k steps are OPEN, then (k - 1) x [fields; CLOSE_OPEN], then [fields;
+ * CLOSE].  Per particle the arithmetic is the single-scene stepper's (each
+ * kick and drift one FMA), so batching does not change a scene's bits.  No
+ * host synchronisation (capturable).  Replaces the padded, masked (B, N, D)
+ * batch stepping of ScatterField._step / rollout (scripts/scatter_field.py:
+ * 677-688, 745) for scenes of uneven particle counts. */
+int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B, int64_t total,
+                          const float* force, int64_t G, float L, float dt, int phase, void* stream);
+
 /* ------------------------------------------ a11 / a12 surrogate network */
 
 /* BF16 MFMA 3x3x3 periodic convolution, FP32 accumulation, on a G^3 grid

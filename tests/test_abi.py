@@ -37,6 +37,13 @@ def test_workspace_queries_are_host_only(lib):
     ("augur_conv3_bf16_fwd", (None, 16, None, None, 1, 48, 0, 3, None, None, None, None, None), "power of two"),
     ("augur_poisson_green_grad", (None, 1, 1024, 1.0, 1.0, 1.0, None, None, None), "power-of-two"),
     ("augur_direct_sum_f64", (None, None, None, 10, 3, None, 4, 0.1, None, None, 0, None), "exactly one"),
+    ("augur_persistent_step", (1, 1, 1, 1, 0, 10, 1, 32, 1.0, 0.1, 0, None), "B must be"),
+    ("augur_persistent_step", (1, 1, 1, 1, 2, 10, 1, 48, 1.0, 0.1, 0, None), "power of two"),
+    ("augur_persistent_step", (1, 1, 1, 1, 2, 10, 1, 32, 1.0, 0.1, 7, None), "unknown phase"),
+    ("augur_persistent_step", (1, None, 1, 1, 2, 10, 1, 32, 1.0, 0.1, 3, None), "vel is required"),
+    ("augur_persistent_step", (1, 1, 1, 1, 2, 10, None, 32, 1.0, 0.1, 2, None), "force is required"),
+    ("augur_conv3_bf16_fwd_tile", (1, 32, 1, 1, 1, 64, 1, 32, None, 1, None, None, 1, 24, None), "zc must"),
+    ("augur_conv3_bf16_fwd_tile", (1, 16, 1, 1, 1, 64, 1, 32, None, 1, None, None, 2, 16, None), "tile variant"),
 ])
 def test_host_validation_errors(lib, name, args, needle):
     with pytest.raises(_lib.AugurError, match=needle):

@@ -103,3 +103,169 @@ def test_batch_graph_replays_match_eager(dev):
         assert torch.equal(l1, l2), k
         for w1, w2 in zip(s1.net.w + s1.net.b, s2.net.w + s2.net.b):
             assert torch.equal(w1, w2), k
+
+
+# ------------------------------------------------- batched ragged stepping
+def _ragged_state(sizes, seed, dev):
+    rng = np.random.default_rng(seed)
+    total = int(sum(sizes))
+    pos = torch.from_numpy(rng.random((total, 3), dtype=np.float32)).to(dev)
+    vel = torch.from_numpy((0.05 * rng.standard_normal((total, 3))).astype(np.float32)).to(dev)
+    acc = torch.from_numpy(rng.standard_normal((total, 3)).astype(np.float32)).to(dev)
+    starts = torch.tensor([0] + list(np.cumsum(sizes)), dtype=torch.int64, device=dev)
+    return pos, vel, acc, starts
+
+
+def test_persistent_step_matches_single_scene_launches(dev):
+    """Every phase of the ragged launch gives each scene the bits it gets
+    stepped alone (empty and one-particle scenes included); the gather equals
+    the generic CIC gather of the scene's own field, the kicks and drift the
+    KDK formulas."""
+    from augur_amd import batch as RB
+    from augur_amd import ops
+    G, L, dt = 32, 1.0, 2e-3
+    sizes = [1000, 0, 3001, 1, 2500, 0]
+    pos, vel, acc0, starts = _ragged_state(sizes, 11, dev)
+    g = torch.Generator(device=dev).manual_seed(1)
+    force = torch.randn(len(sizes), 3, G, G, G, device=dev, generator=g)
+    out = {}
+    for phase in (RB.PHASE_GATHER, RB.PHASE_OPEN, RB.PHASE_CLOSE, RB.PHASE_CLOSE_OPEN):
+        p, v, a = pos.clone(), vel.clone(), acc0.clone()
+        RB.persistent_step(p, v, a, starts, force, G, L, dt, phase)
+        out[phase] = (p, v, a)
+        lo = 0
+        for b, n in enumerate(sizes):
+            sl = slice(lo, lo + n)
+            lo += n
+            if n == 0:
+                continue
+            pb, vb, ab = pos[sl].clone(), vel[sl].clone(), acc0[sl].clone()
+            RB.persistent_step(pb, vb, ab, torch.tensor([0, n], dtype=torch.int64, device=dev),
+                               force[b:b + 1].contiguous(), G, L, dt, phase)
+            assert torch.equal(p[sl], pb) and torch.equal(v[sl], vb) and torch.equal(a[sl], ab), (phase, b)
+    # the gather is each scene's CIC gather of its own field
+    a = out[RB.PHASE_GATHER][2]
+    lo = 0
+    for b, n in enumerate(sizes):
+        sl = slice(lo, lo + n)
+        lo += n
+        if n:
+            ref = ops.cic_gather(force[b:b + 1], pos[sl][None].contiguous(), 1.0 / G, 0.0, True)[0]
+            assert (a[sl] - ref).abs().max() <= 1e-6 * ref.abs().max()
+    a_g = a.double()
+    hdt = 0.5 * dt
+    # OPEN: v + dt/2 acc, then x + dt v wrapped; CLOSE: v + dt/2 a, acc = a
+    p1, v1, _ = out[RB.PHASE_OPEN]
+    v_exp = vel.double() + hdt * acc0.double()
+    assert (v1.double() - v_exp).abs().max() < 1e-6
+    dx = ((p1.double() - pos.double() - dt * v1.double()) + 0.5) % 1.0 - 0.5
+    assert dx.abs().max() < 1e-6 and bool(((p1 >= 0) & (p1 < 1)).all())
+    p2, v2, a2 = out[RB.PHASE_CLOSE]
+    assert torch.equal(p2, pos) and torch.equal(a2, a)
+    assert (v2.double() - (vel.double() + hdt * a_g)).abs().max() < 1e-6
+    # CLOSE_OPEN = CLOSE followed by OPEN, bit for bit
+    p3, v3, _ = out[RB.PHASE_CLOSE_OPEN]
+    pc, vc, ac = p2.clone(), v2.clone(), a2.clone()
+    RB.persistent_step(pc, vc, ac, starts, None, G, L, dt, RB.PHASE_OPEN)
+    assert torch.equal(pc, p3) and torch.equal(vc, v3)
+
+
+def test_ragged_kdk_matches_oracle_per_scene(dev):
+    """A KDK rollout of three scenes of uneven sizes stepped as one batch
+    against the FP64 oracle's PM rollout of each scene alone."""
+    from augur_amd import batch as RB
+    from augur_amd.surrogate import ragged_batch
+    from oracle import pm as opm
+    G, dt, steps = 32, 1e-3, 3
+    sizes = [3000, 700, 5000]
+    sc = scenes_np(7, sizes)
+    rng = np.random.default_rng(8)
+    vels = [(0.02 * rng.standard_normal((n, 3))).astype(np.float32) for n in sizes]
+    pos, starts, mass = ragged_batch(sc, dev)
+    vel = torch.from_numpy(np.concatenate(vels)).to(dev)
+    st = RB.RaggedStepper(G, device=dev)
+    acc = st.accel(pos, starts, mass)
+    lo = 0
+    for p, n in zip(sc, sizes):
+        ref = opm.pm_acc_3d_periodic(p, 1.0 / n, G)
+        a = acc[lo: lo + n].cpu().numpy()
+        assert np.sqrt(((a - ref) ** 2).sum() / (ref ** 2).sum()) < 1e-5
+        lo += n
+    st.kdk(pos, vel, acc, starts, mass, dt, steps)
+    lo = 0
+    for p, v, n in zip(sc, vels, sizes):
+        rp, rv = opm.kdk_pm_rollout(p, v, 1.0 / n, G, dt, steps)
+        dp = np.abs(((pos[lo: lo + n].cpu().numpy() - rp) + 0.5) % 1.0 - 0.5).max()
+        dv = np.abs(vel[lo: lo + n].cpu().numpy() - rv).max() / np.abs(rv).max()
+        assert dp < 1e-5 and dv < 1e-4, (n, dp, dv)
+        lo += n
+
+
+@pytest.mark.parametrize("with_net", [False, True])
+def test_ragged_kdk_graph_replays_match_eager(dev, with_net):
+    """A 2-step call of the batched stepper (plain PM, or PM with the
+    correction net at batch B) captured once in a HIP graph: three replays,
+    each from a new state with eager work between, equal eager calls bit for
+    bit; the batched surrogate field of a scene matches the scene alone."""
+    from augur_amd import batch as RB
+    from augur_amd.surrogate import PMSurrogate, ragged_batch
+    G, dt = 32, 1e-3
+    sizes = [2000, 333, 4100, 64]
+    sc = scenes_np(9, sizes)
+    pos, starts, mass = ragged_batch(sc, dev)
+    sur = None
+    if with_net:
+        sur = PMSurrogate(G, 1.0, 1.0, dev, mode="correction", init_std=0.05)
+        rho, force = RB.RaggedStepper(G, device=dev).grids(pos, starts, mass)
+        sur.calibrate(rho[0], force[0])
+    st = RB.RaggedStepper(G, device=dev, surrogate=sur)
+    rng = np.random.default_rng(10)
+    vel = torch.from_numpy((0.02 * rng.standard_normal((pos.shape[0], 3))).astype(np.float32)).to(dev)
+    acc = st.accel(pos, starts, mass)
+    if with_net:   # scene 2 of the batch against the same scene alone
+        f_b = st.force_field(pos, starts, mass)[2]
+        f_1, _ = sur.field(torch.from_numpy(sc[2]).to(dev), mass_value=1.0 / sizes[2])
+        assert float((f_b - f_1).pow(2).sum().sqrt() / f_1.pow(2).sum().sqrt()) < 1e-3
+    P, V, A = pos.clone(), vel.clone(), acc.clone()
+    graph = st.capture(P, V, A, starts, mass, dt, 2)
+    state = (pos, vel, acc)
+    for r in range(3):
+        P.copy_(state[0])
+        V.copy_(state[1])
+        A.copy_(state[2])
+        graph.replay()
+        torch.cuda.synchronize()
+        p, v, a = (t.clone() for t in state)
+        st.kdk(p, v, a, starts, mass, dt, 2)
+        torch.cuda.synchronize()
+        assert torch.equal(P, p) and torch.equal(V, v) and torch.equal(A, a), r
+        state = (p, v, a)
+
+
+def test_ragged_sort_keeps_scenes_and_bits(dev):
+    """Sorting a ragged batch into cell order permutes particles only within
+    their scenes, and stepping the sorted batch gives every particle the bits
+    it gets unsorted (the fixed-point deposit does not depend on order)."""
+    from augur_amd import batch as RB
+    from augur_amd.surrogate import ragged_batch
+    G, dt = 32, 1e-3
+    sizes = [2000, 333, 4100]
+    sc = scenes_np(12, sizes)
+    pos, starts, mass = ragged_batch(sc, dev)
+    rng = np.random.default_rng(13)
+    vel = torch.from_numpy((0.02 * rng.standard_normal((pos.shape[0], 3))).astype(np.float32)).to(dev)
+    st = RB.RaggedStepper(G, device=dev)
+    perm, P, V, M = st.sort(pos, starts, vel, mass)
+    n = pos.shape[0]
+    assert torch.equal(torch.sort(perm).values, torch.arange(n, device=dev))
+    scene = torch.searchsorted(starts[1:].contiguous(), torch.arange(n, device=dev), right=True)
+    assert torch.equal(scene[perm], scene)
+    cell = (P.double() * G).long().clamp(0, G - 1)
+    key = ((scene * G + cell[:, 2]) * G + cell[:, 1]) * G + cell[:, 0]
+    assert bool((key[1:] >= key[:-1]).all())
+    A0 = st.accel(pos, starts, mass)
+    A1 = st.accel(P, starts, M)
+    assert torch.equal(A1, A0[perm])
+    st.kdk(pos, vel, A0, starts, mass, dt, 2)
+    st.kdk(P, V, A1, starts, M, dt, 2)
+    assert torch.equal(P, pos[perm]) and torch.equal(V, vel[perm]) and torch.equal(A1, A0[perm])
