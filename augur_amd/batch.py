@@ -96,9 +96,10 @@ class RaggedStepper:
     # ------------------------------------------------------------ ordering
     def sort(self, pos, starts, *arrays):
         """Permutation putting every scene's particles in cell order (scene
-        ranges unchanged): a stable radix sort by scene x G^3 + cell, so that
-        the deposit's and the gather's accesses of consecutive particles fall
-        in nearby cells.  Returns (perm, pos[perm], *(a[perm] for a in
+        ranges unchanged): a stable radix sort by scene x G^3 + CIC base cell
+        (floor(x / h - 1/2), wrapped), so that consecutive particles share
+        their eight corners (the deposit merges such runs within a wave before
+        its atomics) and the gather's accesses fall in nearby cells.  Returns (perm, pos[perm], *(a[perm] for a in
         arrays)); None arrays stay None.  Order only affects speed: the
         fixed-point deposit is order-independent and each particle's step is
         its own."""
@@ -107,7 +108,7 @@ class RaggedStepper:
         B = starts.shape[0] - 1
         idx = torch.arange(total, dtype=torch.int64, device=pos.device)
         scene = torch.searchsorted(starts[1:].contiguous(), idx, right=True)
-        c = (pos.double() * (G / self.L)).long().clamp_(0, G - 1)
+        c = torch.floor(pos.double() * (G / self.L) - 0.5).long().remainder_(G)
         keys = ((scene * G + c[:, 2]) * G + c[:, 1]) * G + c[:, 0]
         perm = ops.sort_by_cell(keys, B * G ** 3)
         return (perm, pos[perm].contiguous()) + tuple(None if a is None else a[perm].contiguous() for a in arrays)

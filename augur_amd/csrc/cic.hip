@@ -430,8 +430,41 @@ __device__ __forceinline__ int64_t scene_of(const int64_t* __restrict__ starts, 
   return lo;
 }
 
+// acc[key] += q for every active lane of the wave, with runs of equal keys in
+// adjacent lanes summed first (a segmented scan over the wave) so that only
+// the run's last lane issues the atomic.  Cell-sorted particles put a dense
+// cell's visits in adjacent lanes, where same-address atomics would
+// serialise; the integer sums are exact, so merging changes no bits.  Every
+// lane of the wave must call it (inactive lanes with active = false).
+__device__ __forceinline__ void wave_merged_add(unsigned long long* __restrict__ acc, int64_t key, long long q,
+                                                bool active) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t k = active ? key : (int64_t)(-1 - lane);  // inactive lanes match nothing
+  const int64_t kprev = __shfl_up(k, 1);
+  const bool head = lane == 0 || kprev != k;
+  if (__ballot(head) == ~0ull) {  // no runs in this wave: one atomic per lane
+    if (active && q) atomicAdd(&acc[key], (unsigned long long)q);
+    return;
+  }
+  long long v = active ? q : 0;
+  int f = head ? 1 : 0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const long long vo = __shfl_up(v, off);
+    const int fo = __shfl_up(f, off);
+    if (lane >= off && !f) {
+      v += vo;
+      f |= fo;
+    }
+  }
+  const int64_t knext = __shfl_down(k, 1);
+  const bool tail = lane == 63 || knext != k;
+  if (active && tail && v) atomicAdd(&acc[key], (unsigned long long)v);
+}
+
 // starts (may be NULL): ragged batch, scene b holds particles [starts[b], starts[b+1]);
-// otherwise B scenes of N particles each
+// otherwise B scenes of N particles each.  The loop runs in whole waves (a
+// lane past the end is inactive) so that wave_merged_add sees every lane.
 template <int D>
 __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos, const float* __restrict__ vel,
                                                    const float* __restrict__ mass, float mass_value,
@@ -444,7 +477,11 @@ __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos
   for (int c = 0; c < C; ++c) E[c] = fixed_exponent(maxbits[c], tot);
   int64_t GD = 1;
   for (int d = 0; d < D; ++d) GD *= G;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < tot; base += stride) {
+    const int64_t ii = base + threadIdx.x;
+    const bool live = ii < tot;
+    const int64_t i = live ? ii : tot - 1;  // a dead lane computes on a valid row and adds nothing
     const int64_t b = starts ? scene_of(starts, B, i) : i / N;
     float mm = mass ? mass[i] : mass_value;
     if (mask) mm = __fmul_rn(mm, mask[i]);
@@ -462,7 +499,7 @@ __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos
 #pragma unroll
     for (int o = 0; o < (1 << D); ++o) {
       float wt = 1.0f;
-      bool ok = true;
+      bool ok = live;
       int64_t flat = 0, mul = 1;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -478,11 +515,10 @@ __global__ __launch_bounds__(256) void k_dep_fixed(const float* __restrict__ pos
         flat += (int64_t)ix * mul;
         mul *= G;
       }
-      if (!ok) continue;
       for (int c = 0; c < C; ++c) {
         // rint(val * wt * 2^e) with the exact power-of-two scaling applied to val
         const long long q = fixed_round(__fmul_rn(valE[c], wt), 0);
-        if (q) atomicAdd(&acc[((b * C) + c) * GD + flat], (unsigned long long)q);
+        wave_merged_add(acc, ((b * C) + c) * GD + (ok ? flat : 0), q, ok);
       }
     }
   }

@@ -260,7 +260,7 @@ def test_ragged_sort_keeps_scenes_and_bits(dev):
     assert torch.equal(torch.sort(perm).values, torch.arange(n, device=dev))
     scene = torch.searchsorted(starts[1:].contiguous(), torch.arange(n, device=dev), right=True)
     assert torch.equal(scene[perm], scene)
-    cell = (P.double() * G).long().clamp(0, G - 1)
+    cell = torch.floor(P.double() * G - 0.5).long().remainder(G)
     key = ((scene * G + cell[:, 2]) * G + cell[:, 1]) * G + cell[:, 0]
     assert bool((key[1:] >= key[:-1]).all())
     A0 = st.accel(pos, starts, mass)
