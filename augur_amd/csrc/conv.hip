@@ -8,6 +8,8 @@
 // weight tensor are staged in LDS, and each wave runs
 // v_mfma_f32_32x32x16_bf16 over 27 taps x CIN/16 K-steps for two 32-voxel
 // rows, sharing each weight fragment between its two rows.
+#include <type_traits>
+
 #include "internal.hpp"
 
 using namespace augur;
@@ -542,26 +544,35 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
 
 // ------------------------------------------------------------ weight grad
 // dW[tap][co][ci] = sum_v dz[v][co] * x[v + o_tap][ci]; db[co] = sum_v dz[v][co].
-// Persistent workgroups walk tiles in a fixed order and write one partial per
-// workgroup; k_sum_partials adds them in workgroup order (deterministic).
-// In LDS the tile is stored channel-major so that eight consecutive voxels of
-// one channel form a 16-byte MFMA fragment: A = dz^T (co x voxels),
-// B = shifted x (voxels x ci).  Wave w accumulates taps w, w+4, ...
+// Persistent workgroups walk 32x4x2-voxel tiles in a fixed order and write
+// one partial per workgroup; k_sum_partials adds them in workgroup order
+// (deterministic).  In LDS the tiles are channel-major, so eight consecutive
+// voxels of one channel form a 16-byte MFMA fragment: A = dz^T (co x voxels),
+// B = x shifted by the tap (voxels x ci).  x is staged twice, the second copy
+// one voxel to the left, so the B fragment of a dx = 0 or dx = 1 tap is one
+// aligned 16-byte read and that of a dx = 2 tap four 4-byte reads.  Channel
+// strides are padded to 36 (mod 64) banks, so the 16 lanes of a 16-byte read
+// hit distinct banks.  Wave w accumulates taps w, w+4, ... (compile-time
+// tap lists per wave).
 constexpr int DW_TAPS_PER_WAVE = 7;  // ceil(27 / 4)
-constexpr int XS = HX + 6;           // x stride of the channel-major input tile (padded)
-constexpr int DS = CT_X + 8;         // x stride of the channel-major dz tile (padded)
+constexpr int XS = HX + 6;           // x stride of a row of the channel-major input copies (40)
+constexpr int DS = CT_X + 8;         // x stride of a row of the channel-major dz tile (40)
+constexpr int XCS = HZ * HY * XS + 8;    // per-channel stride of an input copy (968 = 484 dwords)
+constexpr int DCS = CT_Z * CT_Y * DS + 8;  // per-channel stride of the dz tile (328 = 164 dwords)
+static_assert((XCS / 2) % 64 == 36 && (DCS / 2) % 64 == 36, "conflict-free 16-byte fragment reads");
+static_assert(HX % 2 == 0, "x halo rows load in voxel pairs");
 
 template <int CIN>
 __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __restrict__ x,
                                                               const uint16_t* __restrict__ dz, int B, int G,
                                                               int64_t ntiles, float* __restrict__ part) {
   constexpr int NB = CIN / 32 > 0 ? CIN / 32 : 1;
-  constexpr int XCH = HZ * HY * HX * (CIN / 8);           // 16-byte chunks of the x tile (with halo)
+  constexpr int XCH = HZ * HY * (HX / 2) * (CIN / 8);      // voxel pairs x 16-byte chunks of the x tile
   constexpr int XPER = (XCH + CONV_THREADS - 1) / CONV_THREADS;
   constexpr int DCH = CT_Z * CT_Y * CT_X * 4;              // 16-byte chunks of the dz tile
   constexpr int DPER = DCH / CONV_THREADS;
-  __shared__ __attribute__((aligned(16))) uint16_t xl[CIN * HZ * HY * XS];
-  __shared__ __attribute__((aligned(16))) uint16_t dl[32 * CT_Z * CT_Y * DS];
+  __shared__ __attribute__((aligned(16))) uint16_t xl[2][CIN * XCS];  // [0]: voxel tx at tx; [1]: at tx - 1
+  __shared__ __attribute__((aligned(16))) uint16_t dl[32 * DCS];
   __shared__ float bred[4][32];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i = lane & 31, kh = lane >> 5;
@@ -584,8 +595,9 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
   for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
 
   // the next tile's chunks travel global -> registers (in flight during the
-  // current tile's MFMAs) -> LDS, transposed to channel-major
-  u32x4 rx[XPER], rd[DPER];
+  // current tile's MFMAs) -> LDS, transposed to channel-major; x in pairs of
+  // voxels adjacent in x, so the unshifted copy takes 4-byte stores
+  u32x4 rx[XPER][2], rd[DPER];
   auto load_tile = [&](int64_t tile) {
     int64_t t = tile;
     const int bx = (int)(t % ntx);
@@ -601,12 +613,17 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
       if (c < XCH) {
         const int ch = c % (CIN / 8);
         int v = c / (CIN / 8);
-        const int tx = v % HX;
-        v /= HX;
+        const int tp = v % (HX / 2);
+        v /= HX / 2;
         const int ty = v % HY;
         const int tz = v / HY;
-        const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1), gz = (z0 + tz - 1) & (G - 1);
-        rx[k] = *reinterpret_cast<const u32x4*>(&x[((int64_t)b * G3 + ((int64_t)gz * G + gy) * G + gx) * CIN + ch * 8]);
+        const int gy = (y0 + ty - 1) & (G - 1), gz = (z0 + tz - 1) & (G - 1);
+        const int64_t row = (int64_t)b * G3 + ((int64_t)gz * G + gy) * G;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int gx = (x0 + 2 * tp + u - 1) & (G - 1);
+          rx[k][u] = *reinterpret_cast<const u32x4*>(&x[(row + gx) * CIN + ch * 8]);
+        }
       }
     }
 #pragma unroll
@@ -629,14 +646,24 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
       if (c < XCH) {
         const int ch = c % (CIN / 8);
         int v = c / (CIN / 8);
-        const int tx = v % HX;
-        v /= HX;
+        const int tp = v % (HX / 2);
+        v /= HX / 2;
         const int ty = v % HY;
         const int tz = v / HY;
+        const int o = (tz * HY + ty) * XS + 2 * tp;  // voxel 2 tp of the row (4-byte aligned)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          xl[(((ch * 8 + 2 * e) * HZ + tz) * HY + ty) * XS + tx] = (uint16_t)(rx[k][e] & 0xffffu);
-          xl[(((ch * 8 + 2 * e + 1) * HZ + tz) * HY + ty) * XS + tx] = (uint16_t)(rx[k][e] >> 16);
+          const uint32_t a0 = rx[k][0][e], a1 = rx[k][1][e];  // channels 2e (low half), 2e + 1 (high half)
+          const int c0 = (ch * 8 + 2 * e) * XCS, c1 = c0 + XCS;
+          *reinterpret_cast<uint32_t*>(&xl[0][c0 + o]) = (a0 & 0xffffu) | (a1 << 16);
+          *reinterpret_cast<uint32_t*>(&xl[0][c1 + o]) = (a0 >> 16) | (a1 & 0xffff0000u);
+          // copy 1 holds voxel tx at tx - 1: voxel 2 tp + 1 at 2 tp, voxel 2 tp at 2 tp - 1
+          xl[1][c0 + o] = (uint16_t)(a1 & 0xffffu);
+          xl[1][c1 + o] = (uint16_t)(a1 >> 16);
+          if (tp > 0) {
+            xl[1][c0 + o - 1] = (uint16_t)(a0 & 0xffffu);
+            xl[1][c1 + o - 1] = (uint16_t)(a0 >> 16);
+          }
         }
       }
     }
@@ -651,8 +678,49 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
       const int tz = v / CT_Y;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        dl[(((ch * 8 + 2 * e) * CT_Z + tz) * CT_Y + ty) * DS + tx] = (uint16_t)(rd[k][e] & 0xffffu);
-        dl[(((ch * 8 + 2 * e + 1) * CT_Z + tz) * CT_Y + ty) * DS + tx] = (uint16_t)(rd[k][e] >> 16);
+        dl[(ch * 8 + 2 * e) * DCS + (tz * CT_Y + ty) * DS + tx] = (uint16_t)(rd[k][e] & 0xffffu);
+        dl[(ch * 8 + 2 * e + 1) * DCS + (tz * CT_Y + ty) * DS + tx] = (uint16_t)(rd[k][e] >> 16);
+      }
+    }
+  };
+
+  // K loop of one tile for wave WV: 8 rows x 32 voxels = 16 steps of 16
+  // voxels (two per row); each step's dz fragment feeds the wave's 7 taps,
+  // 7 independent MFMA chains
+  auto mfma_tile = [&](auto wvc) __attribute__((always_inline)) {
+    constexpr int WV = decltype(wvc)::value;
+#pragma unroll 2
+    for (int st = 0; st < 2 * CT_Z * CT_Y; ++st) {
+      const int r = st >> 1, half = st & 1;
+      const int rz = r / CT_Y, ry = r % CT_Y;
+      const int vb = half * 16 + kh * 8;  // first of this lane's 8 voxels
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&dl[i * DCS + (rz * CT_Y + ry) * DS + vb]);
+      if ((st & 3) == WV) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ones, accb, 0, 0, 0);
+#pragma unroll
+      for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+        const int tap = WV + 4 * k;
+        if (tap < 27) {
+          const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
+          const int row = ((rz + dzo) * HY + ry + dyo) * XS + vb;
+#pragma unroll
+          for (int n = 0; n < NB; ++n) {
+            const int ci = n * 32 + i;
+            bf16x8 bv;
+            if (ci < CIN) {
+              if (dxo == 2) {  // voxels vb + 2 .. vb + 9 of the unshifted copy: 4-byte aligned
+                const uint32_t* p = reinterpret_cast<const uint32_t*>(&xl[0][ci * XCS + row + 2]);
+                const u32x4 u = {p[0], p[1], p[2], p[3]};
+                bv = __builtin_bit_cast(bf16x8, u);
+              } else {
+                bv = *reinterpret_cast<const bf16x8*>(&xl[dxo][ci * XCS + row]);
+              }
+            } else {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) bv[e] = (__bf16)0.0f;
+            }
+            acc[k][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[k][n], 0, 0, 0);
+          }
+        }
       }
     }
   };
@@ -664,36 +732,11 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
     store_tile();
     __syncthreads();
     if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
-    // K loop: 8 rows x 32 voxels = 16 steps of 16 voxels (two per row); each
-    // step's dz fragment feeds the wave's 7 taps, 7 independent MFMA chains
-#pragma unroll 2
-    for (int st = 0; st < 2 * CT_Z * CT_Y; ++st) {
-      const int r = st >> 1, half = st & 1;
-      const int rz = r / CT_Y, ry = r % CT_Y;
-      const int vb = half * 16 + kh * 8;  // first of this lane's 8 voxels
-      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&dl[((i * CT_Z + rz) * CT_Y + ry) * DS + vb]);
-      if ((st & 3) == wv) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ones, accb, 0, 0, 0);
-#pragma unroll
-      for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
-        const int tap = wv + 4 * k;
-        if (tap < 27) {
-          const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
-#pragma unroll
-          for (int n = 0; n < NB; ++n) {
-            const int ci = n * 32 + i;
-            bf16x8 bv;
-            if (ci < CIN) {
-              const uint16_t* bp = &xl[((ci * HZ + rz + dzo) * HY + ry + dyo) * XS + vb + dxo];
-#pragma unroll
-              for (int e = 0; e < 8; ++e) bv[e] = __builtin_bit_cast(__bf16, bp[e]);
-            } else {
-#pragma unroll
-              for (int e = 0; e < 8; ++e) bv[e] = (__bf16)0.0f;
-            }
-            acc[k][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[k][n], 0, 0, 0);
-          }
-        }
-      }
+    switch (wv) {  // wave-uniform: each wave's tap list is compile-time
+      case 0: mfma_tile(std::integral_constant<int, 0>{}); break;
+      case 1: mfma_tile(std::integral_constant<int, 1>{}); break;
+      case 2: mfma_tile(std::integral_constant<int, 2>{}); break;
+      default: mfma_tile(std::integral_constant<int, 3>{}); break;
     }
   }
   // bias partial of the wave: accb[q] holds (co = 8 (q/4) + 4 kh + q%4, any column): lanes 0 and 32
