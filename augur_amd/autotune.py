@@ -91,7 +91,7 @@ class ConvTuner:
         return None if t is None else int(t)
 
     # ------------------------------------------------------------ tuning
-    def tune_fwd(self, x, w, bias, mode, cout, z_out=None, addend=None):
+    def tune_fwd(self, x, w, bias, mode, cout, z_out=None, addend=None, z_in=None):
         """Time every tile for this layer on these tensors; returns the best (family, zc)."""
         from .surrogate import conv3_fwd
         B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
@@ -101,8 +101,8 @@ class ConvTuner:
         stream = torch.cuda.current_stream(x.device)
         res = {}
         for tile in candidates(cin, mode, cout, G):
-            res[tile] = _time(lambda: conv3_fwd(x, w, bias, mode, cout, z_out=z_out, addend=addend, tile=tile),
-                              self.reps, stream)
+            res[tile] = _time(lambda: conv3_fwd(x, w, bias, mode, cout, z_out=z_out, addend=addend, tile=tile,
+                                                z_in=z_in), self.reps, stream)
         best = min(res, key=res.get)
         self.table[key] = list(best)
         self.timings[key] = {f"{FAMILY[v]}/zc{zc}": ms for (v, zc), ms in res.items()}

@@ -58,6 +58,8 @@ enum ConvMode : int {
   CONV_GELU = 1,         // z = conv + bias -> z_out; h_out = gelu(z)            (BF16 channels-last)
   CONV_RES_GELU = 2,     // z -> z_out; h_out = x_center + gelu(z)               (BF16, CIN = 32)
   CONV_OUT_F32_CL = 3,   // y[vox][co] = conv (+ addend[vox][co])                (FP32 channels-last, no bias)
+  CONV_OUT_F32_CL_DGELU = 4,  // as 3 (32 -> 32), and dz[vox][co] = BF16(y * gelu'(z_in[vox][co])): the next
+                              // layer's GELU backward fused into this data gradient (z_out: z_in, h_out: dz)
 };
 
 // ------------------------------------------------------- z-marching forward
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
   const int lane = tid & 63, wv = tid >> 6;
   const int half = wv >> 2, wrow = wv & 3;  // output channels 16 half .. + 15, rows 4 wrow .. + 3
   const int n16 = lane & 15, kq = lane >> 4;
-  if (tid < 32) sbias[tid] = MODE == CONV_OUT_F32_CL ? 0.0f : bias[tid];
+  if (tid < 32) sbias[tid] = (MODE == CONV_OUT_F32_CL || MODE == CONV_OUT_F32_CL_DGELU) ? 0.0f : bias[tid];
 
   // A operand: w[tap][co = 16 half + n16][ci = 8 kq .. 8 kq + 7]
   bf16x8 wr[27];
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
         const int64_t o = ((int64_t)b * G3 + ((int64_t)z * G + y0 + ly) * G + x0 + lx) * 32 + co0;
         const float v0 = acc[r][hf][0] + bj.x, v1 = acc[r][hf][1] + bj.y, v2 = acc[r][hf][2] + bj.z,
                     v3 = acc[r][hf][3] + bj.w;
-        if (MODE == CONV_OUT_F32_CL) {  // FP32 channels-last, 32 channels (+ addend)
+        if (MODE == CONV_OUT_F32_CL || MODE == CONV_OUT_F32_CL_DGELU) {  // FP32 channels-last, 32 channels
           float4 val = {v0, v1, v2, v3};
           if (addend) {
             const float4 ad = *reinterpret_cast<const float4*>(&addend[o]);
@@ -512,6 +514,15 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
             val.w += ad.w;
           }
           *reinterpret_cast<float4*>(&y_out[o]) = val;
+          if (MODE == CONV_OUT_F32_CL_DGELU) {  // dz = y * gelu'(z_in), BF16 (k_gelu_bwd's arithmetic)
+            const uint2 zr = *reinterpret_cast<const uint2*>(&z_out[o]);
+            uint2 dp;
+            dp.x = (uint32_t)f2bf(val.x * gelu_grad(__uint_as_float(zr.x << 16))) |
+                   ((uint32_t)f2bf(val.y * gelu_grad(__uint_as_float(zr.x & 0xffff0000u))) << 16);
+            dp.y = (uint32_t)f2bf(val.z * gelu_grad(__uint_as_float(zr.y << 16))) |
+                   ((uint32_t)f2bf(val.w * gelu_grad(__uint_as_float(zr.y & 0xffff0000u))) << 16);
+            *reinterpret_cast<uint2*>(&h_out[o]) = dp;
+          }
           continue;
         }
         if (z_out) {
@@ -885,6 +896,7 @@ int conv_variant_mask(int cin, int mode, int cout) {
   if (cin == 32 && (mode == CONV_GELU || mode == CONV_RES_GELU || (mode == CONV_OUT_F32_CL && cout == 32)))
     m |= 1 << TILE_SPLIT;
   if (cin == 32 && cout <= 16 && (mode == CONV_OUT_F32_CF || mode == CONV_OUT_F32_CL)) m |= 1 << TILE_NARROW;
+  if (mode == CONV_OUT_F32_CL_DGELU) m = (cin == 32 && cout == 32) ? 1 << TILE_SPLIT : 0;  // split kernel only
   return m;
 }
 
@@ -915,15 +927,17 @@ int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t 
       return AUGUR_OK;
     }
   } else if (variant == TILE_ZM) {
-    if constexpr (CIN == 16) {  // 108 weight VGPRs: eight waves of 2 rows fit two waves per SIMD
-      hipLaunchKernelGGL((k_conv3_zm<CIN, MODE, 2>), dim3((unsigned)blocks), dim3(64 * (ZM_Y / 2)), 0, s, x, w,
-                         bias, (int)G, zc, cout, z, h, y, addend);
-    } else {
-      hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias,
-                         (int)G, zc, cout, z, h, y, addend);
+    if constexpr (MODE != CONV_OUT_F32_CL_DGELU) {
+      if constexpr (CIN == 16) {  // 108 weight VGPRs: eight waves of 2 rows fit two waves per SIMD
+        hipLaunchKernelGGL((k_conv3_zm<CIN, MODE, 2>), dim3((unsigned)blocks), dim3(64 * (ZM_Y / 2)), 0, s, x, w,
+                           bias, (int)G, zc, cout, z, h, y, addend);
+      } else {
+        hipLaunchKernelGGL((k_conv3_zm<CIN, MODE>), dim3((unsigned)blocks), dim3(ZM_THREADS), 0, s, x, w, bias,
+                           (int)G, zc, cout, z, h, y, addend);
+      }
+      AUGUR_LAUNCHED(name);
+      return AUGUR_OK;
     }
-    AUGUR_LAUNCHED(name);
-    return AUGUR_OK;
   }
   set_error("conv: tile variant %d has no kernel for cin %d, mode %d", variant, CIN, MODE);
   return AUGUR_EUNSUPPORTED;
@@ -934,7 +948,8 @@ int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t 
 extern "C" {
 
 int augur_conv3_tile_variants(int cin, int mode, int cout) {
-  if ((cin != 16 && cin != 32) || mode < CONV_OUT_F32_CF || mode > CONV_OUT_F32_CL || cout < 1 || cout > 32) return 0;
+  if ((cin != 16 && cin != 32) || mode < CONV_OUT_F32_CF || mode > CONV_OUT_F32_CL_DGELU || cout < 1 || cout > 32)
+    return 0;
   if (mode == CONV_RES_GELU && cin != 32) return 0;
   return conv_variant_mask(cin, mode, cout);
 }
@@ -947,6 +962,8 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
   AUGUR_REQUIRE(cin == 16 || cin == 32, AUGUR_EUNSUPPORTED, "conv: cin must be 16 or 32 (pad channels)");
   AUGUR_REQUIRE(cout >= 1 && cout <= 32, AUGUR_EINVAL, "conv: cout must be 1..32");
   AUGUR_REQUIRE(is_pow2(zc) && zc >= 2 && zc <= G, AUGUR_EINVAL, "conv: zc must be a power of two in [2, G]");
+  AUGUR_REQUIRE(mode != CONV_OUT_F32_CL_DGELU || (cin == 32 && cout == 32), AUGUR_EINVAL,
+                "conv: mode 4 needs y_out, z_out (read), h_out (dz), cin = cout = 32");
   const int mask = conv_variant_mask(cin, mode, cout);
   if (variant == TILE_DEFAULT) variant = conv_default_variant(mask);
   AUGUR_REQUIRE(variant >= TILE_ZM && variant <= TILE_NARROW && ((mask >> variant) & 1), AUGUR_EUNSUPPORTED,
@@ -975,6 +992,11 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
                                                          variant, zc, s, "conv_bwd_data")
                        : launch_fwd<32, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend,
                                                          variant, zc, s, "conv_bwd_data");
+    case CONV_OUT_F32_CL_DGELU:
+      AUGUR_REQUIRE(y_out && z_out && h_out && cin == 32 && cout == 32, AUGUR_EINVAL,
+                    "conv: mode 4 needs y_out, z_out (read), h_out (dz), cin = cout = 32");
+      return launch_fwd<32, CONV_OUT_F32_CL_DGELU>(x, w, nullptr, B, G, 32, z_out, h_out, y_out, addend, variant,
+                                                   zc, s, "conv_bwd_data");
     default:
       set_error("conv: unknown mode %d", mode);
       return AUGUR_EINVAL;

@@ -26,7 +26,7 @@ from .ops import _dev, _p, _stream, _ws
 
 BF16 = torch.bfloat16
 
-MODE_OUT_CF, MODE_GELU, MODE_RES_GELU, MODE_OUT_CL = 0, 1, 2, 3
+MODE_OUT_CF, MODE_GELU, MODE_RES_GELU, MODE_OUT_CL, MODE_OUT_CL_DGELU = 0, 1, 2, 3, 4
 
 
 def pad16(c):
@@ -52,10 +52,11 @@ def kernel_weights(w_taps, cinp):
     return out
 
 
-def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None):
+def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None, z_in=None):
     """x: BF16 channels-last (B, G, G, G, cin); w: BF16 (27, 32, cin); bias FP32 (32,).
     tile: (kernel family, z planes per workgroup) from the autotuner, or None
-    for the library's default tile."""
+    for the library's default tile.  Mode 4 (the data gradient fused with the
+    previous layer's GELU backward) reads z_in and returns (y, dz)."""
     dev = _dev(x)
     B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
     y = h = None
@@ -63,6 +64,11 @@ def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None):
         y = torch.empty(B, cout, G, G, G, dtype=torch.float32, device=dev)
     elif mode == MODE_OUT_CL:
         y = torch.empty(B, G, G, G, cout, dtype=torch.float32, device=dev)
+    elif mode == MODE_OUT_CL_DGELU:
+        assert z_in is not None and z_in.dtype == BF16 and z_in.shape == (B, G, G, G, 32)
+        y = torch.empty(B, G, G, G, 32, dtype=torch.float32, device=dev)
+        h = torch.empty(B, G, G, G, 32, dtype=BF16, device=dev)
+        z_out = z_in.contiguous()
     else:
         h = torch.empty(B, G, G, G, 32, dtype=BF16, device=dev)
     if tile is None:
@@ -71,6 +77,8 @@ def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None):
     else:
         _lib.call("augur_conv3_bf16_fwd_tile", _p(x), cin, _p(w), _p(bias), B, G, mode, cout, _p(z_out), _p(h),
                   _p(y), _p(addend), int(tile[0]), int(tile[1]), _stream(dev))
+    if mode == MODE_OUT_CL_DGELU:
+        return y, h
     return h if h is not None else y
 
 
@@ -182,14 +190,14 @@ class LocalNet3D:
         return [(from_taps(w), b[: w.shape[1]]) for w, b in zip(self.w, self.b)]
 
     # ------------------------------------------------------ tuned tiles
-    def _conv(self, x, w, b, mode, cout, z_out=None, addend=None):
+    def _conv(self, x, w, b, mode, cout, z_out=None, addend=None, z_in=None):
         tile = None
         if self.tuner is not None:
             if self._tuning:
-                tile = self.tuner.tune_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend)
+                tile = self.tuner.tune_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, z_in=z_in)
             else:
                 tile = self.tuner.fwd_tile(x.shape[-1], mode, cout, x.shape[0], x.shape[1], addend is not None)
-        return conv3_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, tile=tile)
+        return conv3_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, tile=tile, z_in=z_in)
 
     def _wgrad(self, x, dz):
         nwg = None
@@ -256,14 +264,14 @@ class LocalNet3D:
         dw, db = self._wgrad(hs[-1], dyc)
         grads[-1] = (dw[:, : self.cout, :], db[: self.cout])
         wt = flip_weights(self.wk[-1], 32, self.cout, 32)
-        dh = self._conv(dyc, wt, None, MODE_OUT_CL, 32)            # dL/dh_L, FP32 channels-last
+        # dL/dh of the last hidden layer (FP32 channels-last) and, fused, dL/dz = dh * gelu'(z)
+        dh, dz = self._conv(dyc, wt, None, MODE_OUT_CL_DGELU, 32, z_in=zs[-1])
         for k in range(self.layers - 1, 0, -1):
-            dz = gelu_bwd(dh, zs[k])
             dw, db = self._wgrad(hs[k], dz)
             grads[k] = (dw, db)
             wt = flip_weights(self.wk[k], 32, 32, 32)
-            dh = self._conv(dz, wt, None, MODE_OUT_CL, 32, addend=dh)  # residual path adds dh
-        dz = gelu_bwd(dh, zs[0])
+            # residual path adds dh; the fused epilogue gives layer k-1's dz
+            dh, dz = self._conv(dz, wt, None, MODE_OUT_CL_DGELU, 32, addend=dh, z_in=zs[k - 1])
         dw, db = self._wgrad(hs[0], dz)
         grads[0] = (dw[:, :, : self.cin], db)
         self._saved = None

@@ -291,7 +291,11 @@ int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* sta
  *   1: z_out BF16 [B][G^3][32] = conv + bias (may be NULL), h_out = gelu(z);
  *   2: as 1 with a residual: h_out = x + gelu(z) (cin = cout = 32);
  *   3: y_out FP32 channels-last [B][G^3][cout] = conv (+ addend, same layout),
- *      no bias (the data gradient, with augur_conv3_flip_weights()).
+ *      no bias (the data gradient, with augur_conv3_flip_weights());
+ *   4: as 3 with cin = cout = 32, and also h_out BF16 [B][G^3][32] =
+ *      y_out * gelu'(z_out), z_out read here (the previous layer's saved
+ *      pre-activation): the data gradient fused with the next GELU backward
+ *      (augur_gelu_bwd's arithmetic), so y_out is not read back.
  * gelu is the tanh form (torch's approximate='tanh').  Replaces the
  * nn.Conv2d + F.gelu layers of LocalNet (scripts/scatter_field.py:15-26),
  * here in 3D with periodic padding.  Runs the library's default tile. */

@@ -243,3 +243,25 @@ def test_autotuned_localnet_matches_default_tiles(dev, tmp_path):
     tuner.save(path)
     t2 = A.ConvTuner(path=path)
     assert t2.table == tuner.table
+
+
+@pytest.mark.parametrize("with_add", [False, True])
+def test_fused_gelu_backward_data_gradient(dev, with_add):
+    """Mode 4 = mode 3 (data gradient, + addend) followed by augur_gelu_bwd on
+    the previous layer's pre-activation, in one launch: the same FP32 output
+    and the same BF16 dz."""
+    from augur_amd import surrogate as S
+    g = torch.Generator().manual_seed(41 + int(with_add))
+    B, G = 2, 64
+    dz_in = bf16_round(torch.randn(B, G, G, G, 32, generator=g)).to(dev).to(torch.bfloat16)
+    w = bf16_round(torch.randn(27, 32, 32, generator=g) * 0.1)
+    wk = S.kernel_weights(w.to(dev), 32)
+    z_prev = (2.0 * torch.randn(B, G, G, G, 32, generator=g)).to(dev).to(torch.bfloat16)
+    add = torch.randn(B, G, G, G, 32, generator=g).to(dev) if with_add else None
+    y3 = S.conv3_fwd(dz_in, wk, None, S.MODE_OUT_CL, 32, addend=add)
+    dz3 = S.gelu_bwd(y3, z_prev)
+    y4, dz4 = S.conv3_fwd(dz_in, wk, None, S.MODE_OUT_CL_DGELU, 32, addend=add, z_in=z_prev)
+    assert torch.equal(y4, y3)
+    d = (dz4.float() - dz3.float()).abs()
+    assert float(d.max()) <= float((2.0 ** -7 * dz3.float().abs()).max()) + 1e-30
+    assert float((dz4 != dz3).float().mean()) < 1e-3
