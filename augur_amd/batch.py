@@ -144,8 +144,12 @@ class RaggedStepper:
 
 
 class RaggedGraph:
+    """A captured k-step call; it keeps every tensor the graph reads or writes
+    alive (the replay uses their addresses)."""
+
     def __init__(self, stepper, pos, vel, acc, starts, mass, dt, steps):
         self.state = (pos, vel, acc)
+        self.inputs = (starts, mass)
         # eager warm-up on copies (allocations and kernel loads), then capture;
         # the scenes' densities are recomputed inside the graph from `mass`
         p, v, a = pos.clone(), vel.clone(), acc.clone()
