@@ -441,8 +441,17 @@ class PMSurrogate:
         adjoint of the CIC gather (a CIC deposit of dL/da) and the network
         through its own backward kernels.  Returns the loss."""
         pos = pos.detach().to(self.device, torch.float32).reshape(-1, 3).contiguous()
-        f, _ = self.field(pos, mass, mass_value, save=True)
-        pt = pos[targets].contiguous()
+        rho, force = self.pm.fields(pos, mass, mass_value)
+        return self.train_step_grids(rho, force, pos[targets].contiguous(), a_true, lr=lr)
+
+    def train_step_grids(self, rho, force, pt, a_true, lr=3e-4):
+        """train_step on a scene's PM grids computed once (rho (G, G, G),
+        force (3, G, G, G)) and its target positions pt (T, 3): the particles
+        are not needed again, which lets a large scene be dropped after its
+        grids are made."""
+        x = self.features_from_grids(rho, force)
+        y = self.net.forward(x, save=True)[0] * self.force_scale
+        f = force + y if self.mode == "correction" else y
         a = self.gather(f, pt).double()
         at = a_true.to(self.device, torch.float64)
         norm = float((at ** 2).sum())

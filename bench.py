@@ -184,19 +184,69 @@ def cpu_baseline(pos, vel, G, dt, steps=1):
                       f"restatement (oracle/torch_cpu.py), {sec:.2f} s/step"}
 
 
+def large_surrogate_train(a, N, G, NC, dev):
+    """The correction net for the large workload, trained at its own size: one
+    scene of the same mixture (another seed) drawn on the GPU, FP64 Ewald
+    forces on seeded targets, its PM grids made once (the scene is then
+    dropped), conv tiles autotuned at G^3, then Adam steps on those grids."""
+    import numpy as np
+    import torch
+    from augur_amd import ops, scenes
+    from augur_amd.surrogate import PMSurrogate
+
+    t0 = time.perf_counter()
+    sur = PMSurrogate(G, 1.0, 1.0, dev, mode="correction")
+    pos, vel = scenes.clumpy_device(N, dev, seed=1000, n_clumps=NC)
+    del vel
+    tg = torch.from_numpy(scenes.targets(N, a.targets, seed=1000)).to(dev)
+    a_true = ops.ewald_f64(pos, 1.0 / N, tg, L=1.0, eps=1.0 / G)
+    pt = pos[tg].contiguous()
+    rho, force = sur.pm.fields(pos, mass_value=1.0 / N)
+    del pos
+    sur.pm._scratch = None
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    t_data = time.perf_counter() - t0
+    sur.calibrate(rho, force)
+    t0 = time.perf_counter()
+    sur.net.autotune(1, G)
+    torch.cuda.empty_cache()
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = [sur.train_step_grids(rho, force, pt, a_true, lr=a.sur_lr) for _ in range(a.large_sur_steps)]
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del rho, force, pt, a_true
+    torch.cuda.empty_cache()
+    print(f"[bench] large surrogate: data {t_data:.1f}s, tiles {t_tune:.1f}s, {a.large_sur_steps} steps in "
+          f"{t_train:.1f}s, loss {losses[0]:.3e} -> {losses[-1]:.3e}", file=sys.stderr, flush=True)
+    info = {"scene": f"same mixture, seed 1000, N={N}", "targets": a.targets, "steps": a.large_sur_steps,
+            "lr": a.sur_lr, "seconds_data": t_data, "seconds_tune": t_tune, "seconds_train": t_train,
+            "loss_first": losses[0], "loss_last": losses[-1]}
+    return sur, info
+
+
 def large_bench(a):
     """The single-GPU large-scale workload: N = 2^30 (50% uniform + 50% in 4096
     Gaussian clumps, sigma 0.02, drawn on the GPU), 512^3 grid, KDK steps with
     the FP64 momentum and kinetic energy of every step (fused into the gather);
     force rel-RMS of the initial PM forces vs FP64 Ewald on seeded targets;
-    peak device memory."""
+    peak device memory.  Then (unless --no-surrogate) the same number of
+    surrogate KDK steps with a correction net trained at this size, with its
+    force rel-RMS on the same targets."""
     import numpy as np
     import torch
     from augur_amd import _lib, ops, scenes
     from augur_amd.pm import PeriodicPM
+    from augur_amd.surrogate import rel_rms
 
     dev = torch.device("cuda:0")
     N, G, NC = 1 << 30, 512, 4096
+    sur = sur_info = None
+    if not a.no_surrogate:
+        sur, sur_info = large_surrogate_train(a, N, G, NC, dev)
+        sur_info["peak_device_bytes_training"] = torch.cuda.max_memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     pos, vel = scenes.clumpy_device(N, dev, seed=4738, n_clumps=NC)
     torch.cuda.synchronize()
@@ -220,6 +270,16 @@ def large_bench(a):
         rel = float((((a_pm - a_ex) ** 2).sum(-1).mean() / (a_ex ** 2).sum(-1).mean()).sqrt())
         print(f"[bench] PM force rel-RMS vs FP64 Ewald on {a.targets} targets: {rel:.4e} "
               f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+        if sur is not None:
+            f, _ = sur.field(st.pos, mass_value=1.0 / N)
+            sur.pm._scratch = None
+            rel_sur = rel_rms(sur.gather(f, st.pos[tg].contiguous()), a_ex)
+            del f
+            torch.cuda.empty_cache()
+            sur_info["force_rel_rms_vs_fp64"] = rel_sur
+            sur_info["rel_rms_ratio_vs_plain_pm"] = rel_sur / rel
+            print(f"[bench] surrogate force rel-RMS vs FP64 Ewald: {rel_sur:.4e} (plain PM {rel:.4e})",
+                  file=sys.stderr, flush=True)
     m = 1.0 / N
     pm.kdk(st, a.dt, a.warmup) if a.warmup > 0 else None
     # per-kernel HIP-event breakdown on its own run, then the timed run with
@@ -245,6 +305,20 @@ def large_bench(a):
     em = torch.cat([em0[None], diag]).cpu().numpy()
     print(f"[bench] {a.steps} steps (FP64 sums fused) in {sec * 1e3:.1f} ms -> {value:.4e} particle-steps/s",
           file=sys.stderr, flush=True)
+    if sur is not None:
+        # surrogate KDK steps of the same state (probe residual check on the first step of each call)
+        probe = torch.arange(0, N, N // 4096, device=dev)
+        sur.kdk(st, a.dt, 1, probe=probe)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        fell_back, res = sur.kdk(st, a.dt, a.steps, probe=probe)
+        torch.cuda.synchronize()
+        sec_s = time.perf_counter() - t1
+        sur_info.update({"value": N * a.steps / sec_s, "unit": "particle-steps/s", "ms_per_step": sec_s * 1e3 / a.steps,
+                         "speed_ratio_vs_plain_pm": (N * a.steps / sec_s) / value, "fell_back": bool(fell_back),
+                         "probe_residual": res})
+        print(f"[bench] surrogate {a.steps} steps in {sec_s * 1e3:.1f} ms -> {N * a.steps / sec_s:.4e} "
+              f"particle-steps/s (fell back: {fell_back})", file=sys.stderr, flush=True)
     kernels = {k: {"ms_total": v[0], "launches": v[1], "us_avg": v[0] / v[1] * 1e3} for k, v in timing.items()}
     dom = max(timing.items(), key=lambda kv: kv[1][0])
     dname, (dms, dcnt) = dom
@@ -264,7 +338,7 @@ def large_bench(a):
         "roofline": {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
                      "avg_launch_us": dms / dcnt * 1e3, "algorithmic_bytes": ab},
-        "kernels": kernels, "cpu_baseline": None,
+        "kernels": kernels, "cpu_baseline": None, "surrogate": sur_info,
     }
     print(json.dumps(out), flush=True)
 
@@ -424,6 +498,7 @@ def main():
     ap.add_argument("--sur-train-steps", type=int, default=120)
     ap.add_argument("--sur-lr", type=float, default=1e-3)
     ap.add_argument("--sur-targets", type=int, default=4096, help="FP64 target particles per training scene")
+    ap.add_argument("--large-sur-steps", type=int, default=40, help="surrogate training steps of the large workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run one 256 MiB device-to-device copy (the byte-counter "
                          "calibration kernel of a rocprofv3 --pmc run)")
