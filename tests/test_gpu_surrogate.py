@@ -125,3 +125,21 @@ def test_residual_check_falls_back_to_plain_pm(dev):
     fell_back, res = sur.kdk(s2, 1e-3, 3, probe=probe)
     assert fell_back
     assert torch.equal(s1.pos, s2.pos) and torch.equal(s1.vel, s2.vel)
+
+
+def test_train_step_on_grids_equals_train_step(dev):
+    """train_step_grids on grids made once gives the loss and the weights that
+    train_step (which makes the grids itself) gives, bit for bit."""
+    from augur_amd.surrogate import PMSurrogate
+    n, G = 30000, 64
+    pos, tg, a_ex = clumpy_scene(5, n, dev, K=512, G=G)
+    s1 = PMSurrogate(G, 1.0, 1.0, dev, mode="correction", init_std=0.05)
+    s2 = PMSurrogate(G, 1.0, 1.0, dev, mode="correction", init_std=0.05)
+    rho, force = s2.pm.fields(pos, mass_value=1.0 / n)
+    pt = pos[tg].contiguous()
+    for _ in range(3):
+        l1 = s1.train_step(pos, tg, a_ex, mass_value=1.0 / n, lr=1e-3)
+        l2 = s2.train_step_grids(rho, force, pt, a_ex, lr=1e-3)
+        assert l1 == l2
+    for w1, w2 in zip(s1.net.parameters(), s2.net.parameters()):
+        assert torch.equal(w1, w2)
