@@ -573,6 +573,17 @@ constexpr int DCS = CT_Z * CT_Y * DS + 8;  // per-channel stride of the dz tile 
 static_assert((XCS / 2) % 64 == 36 && (DCS / 2) % 64 == 36, "conflict-free 16-byte fragment reads");
 static_assert(HX % 2 == 0, "x halo rows load in voxel pairs");
 
+// 16 input channels: one 32-column MFMA serves two taps of the same dx (lanes
+// 0-15 the first tap's channels, 16-31 the second's), so the 27 taps are 12
+// pairs and 3 singles, 15 MFMA chains (waves take chains w, w + 4, ...).
+// Chain c: taps wg16_tap(c, 0) and wg16_tap(c, 1) (-1: none).
+constexpr int WG16_CHAINS = 15, WG16_PER_WAVE = 4;
+__host__ __device__ constexpr int wg16_tap(int c, int second) {
+  // taps of one dx in (dz, dy) order: m -> 9 (m / 3) + 3 (m % 3) + dx
+  return c < 12 ? 9 * ((2 * (c % 4) + second) / 3) + 3 * ((2 * (c % 4) + second) % 3) + c / 4
+                : (second ? -1 : 9 * (8 / 3) + 3 * (8 % 3) + (c - 12));
+}
+
 template <int CIN>
 __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __restrict__ x,
                                                               const uint16_t* __restrict__ dz, int B, int G,
@@ -589,9 +600,10 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
   const int i = lane & 31, kh = lane >> 5;
   const int ntx = G / CT_X, nty = G / CT_Y, ntz = G / CT_Z;
   const int64_t G3 = (int64_t)G * G * G;
-  f32x16 acc[DW_TAPS_PER_WAVE][NB];
+  constexpr int NCH = CIN == 16 ? WG16_PER_WAVE : DW_TAPS_PER_WAVE;  // MFMA chains per wave
+  f32x16 acc[NCH][NB];
 #pragma unroll
-  for (int k = 0; k < DW_TAPS_PER_WAVE; ++k)
+  for (int k = 0; k < NCH; ++k)
 #pragma unroll
     for (int n = 0; n < NB; ++n)
 #pragma unroll
@@ -707,8 +719,35 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
       const int vb = half * 16 + kh * 8;  // first of this lane's 8 voxels
       const bf16x8 av = *reinterpret_cast<const bf16x8*>(&dl[i * DCS + (rz * CT_Y + ry) * DS + vb]);
       if ((st & 3) == WV) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ones, accb, 0, 0, 0);
+      if constexpr (CIN == 16) {
 #pragma unroll
-      for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+        for (int k = 0; k < WG16_PER_WAVE; ++k) {
+          const int c = WV + 4 * k;
+          if (c < WG16_CHAINS) {
+            const int dxo = c < 12 ? c / 4 : c - 12;  // shared by the chain's two taps
+            const int tp = (i < 16) ? wg16_tap(c, 0) : wg16_tap(c, 1);
+            const int ci = i & 15;
+            bf16x8 bv;
+            if (tp >= 0) {
+              const int dzo = tp / 9, dyo = (tp / 3) % 3;
+              const int row = ((rz + dzo) * HY + ry + dyo) * XS + vb;
+              if (dxo == 2) {
+                const uint32_t* p = reinterpret_cast<const uint32_t*>(&xl[0][ci * XCS + row + 2]);
+                const u32x4 u = {p[0], p[1], p[2], p[3]};
+                bv = __builtin_bit_cast(bf16x8, u);
+              } else {
+                bv = *reinterpret_cast<const bf16x8*>(&xl[dxo][ci * XCS + row]);
+              }
+            } else {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) bv[e] = (__bf16)0.0f;
+            }
+            acc[k][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[k][0], 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < (CIN == 16 ? 0 : DW_TAPS_PER_WAVE); ++k) {
         const int tap = WV + 4 * k;
         if (tap < 27) {
           const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
@@ -757,8 +796,22 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
   __syncthreads();
   // partial layout: [wg][27][32 co][CIN] then [wg] bias block of 32
   float* pw = part + (int64_t)blockIdx.x * (27 * 32 * CIN + 32);
+  if constexpr (CIN == 16) {
 #pragma unroll
-  for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+    for (int k = 0; k < WG16_PER_WAVE; ++k) {
+      const int c = wv + 4 * k;
+      const int tp = c < WG16_CHAINS ? ((i < 16) ? wg16_tap(c, 0) : wg16_tap(c, 1)) : -1;
+      if (tp >= 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int co = 8 * (q / 4) + 4 * kh + (q % 4);
+          pw[(tp * 32 + co) * CIN + (i & 15)] = acc[k][0][q];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < (CIN == 16 ? 0 : DW_TAPS_PER_WAVE); ++k) {
     const int tap = wv + 4 * k;
     if (tap >= 27) break;
 #pragma unroll
