@@ -513,7 +513,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
             val.z += ad.z;
             val.w += ad.w;
           }
-          *reinterpret_cast<float4*>(&y_out[o]) = val;
+          if (MODE == CONV_OUT_F32_CL || y_out) *reinterpret_cast<float4*>(&y_out[o]) = val;
           if (MODE == CONV_OUT_F32_CL_DGELU) {  // dz = y * gelu'(z_in), BF16 (k_gelu_bwd's arithmetic)
             const uint2 zr = *reinterpret_cast<const uint2*>(&z_out[o]);
             uint2 dp;
@@ -1016,7 +1016,7 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
   AUGUR_REQUIRE(cout >= 1 && cout <= 32, AUGUR_EINVAL, "conv: cout must be 1..32");
   AUGUR_REQUIRE(is_pow2(zc) && zc >= 2 && zc <= G, AUGUR_EINVAL, "conv: zc must be a power of two in [2, G]");
   AUGUR_REQUIRE(mode != CONV_OUT_F32_CL_DGELU || (cin == 32 && cout == 32), AUGUR_EINVAL,
-                "conv: mode 4 needs y_out, z_out (read), h_out (dz), cin = cout = 32");
+                "conv: mode 4 needs z_out (read), h_out (dz), cin = cout = 32");
   const int mask = conv_variant_mask(cin, mode, cout);
   if (variant == TILE_DEFAULT) variant = conv_default_variant(mask);
   AUGUR_REQUIRE(variant >= TILE_ZM && variant <= TILE_NARROW && ((mask >> variant) & 1), AUGUR_EUNSUPPORTED,
@@ -1046,8 +1046,8 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
                        : launch_fwd<32, CONV_OUT_F32_CL>(x, w, nullptr, B, G, cout, nullptr, nullptr, y_out, addend,
                                                          variant, zc, s, "conv_bwd_data");
     case CONV_OUT_F32_CL_DGELU:
-      AUGUR_REQUIRE(y_out && z_out && h_out && cin == 32 && cout == 32, AUGUR_EINVAL,
-                    "conv: mode 4 needs y_out, z_out (read), h_out (dz), cin = cout = 32");
+      AUGUR_REQUIRE(z_out && h_out && cin == 32 && cout == 32, AUGUR_EINVAL,
+                    "conv: mode 4 needs z_out (read), h_out (dz), cin = cout = 32");
       return launch_fwd<32, CONV_OUT_F32_CL_DGELU>(x, w, nullptr, B, G, 32, z_out, h_out, y_out, addend, variant,
                                                    zc, s, "conv_bwd_data");
     default:

@@ -52,11 +52,12 @@ def kernel_weights(w_taps, cinp):
     return out
 
 
-def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None, z_in=None):
+def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None, z_in=None, want_y=True):
     """x: BF16 channels-last (B, G, G, G, cin); w: BF16 (27, 32, cin); bias FP32 (32,).
     tile: (kernel family, z planes per workgroup) from the autotuner, or None
     for the library's default tile.  Mode 4 (the data gradient fused with the
-    previous layer's GELU backward) reads z_in and returns (y, dz)."""
+    previous layer's GELU backward) reads z_in and returns (y, dz); with
+    want_y=False it writes dz only and returns (None, dz)."""
     dev = _dev(x)
     B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
     y = h = None
@@ -66,7 +67,7 @@ def conv3_fwd(x, w, bias, mode, cout, z_out=None, addend=None, tile=None, z_in=N
         y = torch.empty(B, G, G, G, cout, dtype=torch.float32, device=dev)
     elif mode == MODE_OUT_CL_DGELU:
         assert z_in is not None and z_in.dtype == BF16 and z_in.shape == (B, G, G, G, 32)
-        y = torch.empty(B, G, G, G, 32, dtype=torch.float32, device=dev)
+        y = torch.empty(B, G, G, G, 32, dtype=torch.float32, device=dev) if want_y else None
         h = torch.empty(B, G, G, G, 32, dtype=BF16, device=dev)
         z_out = z_in.contiguous()
     else:
@@ -190,14 +191,14 @@ class LocalNet3D:
         return [(from_taps(w), b[: w.shape[1]]) for w, b in zip(self.w, self.b)]
 
     # ------------------------------------------------------ tuned tiles
-    def _conv(self, x, w, b, mode, cout, z_out=None, addend=None, z_in=None):
+    def _conv(self, x, w, b, mode, cout, z_out=None, addend=None, z_in=None, want_y=True):
         tile = None
         if self.tuner is not None:
             if self._tuning:
                 tile = self.tuner.tune_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, z_in=z_in)
             else:
                 tile = self.tuner.fwd_tile(x.shape[-1], mode, cout, x.shape[0], x.shape[1], addend is not None)
-        return conv3_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, tile=tile, z_in=z_in)
+        return conv3_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, tile=tile, z_in=z_in, want_y=want_y)
 
     def _wgrad(self, x, dz):
         nwg = None
@@ -271,7 +272,8 @@ class LocalNet3D:
             grads[k] = (dw, db)
             wt = flip_weights(self.wk[k], 32, 32, 32)
             # residual path adds dh; the fused epilogue gives layer k-1's dz
-            dh, dz = self._conv(dz, wt, None, MODE_OUT_CL_DGELU, 32, addend=dh, z_in=zs[k - 1])
+            # (the last one's dh is not needed: only its dz is written)
+            dh, dz = self._conv(dz, wt, None, MODE_OUT_CL_DGELU, 32, addend=dh, z_in=zs[k - 1], want_y=k > 1)
         dw, db = self._wgrad(hs[0], dz)
         grads[0] = (dw[:, :, : self.cin], db)
         self._saved = None

@@ -295,7 +295,8 @@ int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* sta
  *   4: as 3 with cin = cout = 32, and also h_out BF16 [B][G^3][32] =
  *      y_out * gelu'(z_out), z_out read here (the previous layer's saved
  *      pre-activation): the data gradient fused with the next GELU backward
- *      (augur_gelu_bwd's arithmetic), so y_out is not read back.
+ *      (augur_gelu_bwd's arithmetic), so y_out is not read back; y_out may
+ *      be NULL when only dz is wanted (the last layer of the chain).
  * gelu is the tanh form (torch's approximate='tanh').  Replaces the
  * nn.Conv2d + F.gelu layers of LocalNet (scripts/scatter_field.py:15-26),
  * here in 3D with periodic padding.  Runs the library's default tile. */

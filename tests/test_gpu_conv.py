@@ -262,6 +262,8 @@ def test_fused_gelu_backward_data_gradient(dev, with_add):
     dz3 = S.gelu_bwd(y3, z_prev)
     y4, dz4 = S.conv3_fwd(dz_in, wk, None, S.MODE_OUT_CL_DGELU, 32, addend=add, z_in=z_prev)
     assert torch.equal(y4, y3)
+    y5, dz5 = S.conv3_fwd(dz_in, wk, None, S.MODE_OUT_CL_DGELU, 32, addend=add, z_in=z_prev, want_y=False)
+    assert y5 is None and torch.equal(dz5, dz4)
     d = (dz4.float() - dz3.float()).abs()
     assert float(d.max()) <= float((2.0 ** -7 * dz3.float().abs()).max()) + 1e-30
     assert float((dz4 != dz3).float().mean()) < 1e-3
