@@ -24,8 +24,8 @@ import torch
 
 from . import _lib
 
-TILE_DEFAULT, TILE_ZM, TILE_SPLIT, TILE_NARROW = 0, 1, 2, 3
-FAMILY = {TILE_ZM: "zm", TILE_SPLIT: "split", TILE_NARROW: "narrow"}
+TILE_DEFAULT, TILE_ZM, TILE_SPLIT, TILE_NARROW, TILE_IN4 = 0, 1, 2, 3, 4
+FAMILY = {TILE_ZM: "zm", TILE_SPLIT: "split", TILE_NARROW: "narrow", TILE_IN4: "in4"}
 ZC_CANDIDATES = (8, 16, 32, 64, 128)
 DEFAULT_ZC = 16
 
@@ -33,7 +33,7 @@ DEFAULT_ZC = 16
 def variants(cin, mode, cout):
     """Kernel families that can run a layer (library's own rule)."""
     mask = _lib.get().augur_conv3_tile_variants(int(cin), int(mode), int(cout))
-    return [v for v in (TILE_ZM, TILE_SPLIT, TILE_NARROW) if (mask >> v) & 1]
+    return [v for v in FAMILY if (mask >> v) & 1]
 
 
 def candidates(cin, mode, cout, G):

@@ -252,6 +252,153 @@ __global__ __launch_bounds__(64 * (ZM_Y / RPW)) void k_conv3_zm(const uint16_t* 
   }
 }
 
+// ------------------------------------------- 4-channel input forward
+// The network's input layer has 4 real channels.  With 16 padded channels a
+// 32x32x16 MFMA spends 12 of its 16 K slots on zeros; here K packs 4 taps x
+// 4 channels instead (K index 4 j + ci for tap 4 g + j of K-group g), so a row
+// takes 7 MFMAs (28 tap slots, the last one empty) instead of 27.  Input:
+// BF16 channels-last [B][G^3][4] (8 bytes per voxel); weights [27][32][4].
+// Lane (i, kh) of the A operand holds output channel i for taps 4 g + 2 kh
+// and 4 g + 2 kh + 1; of the B operand, voxel i of the output row for the
+// same two taps, each an 8-byte read of the shifted input voxel.  Same
+// z-marching column and epilogue as k_conv3_zm (modes 1: GELU, z optional).
+constexpr int IN4_RPW = 2;                   // output rows per wave (eight waves)
+constexpr int IN4_THREADS = 64 * (ZM_Y / IN4_RPW);
+constexpr int IN4_GROUPS = 7;                // ceil(27 / 4) K-groups of 4 taps
+
+template <int MODE>
+__global__ __launch_bounds__(IN4_THREADS) void k_conv3_in4(const uint16_t* __restrict__ x,
+                                                         const uint16_t* __restrict__ w,
+                                                         const float* __restrict__ bias, int G, int zc,
+                                                         uint16_t* __restrict__ z_out,
+                                                         uint16_t* __restrict__ h_out) {
+  constexpr int PLANE = ZM_PY * ZM_PX;  // voxels per plane; 4 channels (8 bytes) each
+  constexpr int PER = (PLANE + IN4_THREADS - 1) / IN4_THREADS;
+  __shared__ __attribute__((aligned(16))) uint2 ring[3 * PLANE];
+  __shared__ __attribute__((aligned(16))) float sbias[32];
+  const int tid = threadIdx.x;
+  const int ntx = G / ZM_X, nty = G / ZM_Y, ntz = G / zc;
+  int t = blockIdx.x;
+  const int bx = t % ntx;
+  t /= ntx;
+  const int by = t % nty;
+  t /= nty;
+  const int bz = t % ntz;
+  const int b = t / ntz;
+  const int x0 = bx * ZM_X, y0 = by * ZM_Y, zs = bz * zc;
+  const int64_t G3 = (int64_t)G * G * G;
+  const uint2* xb = reinterpret_cast<const uint2*>(x) + (int64_t)b * G3;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int i = lane & 31, kh = lane >> 5;
+  if (tid < 32) sbias[tid] = bias[tid];
+
+  // A operand of K-group g: w[tap][co = i][0..3] for taps 4 g + 2 kh, + 1 (zero past tap 26)
+  bf16x8 wr[IN4_GROUPS];
+#pragma unroll
+  for (int g = 0; g < IN4_GROUPS; ++g) {
+    u32x4 u = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const int tap = 4 * g + 2 * kh + jj;
+      if (tap < 27) {
+        const uint2 q = *reinterpret_cast<const uint2*>(&w[((int64_t)tap * 32 + i) * 4]);
+        u[2 * jj] = q.x;
+        u[2 * jj + 1] = q.y;
+      }
+    }
+    wr[g] = __builtin_bit_cast(bf16x8, u);
+  }
+
+  auto load_plane = [&](int zz, uint2 (&r)[PER]) __attribute__((always_inline)) {
+    const int gz = zz & (G - 1);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int v = tid + k * IN4_THREADS;
+      if (v < PLANE) {
+        const int tx = v % ZM_PX, ty = v / ZM_PX;
+        const int gx = (x0 + tx - 1) & (G - 1), gy = (y0 + ty - 1) & (G - 1);
+        r[k] = xb[((int64_t)gz * G + gy) * G + gx];
+      }
+    }
+  };
+  auto store_plane = [&](int slot, const uint2 (&r)[PER]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int v = tid + k * IN4_THREADS;
+      if (v < PLANE) ring[slot * PLANE + v] = r[k];
+    }
+  };
+  {  // planes zs - 1, zs, zs + 1 -> slots 0, 1, 2
+    uint2 r0[PER];
+#pragma unroll 1
+    for (int sl = 0; sl < 3; ++sl) {
+      load_plane(zs - 1 + sl, r0);
+      store_plane(sl, r0);
+    }
+  }
+  __syncthreads();
+
+#pragma unroll 1
+  for (int zi = 0; zi < zc; ++zi) {
+    const int z = zs + zi;
+    uint2 pre[PER];
+    if (zi + 1 < zc) load_plane(z + 2, pre);  // in flight during the MFMAs below
+    f32x16 acc[IN4_RPW];
+#pragma unroll
+    for (int r = 0; r < IN4_RPW; ++r)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[r][q] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < IN4_RPW; ++r) {
+      const int ly = wv * IN4_RPW + r;  // output row within the column
+#pragma unroll
+      for (int g = 0; g < IN4_GROUPS; ++g) {
+        u32x4 u = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          const int tap = 4 * g + 2 * kh + jj;
+          if (tap < 27) {
+            const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
+            const uint2 q = ring[((zi + dzo) % 3) * PLANE + (ly + dyo) * ZM_PX + i + dxo];
+            u[2 * jj] = q.x;
+            u[2 * jj + 1] = q.y;
+          }
+        }
+        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[g], __builtin_bit_cast(bf16x8, u), acc[r], 0, 0, 0);
+      }
+    }
+    // epilogue: acc[r][4 j + e] holds output channel co = 8 j + 4 kh + e of voxel (x0 + i, y0 + row, z)
+#pragma unroll
+    for (int r = 0; r < IN4_RPW; ++r) {
+      const int ly = wv * IN4_RPW + r;
+      const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + i;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int co0 = 8 * j + 4 * kh;
+        const float4 bj = *reinterpret_cast<const float4*>(&sbias[co0]);
+        const float v0 = acc[r][4 * j] + bj.x, v1 = acc[r][4 * j + 1] + bj.y, v2 = acc[r][4 * j + 2] + bj.z,
+                    v3 = acc[r][4 * j + 3] + bj.w;
+        const int64_t o = ((int64_t)b * G3 + sv) * 32 + co0;
+        if (z_out) {
+          uint2 zp;
+          zp.x = pack_bf16x2(v0, v1);
+          zp.y = pack_bf16x2(v2, v3);
+          *reinterpret_cast<uint2*>(&z_out[o]) = zp;
+        }
+        uint2 hp;
+        hp.x = pack_bf16x2(gelu(v0), gelu(v1));
+        hp.y = pack_bf16x2(gelu(v2), gelu(v3));
+        *reinterpret_cast<uint2*>(&h_out[o]) = hp;
+      }
+    }
+    __syncthreads();  // every wave is done with the slot of plane z - 1
+    if (zi + 1 < zc) {
+      store_plane(zi % 3, pre);  // plane z + 2 replaces plane z - 1
+      __syncthreads();
+    }
+  }
+}
+
 // ------------------------------------------------- narrow output forward
 // The same z-marching column for a layer with at most 16 output channels and
 // CIN = 32 (the network's output convolution): v_mfma_f32_16x16x32_bf16 with
@@ -898,6 +1045,10 @@ __global__ void k_pack_features(const float* __restrict__ rho, const float* __re
     w[1] = pack_bf16x2(fb[G3] * force_mul, fb[2 * G3] * force_mul);
     w[2] = 0u;
     w[3] = 0u;
+    if (CP == 4) {  // the 4-channel input image: 8 bytes per voxel
+      *reinterpret_cast<uint2*>(&out[t * 4]) = uint2{w[0], w[1]};
+      continue;
+    }
     *reinterpret_cast<u32x4*>(&out[t * CP]) = w;
     const u32x4 z = {0u, 0u, 0u, 0u};
     for (int g = 1; g < CP / 8; ++g) *reinterpret_cast<u32x4*>(&out[t * CP + g * 8]) = z;
@@ -942,9 +1093,10 @@ int conv_tiles_ok(int64_t G) { return is_pow2(G) && G >= 32 && G <= 1024; }
 //   NARROW k_conv3_zm_narrow: 16x16x32 MFMAs over at most 16 output channels (CIN 32)
 // Every family computes each output voxel with the same per-voxel MFMA chain
 // for any z planes per workgroup, so results do not depend on zc.
-enum ConvTile : int { TILE_DEFAULT = 0, TILE_ZM = 1, TILE_SPLIT = 2, TILE_NARROW = 3 };
+enum ConvTile : int { TILE_DEFAULT = 0, TILE_ZM = 1, TILE_SPLIT = 2, TILE_NARROW = 3, TILE_IN4 = 4 };
 
 int conv_variant_mask(int cin, int mode, int cout) {
+  if (cin == 4) return (mode == CONV_GELU && cout == 32) ? 1 << TILE_IN4 : 0;  // k_conv3_in4 only
   int m = 1 << TILE_ZM;
   if (cin == 32 && (mode == CONV_GELU || mode == CONV_RES_GELU || (mode == CONV_OUT_F32_CL && cout == 32)))
     m |= 1 << TILE_SPLIT;
@@ -954,6 +1106,7 @@ int conv_variant_mask(int cin, int mode, int cout) {
 }
 
 int conv_default_variant(int mask) {
+  if (mask & (1 << TILE_IN4)) return TILE_IN4;
   if (mask & (1 << TILE_NARROW)) return TILE_NARROW;
   if (AUGUR_CONV_SPLIT && (mask & (1 << TILE_SPLIT))) return TILE_SPLIT;
   return TILE_ZM;
@@ -1001,7 +1154,7 @@ int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t 
 extern "C" {
 
 int augur_conv3_tile_variants(int cin, int mode, int cout) {
-  if ((cin != 16 && cin != 32) || mode < CONV_OUT_F32_CF || mode > CONV_OUT_F32_CL_DGELU || cout < 1 || cout > 32)
+  if ((cin != 4 && cin != 16 && cin != 32) || mode < CONV_OUT_F32_CF || mode > CONV_OUT_F32_CL_DGELU || cout < 1 || cout > 32)
     return 0;
   if (mode == CONV_RES_GELU && cin != 32) return 0;
   return conv_variant_mask(cin, mode, cout);
@@ -1012,16 +1165,28 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
                               int variant, int zc, void* stream) {
   AUGUR_REQUIRE(conv_tiles_ok(G), AUGUR_EUNSUPPORTED, "conv: G must be a power of two in [32, 1024]");
   AUGUR_REQUIRE(B >= 1 && x && w, AUGUR_EINVAL, "conv: bad arguments");
-  AUGUR_REQUIRE(cin == 16 || cin == 32, AUGUR_EUNSUPPORTED, "conv: cin must be 16 or 32 (pad channels)");
+  AUGUR_REQUIRE(cin == 4 || cin == 16 || cin == 32, AUGUR_EUNSUPPORTED,
+                "conv: cin must be 4, 16 or 32 (pad channels)");
   AUGUR_REQUIRE(cout >= 1 && cout <= 32, AUGUR_EINVAL, "conv: cout must be 1..32");
   AUGUR_REQUIRE(is_pow2(zc) && zc >= 2 && zc <= G, AUGUR_EINVAL, "conv: zc must be a power of two in [2, G]");
+  AUGUR_REQUIRE(cin != 4 || (mode == CONV_GELU && cout == 32 && h_out && bias), AUGUR_EUNSUPPORTED,
+                "conv: cin 4 (the input layer) supports mode 1 with 32 outputs only");
   AUGUR_REQUIRE(mode != CONV_OUT_F32_CL_DGELU || (cin == 32 && cout == 32), AUGUR_EINVAL,
                 "conv: mode 4 needs z_out (read), h_out (dz), cin = cout = 32");
   const int mask = conv_variant_mask(cin, mode, cout);
   if (variant == TILE_DEFAULT) variant = conv_default_variant(mask);
-  AUGUR_REQUIRE(variant >= TILE_ZM && variant <= TILE_NARROW && ((mask >> variant) & 1), AUGUR_EUNSUPPORTED,
+  AUGUR_REQUIRE(variant >= TILE_ZM && variant <= TILE_IN4 && ((mask >> variant) & 1), AUGUR_EUNSUPPORTED,
                 "conv: tile variant cannot run this layer");
   hipStream_t s = as_stream(stream);
+  if (cin == 4) {
+    const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / zc);
+    AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
+    TimedScope ts_("conv_fwd_gelu", s);
+    hipLaunchKernelGGL((k_conv3_in4<CONV_GELU>), dim3((unsigned)blocks), dim3(IN4_THREADS), 0, s, x, w, bias, (int)G,
+                       zc, z_out, h_out);
+    AUGUR_LAUNCHED("conv_fwd_gelu");
+    return AUGUR_OK;
+  }
   switch (mode) {
     case CONV_OUT_F32_CF:
       AUGUR_REQUIRE(y_out && bias, AUGUR_EINVAL, "conv: mode 0 needs y_out and bias");
@@ -1131,7 +1296,7 @@ int augur_pack_cl_bf16(const float* in, int C, int64_t B, int64_t G, int CP, flo
 
 int augur_pack_features_bf16(const float* rho, const float* force, int64_t B, int64_t G, float rho_mul, float rho_add,
                              float force_mul, int CP, uint16_t* out, void* stream) {
-  AUGUR_REQUIRE(rho && force && out && B >= 1 && G >= 1 && CP >= 8 && CP % 8 == 0, AUGUR_EINVAL,
+  AUGUR_REQUIRE(rho && force && out && B >= 1 && G >= 1 && (CP == 4 || (CP >= 8 && CP % 8 == 0)), AUGUR_EINVAL,
                 "pack_features: bad arguments");
   const int64_t G3 = G * G * G;
   TimedScope ts_("pack_features", as_stream(stream));

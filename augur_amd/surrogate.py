@@ -161,11 +161,16 @@ class LocalNet3D:
         """BF16 kernel copies of the FP32 masters, rewritten in place once they
         exist (a captured graph keeps reading the same buffers)."""
         wk = [kernel_weights(w, pad16(w.shape[2])) for w in self.w]
-        if getattr(self, "wk", None) is not None and all(a.shape == b.shape for a, b in zip(self.wk, wk)):
-            for a, b in zip(self.wk, wk):
-                a.copy_(b)
+        # the input layer's 4-channel copy for 4-channel input images (k_conv3_in4)
+        wk.append(kernel_weights(self.w[0], 4) if self.cin <= 4 else None)
+        if getattr(self, "wk", None) is not None and all(
+                (a is None and b is None) or (a is not None and b is not None and a.shape == b.shape)
+                for a, b in zip(self.wk + [self.wk_in4], wk)):
+            for a, b in zip(self.wk + [self.wk_in4], wk):
+                if a is not None:
+                    a.copy_(b)
         else:
-            self.wk = wk
+            self.wk, self.wk_in4 = wk[:-1], wk[-1]
 
     def parameters(self):
         return self.w + self.b
@@ -223,6 +228,9 @@ class LocalNet3D:
             if backward:
                 dy = 1e-3 * torch.randn(B, self.cout, G, G, G, device=dev, generator=g)
                 self.backward(dy)
+            if self.wk_in4 is not None:   # the 4-channel input layer of the inference path
+                x4 = torch.randn(B, G, G, G, 4, device=dev, generator=g).to(BF16)
+                self._conv(x4, self.wk_in4, self.b[0], MODE_GELU, 32)
         finally:
             self._tuning = False
             self._saved = None
@@ -238,11 +246,14 @@ class LocalNet3D:
         return self.forward_packed(pack_cl(x, self.cinp), save=save)
 
     def forward_packed(self, xc, save=False):
-        """xc: BF16 channels-last (B, G, G, G, cinp) input -> FP32 (B, cout, G, G, G)."""
+        """xc: BF16 channels-last (B, G, G, G, cinp) input -> FP32 (B, cout, G, G, G).
+        A 4-channel input (inference, cin <= 4) runs the input layer with the
+        taps packed into the MFMA K dimension (k_conv3_in4)."""
         zs, hs = [], [xc]
         need_z = save
         z = torch.empty(xc.shape[:-1] + (32,), dtype=BF16, device=xc.device) if need_z else None
-        h = self._conv(xc, self.wk[0], self.b[0], MODE_GELU, 32, z_out=z)
+        w0 = self.wk_in4 if xc.shape[-1] == 4 else self.wk[0]
+        h = self._conv(xc, w0, self.b[0], MODE_GELU, 32, z_out=z)
         zs.append(z)
         hs.append(h)
         for k in range(1, self.layers):
@@ -359,9 +370,10 @@ class PMSurrogate:
         pass (same values as pack_cl(features_from_grids(...)) up to the
         rounding of the contrast, computed here as one FMA)."""
         G = self.G
-        out = torch.empty(1, G, G, G, self.net.cinp, dtype=BF16, device=self.device)
+        cp = 4 if self.net.cin <= 4 else self.net.cinp   # 4 channels: the packed-tap input layer
+        out = torch.empty(1, G, G, G, cp, dtype=BF16, device=self.device)
         _lib.call("augur_pack_features_bf16", _p(rho), _p(force), 1, G, float(self.rho_scale / mean_density),
-                  float(-self.rho_scale), float(1.0 / self.force_scale), self.net.cinp, _p(out),
+                  float(-self.rho_scale), float(1.0 / self.force_scale), cp, _p(out),
                   _stream(self.device))
         return out
 

@@ -284,7 +284,8 @@ int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* sta
 
 /* BF16 MFMA 3x3x3 periodic convolution, FP32 accumulation, on a G^3 grid
  * (G a power of two in [32, 1024]).  x: BF16 channels-last [B][G^3][cin],
- * cin in {16, 32} (zero-pad narrower inputs); w: BF16 [27][32][cin] with
+ * cin in {16, 32} (zero-pad narrower inputs), or 4 for an input layer in
+ * mode 1 (the network's first convolution); w: BF16 [27][32][cin] with
  * tap = (dz*3 + dy)*3 + dx for offsets d - 1 and rows co >= cout zero;
  * bias: FP32 [32].  Modes (epilogue):
  *   0: y_out FP32 channels-first [B][cout][G^3] = conv + bias;
@@ -311,7 +312,8 @@ int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const fl
  *   AUGUR_CONV_TILE_SPLIT  eight waves, each half the output channels
  *                          (cin 32 with 32 outputs: modes 1, 2, and 3 with cout 32);
  *   AUGUR_CONV_TILE_NARROW 16x16x32 MFMAs over at most 16 outputs (cin 32, modes 0, 3);
- *   AUGUR_CONV_TILE_DEFAULT the library's choice (NARROW, else SPLIT, else ZM).
+ *   AUGUR_CONV_TILE_IN4    the 4-channel input layer (cin 4, mode 1);
+ *   AUGUR_CONV_TILE_DEFAULT the library's choice (IN4, else NARROW, else SPLIT, else ZM).
  * A family computes each voxel the same way for every zc, so zc changes the
  * time and not the bits.  augur_conv3_tile_variants() returns the bit mask
  * (1 << family) of the families that can run a layer (0: unsupported layer).
@@ -321,6 +323,7 @@ int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const fl
 #define AUGUR_CONV_TILE_ZM 1
 #define AUGUR_CONV_TILE_SPLIT 2
 #define AUGUR_CONV_TILE_NARROW 3
+#define AUGUR_CONV_TILE_IN4 4  /* cin 4, mode 1, 32 outputs: 4 taps x 4 channels per MFMA K-step */
 int augur_conv3_tile_variants(int cin, int mode, int cout);
 int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
                               int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
@@ -350,7 +353,7 @@ int augur_pack_cl_bf16(const float* in, int C, int64_t B, int64_t G, int CP, flo
 /* The correction net's input from the PM grids in one pass: BF16
  * channels-last [B][G^3][CP] with channel 0 = rho * rho_mul + rho_add (the
  * scaled density contrast), channels 1..3 = force * force_mul (force: B x 3
- * planes of G^3), the rest zero; CP a multiple of 8.  The input stage of
+ * planes of G^3), the rest zero; CP 4 or a multiple of 8.  The input stage of
  * _grid_force (scripts/scatter_field.py:620-666: the density grid and the
  * field handed to the net), for the 3D correction net. */
 int augur_pack_features_bf16(const float* rho, const float* force, int64_t B, int64_t G, float rho_mul, float rho_add,

@@ -74,6 +74,7 @@ def test_conv_tile_candidates_host_only(lib):
     assert A.variants(32, 3, 16) == [A.TILE_ZM, A.TILE_NARROW]
     assert A.variants(16, 2, 32) == [] and A.variants(8, 1, 32) == [] and A.variants(32, 1, 33) == []
     assert A.variants(32, 4, 32) == [A.TILE_SPLIT] and A.variants(32, 4, 16) == [] and A.variants(16, 4, 32) == []
+    assert A.variants(4, 1, 32) == [A.TILE_IN4] and A.variants(4, 2, 32) == [] and A.variants(4, 0, 3) == []
     assert A.candidates(32, 2, 32, 32) == [(1, 8), (1, 16), (1, 32), (2, 8), (2, 16), (2, 32)]
     assert len(A.candidates(32, 0, 3, 256)) == 10
     t = A.ConvTuner()

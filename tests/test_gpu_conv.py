@@ -227,10 +227,11 @@ def test_autotuned_localnet_matches_default_tiles(dev, tmp_path):
     tuner = net.autotune(B, G, reps=2)
     rep = tuner.report()
     # 6 forward calls: conv_in, 4 residual layers with one shape, conv_out; backward: the output
-    # layer's data gradient, the residual layers' (with addend); three weight-gradient shapes
+    # layer's data gradient, the residual layers' (with addend); the 4-channel input layer of the
+    # inference path; two weight-gradient shapes
     fwd = [k for k in rep if k.startswith("fwd:")]
     wg = [k for k in rep if k.startswith("wgrad:")]
-    assert len(fwd) == 5 and len(wg) == 2, sorted(rep)
+    assert len(fwd) == 6 and len(wg) == 2, sorted(rep)
     for k, v in rep.items():
         assert v["best"] in v["ms"] and min(v["ms"].values()) == v["ms"][v["best"]], (k, v)
     y_t = net.forward(x, save=True)
@@ -267,3 +268,30 @@ def test_fused_gelu_backward_data_gradient(dev, with_add):
     d = (dz4.float() - dz3.float()).abs()
     assert float(d.max()) <= float((2.0 ** -7 * dz3.float().abs()).max()) + 1e-30
     assert float((dz4 != dz3).float().mean()) < 1e-3
+
+
+def test_input_layer_four_channels(dev):
+    """The 4-channel input layer (taps packed into the MFMA K dimension) against
+    the float64 conv, and against the same layer run on 16 padded channels."""
+    from augur_amd import autotune as A
+    from augur_amd import surrogate as S
+    g = torch.Generator().manual_seed(44)
+    B, G = 2, 64
+    x = bf16_round(torch.randn(B, 4, G, G, G, generator=g))
+    w = bf16_round(torch.randn(27, 32, 4, generator=g) * 0.1)
+    b = torch.randn(32, generator=g) * 0.1
+    ref = ref_conv(x, S.from_taps(w), b).permute(0, 2, 3, 4, 1)
+    x4, x16 = S.pack_cl(x.to(dev), 4), S.pack_cl(x.to(dev), 16)
+    w4, w16 = S.kernel_weights(w.to(dev), 4), S.kernel_weights(w.to(dev), 16)
+    z4 = torch.empty(B, G, G, G, 32, dtype=torch.bfloat16, device=dev)
+    z16 = torch.empty_like(z4)
+    h16 = S.conv3_fwd(x16, w16, b.to(dev), S.MODE_GELU, 32, z_out=z16)
+    first = None
+    for tile in A.candidates(4, S.MODE_GELU, 32, G):
+        h4 = S.conv3_fwd(x4, w4, b.to(dev), S.MODE_GELU, 32, z_out=z4, tile=tile)
+        assert (z4.cpu().double() - ref).abs().max() < 1e-2 * ref.abs().max()
+        assert (h4.cpu().double() - F.gelu(ref, approximate='tanh')).abs().max() < 1e-2 * ref.abs().max()
+        assert (h4.float() - h16.float()).abs().max() <= 2 ** -7 * h16.float().abs().max()
+        if first is None:
+            first = h4.clone()
+        assert torch.equal(h4, first), tile
