@@ -14,7 +14,6 @@ Modules
               solve with spectral gradient, KDK leapfrog, force-error metrics.
 ``direct``    FP64 softened direct sum (isolated, the reference's exact_acc),
               FP64 Ewald sum (periodic), energies and momenta.
-``scenes``    seeded 3D Plummer / clumpy / uniform scene generators.
 ``torch_cpu`` PyTorch-CPU FP32 port of the plain PM step (the CPU baseline)
               and the FP64 conv3d reference used for the surrogate network.
 

@@ -257,17 +257,21 @@ def pm_acc_3d_periodic(pos, mass, G, L=1.0, Gconst=1.0):
     return cic_gather(acc[None], pos, h, 0.0, True)[0]
 
 
-def kdk_pm_rollout(pos, vel, mass, G, dt, steps, L=1.0, Gconst=1.0):
+def kdk_pm_rollout(pos, vel, mass, G, dt, steps, L=1.0, Gconst=1.0, force=None):
     """KDK leapfrog with the periodic PM force (restating the integrator of
     scripts/gravity_sim.py:31-44 with substeps=1 and the force replaced by PM).
-    FP64 state; positions wrapped into [0, L)."""
+    FP64 state; positions wrapped into [0, L).  ``force`` (pos -> acc)
+    replaces the PM force when given (the integrator's own pins use it)."""
+    if force is None:
+        def force(p):
+            return pm_acc_3d_periodic(p, mass, G, L, Gconst)
     pos = np.asarray(pos, dtype=np.float64).copy()
     vel = np.asarray(vel, dtype=np.float64).copy()
-    a = pm_acc_3d_periodic(pos, mass, G, L, Gconst)
+    a = force(pos)
     for _ in range(steps):
         vel += 0.5 * dt * a
         pos = np.mod(pos + dt * vel, L)
-        a = pm_acc_3d_periodic(pos, mass, G, L, Gconst)
+        a = force(pos)
         vel += 0.5 * dt * a
     return pos, vel
 

@@ -95,3 +95,41 @@ def localnet3d(x, weights, biases):
     for w, b in zip(weights[1:-1], biases[1:-1]):
         h = h + F.gelu(conv3d_periodic(h, w, b), approximate='tanh')
     return conv3d_periodic(h, weights[-1], biases[-1])
+
+
+def pm_acc_2d(pos, mass, G, extent, eps=0.5):
+    """The reference's 2D plain PM force (scripts/pm_baseline_force_error.py:68-76)
+    on the CPU: CIC deposit with off-grid corners dropped (scripts/scatter_field.py:
+    190-211,237-250), zero-padded rfft2 convolution with -1/sqrt(r^2 + eps^2) on
+    the (2G)^2 domain times h^2, crop, zero-padded central differences
+    (:551-555), CIC gather (:252-261).  pos (N, 2) FP32 on a grid covering
+    [-extent/2, extent/2)^2, mass (N,) -> (N, 2)."""
+    h = extent / G
+    u = (pos + extent / 2) / h - 0.5
+    i0 = torch.floor(u)
+    w = u - i0
+    i0 = i0.long()
+    corners = []
+    for ox in (0, 1):
+        for oy in (0, 1):
+            ix, iy = i0[:, 0] + ox, i0[:, 1] + oy
+            wt = (w[:, 0] if ox else 1 - w[:, 0]) * (w[:, 1] if oy else 1 - w[:, 1])
+            ok = ((ix >= 0) & (ix < G) & (iy >= 0) & (iy < G)).to(pos.dtype)
+            corners.append((iy.clamp(0, G - 1) * G + ix.clamp(0, G - 1), wt * ok))
+    rho = torch.zeros(G * G, dtype=pos.dtype)
+    val = mass / h ** 2
+    for flat, wt in corners:
+        rho.index_add_(0, flat, val * wt)
+    rho = rho.view(G, G)
+    idx = torch.arange(2 * G, dtype=pos.dtype)
+    d = torch.minimum(idx, 2 * G - idx) * h
+    K = -1.0 / torch.sqrt(d[:, None] ** 2 + d[None, :] ** 2 + eps ** 2)
+    full = torch.fft.irfft2(torch.fft.rfft2(rho, s=(2 * G, 2 * G)) * torch.fft.rfft2(K), s=(2 * G, 2 * G)) * h ** 2
+    p = F.pad(full[:G, :G][None, None], (1, 1, 1, 1))[0, 0]
+    gx = -(p[1:-1, 2:] - p[1:-1, :-2]) / (2 * h)
+    gy = -(p[2:, 1:-1] - p[:-2, 1:-1]) / (2 * h)
+    acc = torch.stack([gx.reshape(-1), gy.reshape(-1)], -1)
+    out = torch.zeros_like(pos)
+    for flat, wt in corners:
+        out += acc[flat] * wt[:, None]
+    return out

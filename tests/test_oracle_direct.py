@@ -1,4 +1,4 @@
-"""Pins for oracle.direct and oracle.scenes: reference fixtures
+"""Pins for oracle.direct: reference fixtures
 (web/scatter/tests/truth_*.json, the two-body answer in verify.mjs), closed
 forms and invariants."""
 import json
@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle import direct, scenes
+from oracle import direct
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -84,22 +84,3 @@ def test_ewald_softening_only_changes_near_pairs():
     a1 = direct.ewald_acc(pos, np.ones(2), [0], eps=0.05)
     d_soft = 0.3 / (0.09 + 0.0025) ** 1.5 - 0.3 / 0.09 ** 1.5
     np.testing.assert_allclose(a1[0, 0] - a0[0, 0], d_soft, rtol=1e-9)
-
-
-def test_plummer_mass_profile():
-    rng = np.random.default_rng(4)
-    a = 0.05
-    p = scenes.plummer(20000, a, rng)
-    r = np.linalg.norm(p - 0.5, axis=1)
-    assert r.max() < 0.45
-    for rr in (0.02, 0.05, 0.1, 0.2):
-        expect = (rr ** 3 / (rr ** 2 + a ** 2) ** 1.5) / (0.45 ** 3 / (0.45 ** 2 + a ** 2) ** 1.5)
-        assert abs((r < rr).mean() - expect) < 0.012
-    c = p.mean(0) - 0.5
-    assert np.abs(c).max() < 0.005
-
-
-def test_clumpy_layout():
-    rng = np.random.default_rng(5)
-    p = scenes.clumpy(8192, rng)
-    assert p.shape == (8192, 3) and p.min() >= 0 and p.max() < 1
