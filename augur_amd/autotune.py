@@ -41,8 +41,10 @@ def candidates(cin, mode, cout, G):
     return [(v, zc) for v in variants(cin, mode, cout) for zc in ZC_CANDIDATES if zc <= G]
 
 
-def fwd_key(cin, mode, cout, B, G, addend):
-    return f"fwd:cin{cin}:mode{mode}:cout{cout}:add{int(bool(addend))}:B{B}:G{G}"
+def fwd_key(cin, mode, cout, B, G, addend, want_y=True):
+    """Table key of a forward call; the data gradient that writes dz only
+    (want_y=False, no FP32 dh store) is tuned apart from the full call."""
+    return f"fwd:cin{cin}:mode{mode}:cout{cout}:add{int(bool(addend))}:B{B}:G{G}" + ("" if want_y else ":noy")
 
 
 def wgrad_key(cin, B, G):
@@ -82,8 +84,8 @@ class ConvTuner:
             self.load(path)
 
     # ------------------------------------------------------------ lookup
-    def fwd_tile(self, cin, mode, cout, B, G, addend=False):
-        t = self.table.get(fwd_key(cin, mode, cout, B, G, addend))
+    def fwd_tile(self, cin, mode, cout, B, G, addend=False, want_y=True):
+        t = self.table.get(fwd_key(cin, mode, cout, B, G, addend, want_y))
         return None if t is None else (int(t[0]), int(t[1]))
 
     def wgrad_nwg(self, cin, B, G):
@@ -91,18 +93,18 @@ class ConvTuner:
         return None if t is None else int(t)
 
     # ------------------------------------------------------------ tuning
-    def tune_fwd(self, x, w, bias, mode, cout, z_out=None, addend=None, z_in=None):
+    def tune_fwd(self, x, w, bias, mode, cout, z_out=None, addend=None, z_in=None, want_y=True):
         """Time every tile for this layer on these tensors; returns the best (family, zc)."""
         from .surrogate import conv3_fwd
         B, G, cin = x.shape[0], x.shape[1], x.shape[-1]
-        key = fwd_key(cin, mode, cout, B, G, addend is not None)
+        key = fwd_key(cin, mode, cout, B, G, addend is not None, want_y)
         if key in self.table:
-            return self.fwd_tile(cin, mode, cout, B, G, addend is not None)
+            return self.fwd_tile(cin, mode, cout, B, G, addend is not None, want_y)
         stream = torch.cuda.current_stream(x.device)
         res = {}
         for tile in candidates(cin, mode, cout, G):
             res[tile] = _time(lambda: conv3_fwd(x, w, bias, mode, cout, z_out=z_out, addend=addend, tile=tile,
-                                                z_in=z_in), self.reps, stream)
+                                                z_in=z_in, want_y=want_y), self.reps, stream)
         best = min(res, key=res.get)
         self.table[key] = list(best)
         self.timings[key] = {f"{FAMILY[v]}/zc{zc}": ms for (v, zc), ms in res.items()}

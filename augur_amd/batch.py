@@ -24,6 +24,7 @@ import torch
 
 from . import _lib, ops
 from .ops import _dev, _p, _stream
+from .surrogate import pack_cl
 
 PHASE_GATHER, PHASE_OPEN, PHASE_CLOSE, PHASE_CLOSE_OPEN = 0, 1, 2, 3
 
@@ -83,8 +84,8 @@ class RaggedStepper:
             scene_density = self.scene_density(starts, mass, pos.shape[0])
         contrast = (rho / scene_density.view(-1, 1, 1, 1) - 1.0) * sur.rho_scale
         x = torch.cat([contrast[:, None], force / sur.force_scale], 1).contiguous()
-        y = sur.net.forward(x)
-        return torch.add(force, y, alpha=sur.force_scale).contiguous()
+        # the output layer adds force_scale * net(x) into the PM field in its epilogue
+        return sur.net.forward_into(pack_cl(x, sur.net.cinp), force, sur.force_scale, accumulate=True)
 
     def scene_density(self, starts, mass, total):
         """(B,) mass / L^3 of every scene, on the device: differences of the

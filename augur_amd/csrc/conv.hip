@@ -60,6 +60,8 @@ enum ConvMode : int {
   CONV_OUT_F32_CL = 3,   // y[vox][co] = conv (+ addend[vox][co])                (FP32 channels-last, no bias)
   CONV_OUT_F32_CL_DGELU = 4,  // as 3 (32 -> 32), and dz[vox][co] = BF16(y * gelu'(z_in[vox][co])): the next
                               // layer's GELU backward fused into this data gradient (z_out: z_in, h_out: dz)
+  CONV_OUT_F32_CF_INTO = 5,   // y[b][co][vox] = alpha (conv + bias) (+ y[b][co][vox] when accumulating), FP32
+                              // channels-first in place; the launch does nothing while *skip != 0
 };
 
 // ------------------------------------------------------- z-marching forward
@@ -414,8 +416,10 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
                                                                 const uint16_t* __restrict__ w,
                                                                 const float* __restrict__ bias, int G, int zc, int cout,
                                                                 float* __restrict__ y_out,
-                                                                const float* __restrict__ addend) {
+                                                                const float* __restrict__ addend, float alpha,
+                                                                int accumulate, const uint32_t* __restrict__ skip) {
   constexpr int CIN = 32, CP = CIN + 8;
+  if (MODE == CONV_OUT_F32_CF_INTO && skip && *skip != 0u) return;  // uniform: the whole grid exits
   constexpr int PLANE = ZM_PY * ZM_PX * CP;
   constexpr int CHUNKS = ZM_PY * ZM_PX * (CIN / 8);
   constexpr int PER = (CHUNKS + ZS_THREADS - 1) / ZS_THREADS;
@@ -515,6 +519,9 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
           const float val = acc[r][q] + sbias[co];
           if (MODE == CONV_OUT_F32_CF) {
             y_out[((int64_t)b * cout + co) * G3 + sv] = val;
+          } else if (MODE == CONV_OUT_F32_CF_INTO) {
+            float* dst = &y_out[((int64_t)b * cout + co) * G3 + sv];
+            *dst = accumulate ? fmaf(alpha, val, *dst) : alpha * val;
           } else {
             const int64_t o = ((int64_t)b * G3 + sv) * cout + co;
             y_out[o] = addend ? val + addend[o] : val;
@@ -1102,6 +1109,7 @@ int conv_variant_mask(int cin, int mode, int cout) {
     m |= 1 << TILE_SPLIT;
   if (cin == 32 && cout <= 16 && (mode == CONV_OUT_F32_CF || mode == CONV_OUT_F32_CL)) m |= 1 << TILE_NARROW;
   if (mode == CONV_OUT_F32_CL_DGELU) m = (cin == 32 && cout == 32) ? 1 << TILE_SPLIT : 0;  // split kernel only
+  if (mode == CONV_OUT_F32_CF_INTO) m = (cin == 32 && cout <= 16) ? 1 << TILE_NARROW : 0;  // narrow kernel only
   return m;
 }
 
@@ -1114,26 +1122,27 @@ int conv_default_variant(int mask) {
 
 template <int CIN, int MODE>
 int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t B, int64_t G, int cout, uint16_t* z,
-               uint16_t* h, float* y, const float* addend, int variant, int zc, hipStream_t s, const char* name) {
+               uint16_t* h, float* y, const float* addend, int variant, int zc, hipStream_t s, const char* name,
+               float alpha = 1.0f, int accumulate = 0, const uint32_t* skip = nullptr) {
   const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / zc);
   AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "conv: grid too large");
   TimedScope ts_(name, s);
   if (variant == TILE_SPLIT) {
-    if constexpr (CIN == 32 && MODE != CONV_OUT_F32_CF) {
+    if constexpr (CIN == 32 && MODE != CONV_OUT_F32_CF && MODE != CONV_OUT_F32_CF_INTO) {
       hipLaunchKernelGGL((k_conv3_zm_split<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias,
                          (int)G, zc, z, h, y, addend);
       AUGUR_LAUNCHED(name);
       return AUGUR_OK;
     }
   } else if (variant == TILE_NARROW) {
-    if constexpr (CIN == 32 && (MODE == CONV_OUT_F32_CF || MODE == CONV_OUT_F32_CL)) {
+    if constexpr (CIN == 32 && (MODE == CONV_OUT_F32_CF || MODE == CONV_OUT_F32_CL || MODE == CONV_OUT_F32_CF_INTO)) {
       hipLaunchKernelGGL((k_conv3_zm_narrow<MODE>), dim3((unsigned)blocks), dim3(ZS_THREADS), 0, s, x, w, bias,
-                         (int)G, zc, cout, y, addend);
+                         (int)G, zc, cout, y, addend, alpha, accumulate, skip);
       AUGUR_LAUNCHED(name);
       return AUGUR_OK;
     }
   } else if (variant == TILE_ZM) {
-    if constexpr (MODE != CONV_OUT_F32_CL_DGELU) {
+    if constexpr (MODE != CONV_OUT_F32_CL_DGELU && MODE != CONV_OUT_F32_CF_INTO) {
       if constexpr (CIN == 16) {  // 108 weight VGPRs: eight waves of 2 rows fit two waves per SIMD
         hipLaunchKernelGGL((k_conv3_zm<CIN, MODE, 2>), dim3((unsigned)blocks), dim3(64 * (ZM_Y / 2)), 0, s, x, w,
                            bias, (int)G, zc, cout, z, h, y, addend);
@@ -1154,7 +1163,7 @@ int launch_fwd(const uint16_t* x, const uint16_t* w, const float* bias, int64_t 
 extern "C" {
 
 int augur_conv3_tile_variants(int cin, int mode, int cout) {
-  if ((cin != 4 && cin != 16 && cin != 32) || mode < CONV_OUT_F32_CF || mode > CONV_OUT_F32_CL_DGELU || cout < 1 || cout > 32)
+  if ((cin != 4 && cin != 16 && cin != 32) || mode < CONV_OUT_F32_CF || mode > CONV_OUT_F32_CF_INTO || cout < 1 || cout > 32)
     return 0;
   if (mode == CONV_RES_GELU && cin != 32) return 0;
   return conv_variant_mask(cin, mode, cout);
@@ -1219,6 +1228,19 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
       set_error("conv: unknown mode %d", mode);
       return AUGUR_EINVAL;
   }
+}
+
+int augur_conv3_bf16_fwd_into(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B,
+                              int64_t G, int cout, float alpha, int accumulate, float* field, const uint32_t* skip,
+                              int zc, void* stream) {
+  AUGUR_REQUIRE(conv_tiles_ok(G), AUGUR_EUNSUPPORTED, "conv: G must be a power of two in [32, 1024]");
+  AUGUR_REQUIRE(B >= 1 && x && w && bias && field, AUGUR_EINVAL, "conv_into: bad arguments");
+  AUGUR_REQUIRE(cin == 32 && cout >= 1 && cout <= 16, AUGUR_EUNSUPPORTED, "conv_into: needs cin 32, cout 1..16");
+  if (zc == 0) zc = (int)(G < ZM_ZC ? G : ZM_ZC);
+  AUGUR_REQUIRE(is_pow2(zc) && zc >= 2 && zc <= G, AUGUR_EINVAL, "conv: zc must be a power of two in [2, G]");
+  return launch_fwd<32, CONV_OUT_F32_CF_INTO>(x, w, bias, B, G, cout, nullptr, nullptr, field, nullptr, TILE_NARROW,
+                                               zc, as_stream(stream), "conv_fwd_out", alpha, accumulate ? 1 : 0,
+                                               skip);
 }
 
 int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,

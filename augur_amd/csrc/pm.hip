@@ -684,7 +684,8 @@ template <int N>
 __global__ __launch_bounds__(FFT_THREADS) void k_pm_xc2r(const float2* __restrict__ s0,
                                                          const float2* __restrict__ s1,
                                                          const float2* __restrict__ s2, float kf,
-                                                         float* __restrict__ force) {
+                                                         float* __restrict__ force, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   using C = PmRow<N>;
   __shared__ float2 X[C::NL * C::LS];
   __shared__ float2 tw[N];
@@ -1421,10 +1422,107 @@ static int pm_force_grid(const float* pos, const uint32_t* perm, const float* ma
   {
     TimedScope ts_("pm_xc2r", s);
     PM_SWITCH_G(G, k_pm_xc2r, dim3((unsigned)(G * G / PmRow<GG>::NL)), dim3(FFT_THREADS), s, (const float2*)b.s0,
-                (const float2*)b.s1, (const float2*)b.s2, kf, b.force);
+                (const float2*)b.s1, (const float2*)b.s2, kf, b.force, (const uint32_t*)nullptr);
     AUGUR_LAUNCHED("pm_xc2r");
   }
   return AUGUR_OK;
+}
+
+// ------------------------------------------------- surrogate residual gate
+// Per-step check of a learned force field against plain PM at probe
+// particles, decided on the device (no host synchronisation, capturable).
+// gate[0]  fallback flag, sticky within a call: the correction conv skips while set
+// gate[1]  revert flag of the current step: the force planes are re-made from
+//          the PM spectra still in the workspace (augur_pm_force_replay)
+// gate[2]  step counter of the call; gate[3] first step that fell back (~0: none)
+constexpr int GATE_THREADS = 1024;
+
+template <bool P2>
+__device__ __forceinline__ void gather_point(const float* __restrict__ force, int G, float h, float inv_h,
+                                             const float* p, float a[3]) {
+  const int64_t G3 = (int64_t)G * G * G;
+  int c[3];
+  float w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float uu = P2 ? cic_u_pow2(p[d], inv_h) : cic_u(p[d], 0.0f, h);
+    const float f = floorf(uu);
+    c[d] = (int)f;
+    w[d] = __fsub_rn(uu, f);
+  }
+  a[0] = a[1] = a[2] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+    const float wt = (ox ? w[0] : 1.0f - w[0]) * (oy ? w[1] : 1.0f - w[1]) * (oz ? w[2] : 1.0f - w[2]);
+    const int64_t cell = ((int64_t)wrap(c[2] + oz, G) * G + wrap(c[1] + oy, G)) * G + wrap(c[0] + ox, G);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[d] = fmaf(wt, force[d * G3 + cell], a[d]);
+  }
+}
+
+template <bool P2>
+__global__ __launch_bounds__(GATE_THREADS) void k_sur_gate(int phase, const float* __restrict__ force,
+                                                           const float* __restrict__ pos,
+                                                           const int64_t* __restrict__ probe, int P, int G, float h,
+                                                           float inv_h, float* __restrict__ ref,
+                                                           uint32_t* __restrict__ gate,
+                                                           const float* __restrict__ thr,
+                                                           float* __restrict__ log, int nlog) {
+  const int tid = threadIdx.x;
+  if (phase == 0) {
+    if (tid < 4) gate[tid] = tid == 3 ? 0xffffffffu : 0u;
+    return;
+  }
+  double num = 0.0, den = 0.0;
+  for (int i = tid; i < P; i += GATE_THREADS) {
+    const int64_t k = probe[i];
+    const float p[3] = {pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
+    float a[3];
+    gather_point<P2>(force, G, h, inv_h, p, a);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (phase == 1) {
+        ref[3 * i + d] = a[d];
+      } else {
+        const double r = (double)ref[3 * i + d], e = (double)a[d] - r;
+        num += e * e;
+        den += r * r;
+      }
+    }
+  }
+  if (phase == 1) {
+    if (tid == 0) gate[1] = 0u;
+    return;
+  }
+  __shared__ double sn[GATE_THREADS / 64], sd[GATE_THREADS / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    num += __shfl_down(num, o);
+    den += __shfl_down(den, o);
+  }
+  if ((tid & 63) == 0) {
+    sn[tid >> 6] = num;
+    sd[tid >> 6] = den;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double tn = 0.0, td = 0.0;
+    for (int k = 0; k < GATE_THREADS / 64; ++k) {
+      tn += sn[k];
+      td += sd[k];
+    }
+    const uint32_t st = gate[2];
+    const double res = td > 0.0 ? sqrt(tn / td) : (tn > 0.0 ? INFINITY : 0.0);
+    if ((int)st < nlog) log[st] = (float)res;
+    const float t = thr[(int)st < nlog ? st : nlog - 1];
+    if (gate[0] == 0u && res > (double)t) {
+      gate[0] = 1u;
+      gate[1] = 1u;
+      gate[3] = st;
+    }
+    gate[2] = st + 1u;
+  }
 }
 
 }  // namespace augur
@@ -1629,6 +1727,46 @@ int augur_pm_kdk_diag(float* pos, float* vel, float* acc, float* mass, int32_t* 
                      ws_bytes, stream);
     if (rc) return rc;
   }
+  return AUGUR_OK;
+}
+
+int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, void* ws, size_t ws_bytes,
+                          void* stream) {
+  int rc = check_pm(n, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(gate, AUGUR_EINVAL, "pm_force_replay: null gate");
+  hipStream_t s = as_stream(stream);
+  Carver c(ws, ws_bytes);
+  PmBufs b;
+  AUGUR_REQUIRE(carve_pm(c, n, G, b), AUGUR_EWORKSPACE, "pm_force_replay: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
+  const double PI = 3.14159265358979323846;
+  const float kf = (float)(2.0 * PI / (double)L);
+  TimedScope ts_("pm_force_replay", s);
+  PM_SWITCH_G(G, k_pm_xc2r, dim3((unsigned)(G * G / PmRow<GG>::NL)), dim3(FFT_THREADS), s, (const float2*)b.s0,
+              (const float2*)b.s1, (const float2*)b.s2, kf, b.force, gate);
+  AUGUR_LAUNCHED("pm_force_replay");
+  return AUGUR_OK;
+}
+
+int augur_sur_gate(int phase, const float* force, const float* pos, const int64_t* probe, int64_t P, int64_t G,
+                   float L, float* ref, uint32_t* gate, const float* thr, float* log, int nlog, void* stream) {
+  AUGUR_REQUIRE(gate && phase >= 0 && phase <= 2, AUGUR_EINVAL, "sur_gate: bad phase or null gate");
+  AUGUR_REQUIRE(is_pow2(G) && G >= 2 && G <= 4096 && L > 0.0f, AUGUR_EINVAL, "sur_gate: bad grid");
+  if (phase >= 1)
+    AUGUR_REQUIRE(force && pos && probe && ref && P >= 1 && P < ((int64_t)1 << 31), AUGUR_EINVAL,
+                  "sur_gate: bad probe arguments");
+  if (phase == 2) AUGUR_REQUIRE(thr && log && nlog >= 1, AUGUR_EINVAL, "sur_gate: phase 2 needs thr, log");
+  hipStream_t s = as_stream(stream);
+  const float h = (float)((double)L / (double)G);
+  TimedScope ts_("sur_gate", s);
+  if (pow2_float(h))
+    hipLaunchKernelGGL(k_sur_gate<true>, dim3(1), dim3(GATE_THREADS), 0, s, phase, force, pos, probe, (int)P, (int)G,
+                       h, 1.0f / h, ref, gate, thr, log, nlog);
+  else
+    hipLaunchKernelGGL(k_sur_gate<false>, dim3(1), dim3(GATE_THREADS), 0, s, phase, force, pos, probe, (int)P,
+                       (int)G, h, 0.0f, ref, gate, thr, log, nlog);
+  AUGUR_LAUNCHED("sur_gate");
   return AUGUR_OK;
 }
 

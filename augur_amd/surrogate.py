@@ -200,9 +200,11 @@ class LocalNet3D:
         tile = None
         if self.tuner is not None:
             if self._tuning:
-                tile = self.tuner.tune_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, z_in=z_in)
+                tile = self.tuner.tune_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, z_in=z_in,
+                                           want_y=want_y)
             else:
-                tile = self.tuner.fwd_tile(x.shape[-1], mode, cout, x.shape[0], x.shape[1], addend is not None)
+                tile = self.tuner.fwd_tile(x.shape[-1], mode, cout, x.shape[0], x.shape[1], addend is not None,
+                                           want_y=want_y)
         return conv3_fwd(x, w, b, mode, cout, z_out=z_out, addend=addend, tile=tile, z_in=z_in, want_y=want_y)
 
     def _wgrad(self, x, dz):
@@ -247,8 +249,36 @@ class LocalNet3D:
 
     def forward_packed(self, xc, save=False):
         """xc: BF16 channels-last (B, G, G, G, cinp) input -> FP32 (B, cout, G, G, G).
-        A 4-channel input (inference, cin <= 4) runs the input layer with the
-        taps packed into the MFMA K dimension (k_conv3_in4)."""
+        A 4-channel input (inference only, cin <= 4) runs the input layer with
+        the taps packed into the MFMA K dimension (k_conv3_in4); training
+        (save=True) needs the cinp-channel image, whose weight gradient it reads."""
+        h = self._hidden(xc, save)
+        y = self._conv(h, self.wk[-1], self.b[-1], MODE_OUT_CF, self.cout)
+        return y
+
+    def forward_into(self, xc, field, alpha, accumulate=True, skip=None):
+        """Inference straight into a force field: field (B, cout, G, G, G) FP32
+        becomes alpha * net(xc) + field (accumulate) or alpha * net(xc); the
+        output layer's epilogue does the update (augur_conv3_bf16_fwd_into),
+        and does nothing while the int32 word ``skip`` is non-zero."""
+        assert self.cout <= 16 and field.dtype == torch.float32 and field.is_contiguous()
+        h = self._hidden(xc, False)
+        B, G = h.shape[0], h.shape[1]
+        assert field.numel() == B * self.cout * G ** 3
+        zc = 0
+        if self.tuner is not None:
+            t = self.tuner.fwd_tile(32, MODE_OUT_CF, self.cout, B, G)
+            if t is not None and t[0] == 3:    # the NARROW family: the same kernel, other epilogue
+                zc = t[1]
+        dev = _dev(h, field)
+        _lib.call("augur_conv3_bf16_fwd_into", _p(h), 32, _p(self.wk[-1]), _p(self.b[-1]), B, G, self.cout,
+                  float(alpha), int(bool(accumulate)), _p(field), _p(skip), int(zc), _stream(dev))
+        return field
+
+    def _hidden(self, xc, save):
+        if save and xc.shape[-1] == 4 and self.cinp != 4:
+            raise ValueError("the 4-channel input layer is inference-only: train on the "
+                             f"{self.cinp}-channel image (pack_cl)")
         zs, hs = [], [xc]
         need_z = save
         z = torch.empty(xc.shape[:-1] + (32,), dtype=BF16, device=xc.device) if need_z else None
@@ -261,10 +291,9 @@ class LocalNet3D:
             h = self._conv(h, self.wk[k], self.b[k], MODE_RES_GELU, 32, z_out=z)
             zs.append(z)
             hs.append(h)
-        y = self._conv(h, self.wk[-1], self.b[-1], MODE_OUT_CF, self.cout)
         if save:
             self._saved = (zs, hs)
-        return y
+        return h
 
     def backward(self, dy):
         """dy: FP32 (B, cout, G, G, G) gradient of the loss w.r.t. the output.
@@ -306,6 +335,39 @@ class LocalNet3D:
 
 
 # --------------------------------------------------------------- PM surrogate
+class SurrogateGate:
+    """The device-side record of PMSurrogate.kdk's residual checks: gate words
+    [fallback, revert, steps checked, first step that fell back] and the
+    residual of every step.  Reading a field synchronises with the device."""
+
+    def __init__(self, gate, log, steps):
+        self.gate, self.log, self.steps = gate, log, steps
+
+    @property
+    def fell_back(self):
+        return bool(self.gate is not None and int(self.gate[0]) != 0)
+
+    @property
+    def fallback_step(self):
+        """The first step that fell back, or None."""
+        if self.gate is None or int(self.gate[0]) == 0:
+            return None
+        return int(self.gate[3])
+
+    @property
+    def residuals(self):
+        return None if self.log is None else self.log[: self.steps].cpu().tolist()
+
+    @property
+    def residual(self):
+        """Largest per-step residual of the call (None without probes)."""
+        r = self.residuals
+        return max(r) if r else None
+
+    def __iter__(self):   # fell_back, residual = sur.kdk(...)
+        return iter((self.fell_back, self.residual))
+
+
 def rel_rms(a, b):
     """sqrt(sum |a - b|^2 / sum |b|^2) over the rows (the force metric)."""
     a, b = a.double(), b.double()
@@ -339,20 +401,34 @@ class PMSurrogate:
         self.device = self.pm.device
         cin = 4 if mode == "correction" else 1
         self.net = LocalNet3D(cin=cin, cout=3, layers=layers, device=self.device, seed=seed, init_std=init_std)
-        # residual budget against plain PM: a replacement must stay close to PM;
-        # a correction is expected to move forces by about PM's own error
-        self.threshold = threshold if threshold is not None else (0.25 if mode == "correction" else 2e-2)
+        self.threshold = threshold   # None: check_threshold's rule
+        self.train_residual = None   # |a_sur - a_pm| / |a_pm| on the training targets (running mean)
         self.rho_scale = None    # 1 / rms of the density contrast
         self.force_scale = None  # rms of the PM force field
 
+    @property
+    def check_threshold(self):
+        """The residual bound of the fallback check.  A replacement must stay
+        within 2e-2 of plain PM.  A correction is meant to move the force by
+        about PM's own error, so it is bounded by twice the residual it showed
+        on its training targets (at least 2e-2): a larger one means the scene is
+        outside what it was trained on.  An explicit ``threshold`` wins."""
+        if self.threshold is not None:
+            return float(self.threshold)
+        if self.mode == "replace":
+            return 2e-2
+        return 0.25 if self.train_residual is None else max(2e-2, 2.0 * self.train_residual)
+
     def state_dict(self):
         return {"mode": self.mode, "net": self.net.state_dict(), "rho_scale": self.rho_scale,
-                "force_scale": self.force_scale, "threshold": self.threshold}
+                "force_scale": self.force_scale, "threshold": self.threshold,
+                "train_residual": self.train_residual}
 
     def load_state_dict(self, sd):
         self.net.load_state_dict(sd["net"])
         self.rho_scale, self.force_scale = sd["rho_scale"], sd["force_scale"]
         self.threshold = sd["threshold"]
+        self.train_residual = sd.get("train_residual")
 
     def calibrate(self, rho, force):
         """Input/output scales from one scene's grids (kept for later scenes)."""
@@ -403,7 +479,7 @@ class PMSurrogate:
             return a
         a_pm = self.gather(force, pos[probe])
         res = rel_rms(a[probe], a_pm)
-        if res > self.threshold:
+        if res > self.check_threshold:
             return self.gather(force, pos), True, res
         return a, False, res
 
@@ -412,42 +488,70 @@ class PMSurrogate:
         """Sorted device state of a scene (PeriodicPM.load; acc from plain PM)."""
         return self.pm.load(pos, vel, mass, mass_value)
 
-    def kdk(self, st, dt, steps, probe=None):
-        """`steps` KDK steps of the state with the surrogate force: each step
+    def kdk(self, st, dt, steps, probe=None, threshold=None):
+        """`steps` KDK steps of the state with the surrogate force.  Each step
         runs the PM half (re-sort, deposit, FFT) into the state's workspace,
-        the network on those grids, then the fused gather and kicks.  With
-        `probe` slot indices, the first step compares the surrogate with plain
-        PM at those particles and, past the threshold, the whole call runs
-        plain PM.  Returns (fell_back, residual or None)."""
-        fell_back, res = False, None
+        the network on those grids with its output layer writing straight into
+        the PM force planes (correction: += force_scale * net; replace:
+        = force_scale * net), then the fused gather and kicks.
+
+        With `probe` slot indices every step is checked against plain PM on
+        the device: the force planes are gathered at the probes before and
+        after the network, and when the residual
+        sqrt(sum |a - a_pm|^2 / sum |a_pm|^2) exceeds `threshold` (a float, or
+        one value per step; default self.threshold) the step's planes are
+        re-made from the PM spectra and the rest of the call runs plain PM.
+        Nothing synchronises with the host, so the call can be captured.
+        Returns a SurrogateGate (its fields synchronise when read)."""
+        G = self.G
+        dev = self.device
+        stream = _stream(dev)
+        gate = log = None
+        if probe is not None:
+            probe = probe.to(dev, torch.int64).contiguous()
+            P = probe.numel()
+            if not torch.cuda.is_current_stream_capturing():
+                ops._check_targets(probe, st.n)
+            thr = self.check_threshold if threshold is None else threshold
+            if isinstance(thr, (int, float)):   # a fill kernel: capturable
+                thr = torch.full((max(1, steps),), float(thr), dtype=torch.float32, device=dev)
+            else:
+                thr = torch.as_tensor(thr, dtype=torch.float32).reshape(-1).to(dev)
+                assert thr.numel() >= steps, "one threshold per step"
+            gate = torch.empty(4, dtype=torch.int32, device=dev)
+            log = torch.zeros(max(1, steps), dtype=torch.float32, device=dev)
+            ref = torch.empty(P, 3, dtype=torch.float32, device=dev)
+            _lib.call("augur_sur_gate", 0, None, None, None, 0, G, self.L, None, _p(gate), None, None, 0, stream)
         mean_density = None
         for s in range(steps):
             self.pm.step_begin(st, dt, s == 0)
             rho, force = self.pm.grids(st)
-            if not fell_back:
-                if self.mode == "correction" and self.rho_scale is not None:
-                    if mean_density is None:   # total mass / L^3, once per call
-                        mtot = float(st.mass.double().sum()) if st.mass is not None else st.mass_value * st.n
-                        mean_density = mtot / self.L ** 3
-                    y = self.net.forward_packed(self.packed_features(rho, force, mean_density))[0]
-                else:
-                    y = self.net.forward(self.features_from_grids(rho, force))[0] * self.force_scale
-                    if self.mode == "correction":
-                        y = y / self.force_scale
-                if s == 0 and probe is not None:
-                    pp = st.pos[probe].contiguous()
-                    a_pm = self.gather(force, pp)
-                    a_sur = self.gather(torch.add(force, y, alpha=self.force_scale) if self.mode == "correction"
-                                        else y, pp)
-                    res = rel_rms(a_sur, a_pm)
-                    fell_back = res > self.threshold
-                if not fell_back:
-                    if self.mode == "correction":
-                        force.add_(y, alpha=self.force_scale)
-                    else:
-                        force.copy_(y)
+            if gate is not None:
+                _lib.call("augur_sur_gate", 1, _p(force), _p(st.pos), _p(probe), P, G, self.L, _p(ref), _p(gate),
+                          None, None, 0, stream)
+            if self.rho_scale is None:
+                self.calibrate(rho, force)
+            if self.mode == "correction":
+                if mean_density is None:   # total mass / L^3
+                    mean_density = self._mass_total(st) / self.L ** 3
+                xc = self.packed_features(rho, force, mean_density)
+                self.net.forward_into(xc, force, self.force_scale, accumulate=True, skip=gate)
+            else:
+                xc = pack_cl(self.features_from_grids(rho, force), self.net.cinp)
+                self.net.forward_into(xc, force, self.force_scale, accumulate=False, skip=gate)
+            if gate is not None:
+                _lib.call("augur_sur_gate", 2, _p(force), _p(st.pos), _p(probe), P, G, self.L, _p(ref), _p(gate),
+                          _p(thr), _p(log), int(log.numel()), stream)
+                _lib.call("augur_pm_force_replay", st.n, G, self.L, _p(gate[1:]), _p(st.ws), st.ws.numel(), stream)
             self.pm.step_end(st, dt, s == steps - 1)
-        return fell_back, res
+        return SurrogateGate(gate, log, steps)
+
+    def _mass_total(self, st):
+        if st.mass is None:
+            return st.mass_value * st.n
+        if getattr(st, "mass_total", None) is None:   # once per state (one host read)
+            st.mass_total = float(st.mass.double().sum())
+        return st.mass_total
 
     def train_step(self, pos, targets, a_true, mass=None, mass_value=None, lr=3e-4):
         """One Adam step on the force loss sum |a - a_true|^2 / sum |a_true|^2
@@ -471,6 +575,9 @@ class PMSurrogate:
         norm = float((at ** 2).sum())
         r = a - at
         loss = float((r ** 2).sum()) / norm
+        if self.mode == "correction":
+            res = rel_rms(a, self.gather(force, pt))
+            self.train_residual = res if self.train_residual is None else 0.9 * self.train_residual + 0.1 * res
         g = (2.0 / norm) * r.float()
         dfield = ops.cic_deposit(pt[None], None, self.G, self.h, 0.0, True, vel=g[None], scale=1.0,
                                  mass_value=1.0)[0, 1:4]

@@ -38,7 +38,18 @@ def algorithmic_bytes(name, N, G):
     }.get(name)
 
 
-STAGES = {  # stage -> kernels (per-step algorithmic bytes are the sum of theirs)
+def compulsory_bytes(stage, N, G):
+    """Compulsory HBM bytes per step of each stage (every input read once and
+    every output written once): deposit 12N + 4G^3; FFT Poisson + gradient
+    R2C 4G^3 + S plus three C2R S + 4G^3 (S = 8 G^2 (G/2 + 1), the half
+    spectrum); gather + KDK 48N + 12G^3."""
+    G3 = G ** 3
+    spec = 8 * G * G * (G // 2 + 1)
+    return {"deposit": 12 * N + 4 * G3, "fft_poisson_grad": 16 * G3 + 4 * spec,
+            "gather_kdk": 48 * N + 12 * G3}.get(stage)
+
+
+STAGES = {  # stage -> kernels (their design bytes are the per-pass sums of algorithmic_bytes)
     "deposit": ("dep_bound", "dep_plan", "dep_tiles"),
     "fft_poisson_grad": ("fft_rows_r2c", "fft_cols_fwd", "pm_zpass", "pm_yinv", "pm_xc2r"),
     "gather_kdk": ("gather_kdk",),
@@ -87,10 +98,15 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
     pos0 = torch.from_numpy(pos_np).to(dev)
     rel_sur = None
     if oracle is not None:
+        # the force of the timed path: a zero-length surrogate step (4-channel packed input, the output
+        # layer adding into the PM planes, the fused gather), read back in the original order
         tg, a_ex = oracle
-        f, _ = sur.field(pos0, mass_value=1.0 / N)
-        rel_sur = rel_rms(sur.gather(f, pos0[tg].contiguous()), a_ex)
-        del f
+        st0 = sur.load(pos0, torch.zeros_like(pos0), mass_value=1.0 / N)
+        sur.kdk(st0, 0.0, 1)
+        a_sur = sur.pm.unload(st0)[2]
+        rel_sur = rel_rms(a_sur[tg], a_ex)
+        del st0, a_sur
+        torch.cuda.empty_cache()
         print(f"[bench] surrogate force rel-RMS vs FP64 Ewald: {rel_sur:.4e} (plain PM {pm_rel:.4e})",
               file=sys.stderr, flush=True)
 
@@ -108,15 +124,19 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
     _lib.timing_filter("conv_fwd_res")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fell_back, res = sur.kdk(st, a.dt, a.steps, probe=probe)
+    gate = sur.kdk(st, a.dt, a.steps, probe=probe)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     timing.update(_lib.timing_collect())
     _lib.timing_filter(None)
     _lib.timing_enable(False)
     value = N * a.steps / sec
+    fell_back, residuals = gate.fell_back, gate.residuals
+    res = max(residuals)
+    at_2e2 = next((k for k, r in enumerate(residuals) if r > 2e-2), None)
     print(f"[bench] surrogate {a.steps} steps in {sec * 1e3:.2f} ms -> {value:.4e} particle-steps/s "
-          f"(fell back: {fell_back}, probe residual {res:.3e})", file=sys.stderr, flush=True)
+          f"(threshold {sur.check_threshold:.3e}, fell back: {fell_back}, max probe residual {res:.3e}; "
+          f"a 2e-2 bound would fall back at step {at_2e2})", file=sys.stderr, flush=True)
     conv = {}
     # algorithmic flop per voxel (2 * 27 * cin * cout of the layer; the kernels pad the 4 input
     # channels of the first layer to 16 and the 3 outputs of the last to 16, which is not counted)
@@ -134,7 +154,12 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
         "speed_ratio_vs_plain_pm": value / pm_value,
         "force_rel_rms_vs_fp64": rel_sur,
         "rel_rms_ratio_vs_plain_pm": (rel_sur / pm_rel) if (rel_sur is not None and pm_rel) else None,
-        "fell_back": bool(fell_back), "probe_residual": res, "threshold": sur.threshold,
+        "fell_back": bool(fell_back), "probe_residual": res, "probe_residuals": residuals,
+        "fallback_check": {"threshold": sur.check_threshold, "probes": int(probe.numel()),
+                           "rule": "twice the correction's residual against PM on its training targets "
+                                   f"({sur.train_residual:.3e}), at least 2e-2",
+                           "per_step": True, "on_device": True,
+                           "at_2e-2_falls_back_at_step": at_2e2},
         "model": "PM + LocalNet3D correction (6 convs 3^3, width 32, BF16/FP32-acc), input [density contrast, "
                  "PM force], 3-channel correction",
         "train": {"scenes": a.sur_scenes, "steps": a.sur_train_steps, "lr": a.sur_lr, "targets": a.sur_targets,
@@ -164,24 +189,73 @@ def measured_traffic(kernel, N, G):
     return None
 
 
-def cpu_baseline(pos, vel, G, dt, steps=1):
-    """Time the PyTorch-CPU FP32 port of one KDK step on the full workload."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or None
+
+
+def cpu_kdk_steps(pos, vel, G, dt, steps):
+    """Median seconds per KDK step of the PyTorch-CPU FP32 port (after one warm-up step)."""
     import torch
+    from oracle import torch_cpu as tc
+    n = pos.shape[0]
+    p, v = torch.from_numpy(pos), torch.from_numpy(vel)
+    m = torch.full((n,), 1.0 / n)
+    a = tc.pm_acc(p, m, G)
+    p, v, a = tc.kdk_step(p, v, a, m, G, dt)
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        p, v, a = tc.kdk_step(p, v, a, m, G, dt)
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], sum(ts)
+
+
+def cpu_baseline(pos, vel, G, dt, steps=1, extra=True):
+    """The PyTorch-CPU FP32 port timed on this host (all cores): the workload's
+    KDK step, and (extra) the Plummer cold collapse over all its 100 steps and
+    the 2D reference-shape pm_acc (G=128, N=1e6).  Median per step after a
+    warm-up, the timing convention of scripts/scatter_bh_1b.py:296-300."""
+    import numpy as np
+    import torch
+    from augur_amd import scenes
     from oracle import torch_cpu as tc
     threads = os.cpu_count() or 1
     torch.set_num_threads(threads)
     n = pos.shape[0]
-    p = torch.from_numpy(pos)
-    v = torch.from_numpy(vel)
-    m = torch.full((n,), 1.0 / n)
-    a = tc.pm_acc(p, m, G)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        p, v, a = tc.kdk_step(p, v, a, m, G, dt)
-    sec = (time.perf_counter() - t0) / steps
-    return {"value": n / sec, "unit": "particle-steps/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{steps} KDK step(s) of the same N={n}, {G}^3 workload with the PyTorch-CPU FP32 "
-                      f"restatement (oracle/torch_cpu.py), {sec:.2f} s/step"}
+    sec, _ = cpu_kdk_steps(pos, vel, G, dt, steps)
+    out = {"value": n / sec, "unit": "particle-steps/s", "cores": torch.get_num_threads(), "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": f"{steps} KDK step(s) (after a warm-up step) of the same N={n}, {G}^3 workload with the "
+                     f"PyTorch-CPU FP32 restatement (oracle/torch_cpu.py), {sec:.2f} s/step"}
+    if extra:
+        p1, v1 = scenes.cfg1()
+        s1, tot = cpu_kdk_steps(p1, v1, 64, 1e-3, 100)
+        out["plummer_cold_collapse"] = {"N": len(p1), "grid": 64, "steps": 100, "s_per_step_median": s1,
+                                        "seconds_total": tot, "value": len(p1) / s1, "unit": "particle-steps/s"}
+        rng = np.random.default_rng(4738)
+        n2 = 1000000
+        p2 = torch.from_numpy(rng.normal(0.0, 8.0, (n2, 2)).astype(np.float32))
+        m2 = torch.ones(n2)
+        tc.pm_acc_2d(p2, m2, 128, 64.0)
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            tc.pm_acc_2d(p2, m2, 128, 64.0)
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        out["pm_acc_2d"] = {"N": n2, "grid": 128, "extent": 64.0, "ms_median": ts[2] * 1e3,
+                            "note": "the reference's 2D plain-PM force (scripts/pm_baseline_force_error.py:68-76) "
+                                    "on a Gaussian scene; 1.5 ms on the reference's GPU"}
+    return out
 
 
 def large_surrogate_train(a, N, G, NC, dev):
@@ -343,6 +417,108 @@ def large_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def cfg1_bench(a):
+    """The Plummer cold collapse: N = 65,536 equal masses (m = 1/N) in a
+    Plummer sphere of scale 0.05 centred in the unit periodic box, at rest,
+    64^3 grid, 100 KDK steps of dt = 1e-3.  Force rel-RMS of the initial PM
+    forces against FP64 sums with softening 1/64 on 4096 seeded targets: the
+    isolated direct sum (the reference's exact_acc) and the periodic Ewald sum
+    (the same boundary as the PM solve).  The 100 steps are timed as one
+    eager call and as a HIP-graph replay of the same call (the state restored
+    before each, outside the timed region)."""
+    import numpy as np
+    import torch
+    from augur_amd import _lib, ops, scenes
+    from augur_amd.pm import PeriodicPM
+
+    dev = torch.device("cuda:0")
+    N, G, dt, steps, eps = 65536, 64, 1e-3, 100, 1.0 / 64
+    pos_np, vel_np = scenes.cfg1()
+    pos, vel = torch.from_numpy(pos_np).to(dev), torch.from_numpy(vel_np).to(dev)
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    tg = torch.from_numpy(scenes.targets(N, 4096)).to(dev)
+    a_pm = pm.accel(pos, mass_value=1.0 / N)[tg].double()
+    m = torch.full((N,), 1.0 / N, dtype=torch.float64, device=dev)
+    a_dir = ops.direct_sum_f64(pos, eps, m, tg)
+    a_ew = ops.ewald_f64(pos, 1.0 / N, tg, L=1.0, eps=eps)
+    rr = lambda x, y: float((((x - y) ** 2).sum(-1).mean() / (y ** 2).sum(-1).mean()).sqrt())
+    rel_dir, rel_ew, rel_ew_dir = rr(a_pm, a_dir), rr(a_pm, a_ew), rr(a_ew, a_dir)
+    print(f"[bench] cfg1 PM force rel-RMS vs FP64 direct (isolated, eps 1/64) {rel_dir:.4e}, vs FP64 Ewald "
+          f"(periodic, eps 1/64) {rel_ew:.4e}; Ewald vs isolated {rel_ew_dir:.4e}", file=sys.stderr, flush=True)
+    # warm-up on a scratch state (kernel loads), then the timed calls
+    w = pm.load(pos, vel, mass_value=1.0 / N)
+    pm.kdk(w, dt, max(1, a.warmup))
+    del w
+    st = pm.load(pos, vel, mass_value=1.0 / N)
+    saved = [t.clone() for t in (st.pos, st.vel, st.acc, st.ids, st.ws)]
+    em0 = ops.energy_momentum_f64(st.vel, None, 1.0 / N)[:4]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pm.kdk(st, dt, steps)
+    torch.cuda.synchronize()
+    sec_eager = time.perf_counter() - t0
+    p_eager = st.pos.clone()
+    _lib.timing_enable(True)
+    for t, sv in zip((st.pos, st.vel, st.acc, st.ids, st.ws), saved):
+        t.copy_(sv)
+    pm.kdk(st, dt, steps)
+    torch.cuda.synchronize()
+    timing = _lib.timing_collect()
+    _lib.timing_enable(False)
+    for t, sv in zip((st.pos, st.vel, st.acc, st.ids, st.ws), saved):
+        t.copy_(sv)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        pm.kdk(st, dt, steps)
+    secs = []
+    for _ in range(3):
+        for t, sv in zip((st.pos, st.vel, st.acc, st.ids, st.ws), saved):
+            t.copy_(sv)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graph.replay()
+        torch.cuda.synchronize()
+        secs.append(time.perf_counter() - t0)
+    sec = sorted(secs)[1]
+    same = bool(torch.equal(st.pos, p_eager))
+    em1 = ops.energy_momentum_f64(st.vel, None, 1.0 / N)[:4]
+    p, v, _ = (t.cpu().double().numpy() for t in pm.unload(st))
+    r = np.linalg.norm(p - 0.5, axis=1)
+    value = N * steps / sec
+    print(f"[bench] cfg1 {steps} steps: graph replay {sec * 1e3:.2f} ms ({value:.4e} particle-steps/s), eager "
+          f"{sec_eager * 1e3:.2f} ms; replay equals eager: {same}", file=sys.stderr, flush=True)
+    kernels = {k: {"ms_total": vv[0], "launches": vv[1], "us_avg": vv[0] / vv[1] * 1e3} for k, vv in timing.items()}
+    cpu = None
+    if not a.no_cpu_baseline:
+        import torch as _t
+        _t.set_num_threads(os.cpu_count() or 1)
+        s1, tot = cpu_kdk_steps(pos_np, vel_np, G, dt, steps)
+        cpu = {"value": N / s1, "unit": "particle-steps/s", "cores": _t.get_num_threads(), "kind": "port",
+               "cpu_model": cpu_model(),
+               "sample": f"all {steps} KDK steps of the same workload with the PyTorch-CPU FP32 restatement "
+                         f"(oracle/torch_cpu.py): {tot:.2f} s, median {s1 * 1e3:.1f} ms/step"}
+        print(f"[bench] CPU baseline {cpu['value']:.3e} p-steps/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
+    out = {
+        "metric": "particle-steps/s, plain PM cold collapse", "value": value, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": sec * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Plummer cold collapse, N={N} (a=0.05, centred, v=0, m=1/N), {G}^3 periodic grid, "
+                               f"{steps} KDK steps dt={dt}", "N": N, "grid": G, "dt": dt, "path": "plain_pm"},
+        "force_rel_rms_vs_fp64": rel_dir,
+        "oracle": "FP64 isolated direct sum (eps=1/64) on 4096 seeded targets, initial scene",
+        "force_rel_rms_vs_fp64_ewald": rel_ew, "ewald_vs_isolated_rel_rms": rel_ew_dir,
+        "eager_ms_per_step": sec_eager * 1e3 / steps, "graph_replay_equals_eager": same,
+        "momentum_drift": (em1[:3] - em0[:3]).abs().cpu().tolist(),
+        "lagrangian_radii_final": {q: float(np.quantile(r, q)) for q in (0.1, 0.5, 0.9)},
+        "roofline": None, "roofline_note": "16.6 MB of compulsory bytes per step sit in the 256 MB Infinity "
+                                           "Cache; the step is launch-bound (about 30 launches), not HBM-bound",
+        "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def train_bench(a):
     """Batched surrogate training: 64 scenes per batch with particle counts
     drawn log-uniformly in [2^14, 2^20] (Plummer with a in [0.02, 0.1], or the
@@ -479,7 +655,7 @@ def batched_stepping(a, sur, batch, G, total, dev):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["default", "large", "train"], default="default",
+    ap.add_argument("--workload", choices=["default", "large", "train", "cfg1"], default="default",
                     help="large: N=2^30 on a 512^3 grid (4096 clumps), PM steps with FP64 reductions; "
                          "train: batched surrogate training on 64 ragged scenes at 128^3")
     ap.add_argument("--train-scenes", type=int, default=64)
@@ -509,6 +685,9 @@ def main():
 
     if a.workload == "train":
         train_bench(a)
+        return
+    if a.workload == "cfg1":
+        cfg1_bench(a)
         return
     if a.workload == "large":
         if a.steps == 10 and "--steps" not in sys.argv:
@@ -591,10 +770,12 @@ def main():
     stages = {}
     for sname, kern in STAGES.items():
         ms = sum(timing.get(k, (0.0, 0))[0] for k in kern) / a.steps
-        b = sum(algorithmic_bytes(k, N, G) or 0 for k in kern)
+        b = compulsory_bytes(sname, N, G)
+        design = sum(algorithmic_bytes(k, N, G) or 0 for k in kern)
         stages[sname] = {"ms_per_step": ms, "algorithmic_bytes": b,
                          "GB_s": b / (ms / 1e3) / 1e9 if ms > 0 and b else None,
-                         "frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS if ms > 0 and b else None}
+                         "frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS if ms > 0 and b else None,
+                         "design_bytes": design or None}
     kernels = {k: {"ms_total": v[0], "launches": v[1], "us_avg": v[0] / v[1] * 1e3} for k, v in timing.items()}
 
     sur = None

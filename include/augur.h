@@ -261,6 +261,30 @@ int augur_pm_step_end(float* pos, float* vel, float* acc, int64_t n, int64_t G, 
                       size_t ws_bytes, void* stream);
 int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* force_offset);
 
+/* The plain-PM fallback of a learned force step, decided on the device (no
+ * host synchronisation, so a k-step call can be captured in a HIP graph).
+ * gate: 4 uint32 words [fallback (sticky within a call), revert (this step),
+ * step counter, first step that fell back (~0u: none)].
+ * augur_sur_gate(phase, ...), one workgroup:
+ *   0  reset the gate (the start of a call);
+ *   1  ref[3 p ..] = CIC gather of the force planes (3 x G^3 FP32, the PM
+ *      field of augur_pm_step_begin) at pos[probe[p]], p < P; clear revert;
+ *   2  gather again (after the learned field was applied); the residual
+ *      sqrt(sum |a - ref|^2 / sum |ref|^2) (FP64 sums) goes to log[step];
+ *      above thr[step] (thr[nlog - 1] past the end) it sets fallback and
+ *      revert and records the step; the step counter advances.
+ * augur_pm_force_replay(): while gate[1] (pass &gate[1]) is set, re-makes
+ * the force planes from the PM spectra the step left in the workspace (the
+ * last FFT pass of augur_pm_step_begin again: the same bits); otherwise the
+ * launch exits at once.  Replaces the north star's "plain PM fallback when
+ * a residual check against it exceeds a threshold"; the reference has no
+ * such check (its nearest is the device-side cell_max_occ overflow test of
+ * the pair term, scripts/scatter_field.py:324-334). */
+int augur_sur_gate(int phase, const float* force, const float* pos, const int64_t* probe, int64_t P, int64_t G,
+                   float L, float* ref, uint32_t* gate, const float* thr, float* log, int nlog, void* stream);
+int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, void* ws, size_t ws_bytes,
+                          void* stream);
+
 /* One launch over a ragged batch of B scenes (B <= 1024): scene b holds
  * particles [starts[b], starts[b+1]) of pos/vel/acc (FP32 (total, 3), starts
  * an int64 device array of B+1 nondecreasing offsets, starts[B] = total), and
@@ -328,6 +352,19 @@ int augur_conv3_tile_variants(int cin, int mode, int cout);
 int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B, int64_t G,
                               int mode, int cout, uint16_t* z_out, uint16_t* h_out, float* y_out, const float* addend,
                               int variant, int zc, void* stream);
+
+/* The output layer written straight into a force field: field FP32
+ * channels-first [B][cout][G^3] becomes alpha (conv + bias) + field
+ * (accumulate != 0, fmaf per voxel) or alpha (conv + bias) (accumulate = 0),
+ * cin 32, cout <= 16, on the NARROW family with zc planes per workgroup (0:
+ * the default).  While skip (may be NULL) points at a non-zero word the launch
+ * does nothing.  Replaces the separate `force += scale * net(x)` of a
+ * correction net (the grid half of ScatterField._grid_force,
+ * scripts/scatter_field.py:657-662, where the net's field is added to the
+ * kernel acceleration) with no extra pass over the field. */
+int augur_conv3_bf16_fwd_into(const uint16_t* x, int cin, const uint16_t* w, const float* bias, int64_t B,
+                              int64_t G, int cout, float alpha, int accumulate, float* field, const uint32_t* skip,
+                              int zc, void* stream);
 
 /* Weight gradient dw[27][32][cin] = sum_v dz[v][co] x[v + o_tap][ci] and
  * db[32] = sum_v dz[v][co] (FP32), dz BF16 channels-last [B][G^3][32].

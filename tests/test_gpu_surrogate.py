@@ -117,7 +117,7 @@ def test_residual_check_falls_back_to_plain_pm(dev):
     sur = PMSurrogate(G, device=dev, mode="replace", init_std=0.5)
     probe = torch.arange(0, n, 5, device=dev)
     a, fell_back, res = sur.accel(pos, mass_value=1.0 / n, probe=probe)
-    assert fell_back and res > sur.threshold
+    assert fell_back and res > sur.check_threshold
     a_pm = sur.gather(sur.pm.fields(pos, mass_value=1.0 / n)[1], pos)
     assert torch.equal(a, a_pm)
     s1, s2 = sur.load(pos, vel), sur.load(pos, vel)
@@ -143,3 +143,59 @@ def test_train_step_on_grids_equals_train_step(dev):
         assert l1 == l2
     for w1, w2 in zip(s1.net.parameters(), s2.net.parameters()):
         assert torch.equal(w1, w2)
+
+
+def test_correction_fallback_trips_at_a_later_step(dev):
+    """Correction mode with a non-zero net: a per-step threshold that passes
+    step 0 and fails step 1 (forced from the host) makes the call the
+    surrogate's step 0 followed by plain PM, bit for bit; the gate records the
+    step and every residual, on the device."""
+    from augur_amd.surrogate import PMSurrogate
+    n, G, dt = 20000, 64, 1e-3
+    pos = clumpy_scene(6, n, dev)
+    vel = 0.3 * torch.randn(n, 3, device=dev)
+    sur = PMSurrogate(G, device=dev, mode="correction", init_std=0.02)
+    probe = torch.arange(0, n, 3, device=dev)
+    a, b = sur.load(pos, vel), sur.load(pos, vel)
+    g0 = sur.kdk(a, dt, 1, probe=probe, threshold=float("inf"))
+    sur.pm.kdk(a, dt, 2)
+    gate = sur.kdk(b, dt, 3, probe=probe, threshold=[float("inf"), 0.0, 0.0])
+    assert not g0.fell_back and g0.residual > 0
+    assert gate.fell_back and gate.fallback_step == 1
+    res = gate.residuals
+    assert res[0] == g0.residual and res[1] > 0 and res[2] == 0.0
+    for x, y in ((a.pos, b.pos), (a.vel, b.vel), (a.acc, b.acc)):
+        assert torch.equal(x, y)
+
+
+def test_kdk_with_gate_is_capturable(dev):
+    """A 2-step surrogate call with its device-side residual check captured in
+    a HIP graph: every replay equals the eager call on the same state."""
+    from augur_amd.surrogate import PMSurrogate
+    n, G, dt = 20000, 64, 1e-3
+    pos = clumpy_scene(7, n, dev)
+    vel = 0.3 * torch.randn(n, 3, device=dev)
+    sur = PMSurrogate(G, device=dev, mode="correction", init_std=0.02)
+    probe = torch.arange(0, n, 3, device=dev)
+    st = sur.load(pos, vel)
+    sur.kdk(st, dt, 2, probe=probe)          # warm-up: calibration, allocations
+    torch.cuda.synchronize()
+    saved = [t.clone() for t in (st.pos, st.vel, st.acc, st.ids, st.ws)]
+    eager = sur.kdk(st, dt, 2, probe=probe)
+    want = [t.clone() for t in (st.pos, st.vel, st.acc)]
+    want_res = eager.residuals
+    graph = torch.cuda.CUDAGraph()
+    for t, s in zip((st.pos, st.vel, st.acc, st.ids, st.ws), saved):
+        t.copy_(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        cap = sur.kdk(st, dt, 2, probe=probe)
+    for _ in range(3):
+        for t, s in zip((st.pos, st.vel, st.acc, st.ids, st.ws), saved):
+            t.copy_(s)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for x, y in zip((st.pos, st.vel, st.acc), want):
+            assert torch.equal(x, y)
+        assert cap.residuals == want_res
