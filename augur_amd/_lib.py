@@ -58,6 +58,7 @@ SIGNATURES = {
                                     c_size_t, P]),
     "augur_pm_step_end": (c_int, [P, P, P, c_int64, c_int64, c_float, c_float, c_int, P, c_size_t, P]),
     "augur_pm_grid_offsets": (c_int, [c_int64, c_int64, P, P]),
+    "augur_sur_gate_scratch_bytes": (c_size_t, [c_int64]),
     "augur_sur_gate": (c_int, [c_int, P, P, P, c_int64, c_int64, c_float, P, P, P, P, c_int, P]),
     "augur_pm_force_replay": (c_int, [c_int64, c_int64, c_float, P, P, c_size_t, P]),
     "augur_conv3_bf16_fwd_into": (c_int, [P, c_int, P, P, c_int64, c_int64, c_int, c_float, c_int, P, P, c_int, P]),

@@ -486,6 +486,19 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
     const int z = zs + zi;
     u32x4 pre[PER];
     if (zi + 1 < zc) load_plane(z + 2, pre);
+    // the field values this thread updates, fetched before the MFMAs (in place, accumulating)
+    float old[ZM_RPW][4];
+    if (MODE == CONV_OUT_F32_CF_INTO && accumulate) {
+#pragma unroll
+      for (int r = 0; r < ZM_RPW; ++r) {
+        const int64_t sv = ((int64_t)z * G + y0 + wrow * ZM_RPW + r) * G + x0 + hf * 16 + n16;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int co = 4 * kq + q;
+          old[r][q] = co < cout ? y_out[((int64_t)b * cout + co) * G3 + sv] : 0.0f;
+        }
+      }
+    }
     f32x4 acc[ZM_RPW];
 #pragma unroll
     for (int r = 0; r < ZM_RPW; ++r) acc[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -520,8 +533,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
           if (MODE == CONV_OUT_F32_CF) {
             y_out[((int64_t)b * cout + co) * G3 + sv] = val;
           } else if (MODE == CONV_OUT_F32_CF_INTO) {
-            float* dst = &y_out[((int64_t)b * cout + co) * G3 + sv];
-            *dst = accumulate ? fmaf(alpha, val, *dst) : alpha * val;
+            y_out[((int64_t)b * cout + co) * G3 + sv] = accumulate ? fmaf(alpha, val, old[r][q]) : alpha * val;
           } else {
             const int64_t o = ((int64_t)b * G3 + sv) * cout + co;
             y_out[o] = addend ? val + addend[o] : val;

@@ -1435,7 +1435,7 @@ static int pm_force_grid(const float* pos, const uint32_t* perm, const float* ma
 // gate[1]  revert flag of the current step: the force planes are re-made from
 //          the PM spectra still in the workspace (augur_pm_force_replay)
 // gate[2]  step counter of the call; gate[3] first step that fell back (~0: none)
-constexpr int GATE_THREADS = 1024;
+constexpr int GATE_THREADS = 256;
 
 template <bool P2>
 __device__ __forceinline__ void gather_point(const float* __restrict__ force, int G, float h, float inv_h,
@@ -1461,21 +1461,18 @@ __device__ __forceinline__ void gather_point(const float* __restrict__ force, in
   }
 }
 
+// phases 1 and 2: one probe per thread over ceil(P / GATE_THREADS) workgroups;
+// phase 2 leaves each workgroup's FP64 sums in part[2 blk], part[2 blk + 1]
 template <bool P2>
-__global__ __launch_bounds__(GATE_THREADS) void k_sur_gate(int phase, const float* __restrict__ force,
-                                                           const float* __restrict__ pos,
-                                                           const int64_t* __restrict__ probe, int P, int G, float h,
-                                                           float inv_h, float* __restrict__ ref,
-                                                           uint32_t* __restrict__ gate,
-                                                           const float* __restrict__ thr,
-                                                           float* __restrict__ log, int nlog) {
+__global__ __launch_bounds__(GATE_THREADS) void k_sur_probe(int phase, const float* __restrict__ force,
+                                                            const float* __restrict__ pos,
+                                                            const int64_t* __restrict__ probe, int P, int G, float h,
+                                                            float inv_h, float* __restrict__ ref,
+                                                            double* __restrict__ part, uint32_t* __restrict__ gate) {
   const int tid = threadIdx.x;
-  if (phase == 0) {
-    if (tid < 4) gate[tid] = tid == 3 ? 0xffffffffu : 0u;
-    return;
-  }
+  const int i = blockIdx.x * GATE_THREADS + tid;
   double num = 0.0, den = 0.0;
-  for (int i = tid; i < P; i += GATE_THREADS) {
+  if (i < P) {
     const int64_t k = probe[i];
     const float p[3] = {pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
     float a[3];
@@ -1492,7 +1489,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_sur_gate(int phase, const floa
     }
   }
   if (phase == 1) {
-    if (tid == 0) gate[1] = 0u;
+    if (i == 0) gate[1] = 0u;
     return;
   }
   __shared__ double sn[GATE_THREADS / 64], sd[GATE_THREADS / 64];
@@ -1508,21 +1505,41 @@ __global__ __launch_bounds__(GATE_THREADS) void k_sur_gate(int phase, const floa
   __syncthreads();
   if (tid == 0) {
     double tn = 0.0, td = 0.0;
-    for (int k = 0; k < GATE_THREADS / 64; ++k) {
-      tn += sn[k];
-      td += sd[k];
+    for (int w = 0; w < GATE_THREADS / 64; ++w) {
+      tn += sn[w];
+      td += sd[w];
     }
-    const uint32_t st = gate[2];
-    const double res = td > 0.0 ? sqrt(tn / td) : (tn > 0.0 ? INFINITY : 0.0);
-    if ((int)st < nlog) log[st] = (float)res;
-    const float t = thr[(int)st < nlog ? st : nlog - 1];
-    if (gate[0] == 0u && res > (double)t) {
-      gate[0] = 1u;
-      gate[1] = 1u;
-      gate[3] = st;
-    }
-    gate[2] = st + 1u;
+    part[2 * blockIdx.x] = tn;
+    part[2 * blockIdx.x + 1] = td;
   }
+}
+
+// phase 0 resets the gate; phase 3 sums the partials in workgroup order and decides
+__global__ void k_sur_decide(int phase, int nblk, const double* __restrict__ part, uint32_t* __restrict__ gate,
+                             const float* __restrict__ thr, float* __restrict__ log, int nlog) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (phase == 0) {
+    gate[0] = 0u;
+    gate[1] = 0u;
+    gate[2] = 0u;
+    gate[3] = 0xffffffffu;
+    return;
+  }
+  double tn = 0.0, td = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    tn += part[2 * b];
+    td += part[2 * b + 1];
+  }
+  const uint32_t st = gate[2];
+  const double res = td > 0.0 ? sqrt(tn / td) : (tn > 0.0 ? INFINITY : 0.0);
+  if ((int)st < nlog) log[st] = (float)res;
+  const float t = thr[(int)st < nlog ? st : nlog - 1];
+  if (gate[0] == 0u && res > (double)t) {
+    gate[0] = 1u;
+    gate[1] = 1u;
+    gate[3] = st;
+  }
+  gate[2] = st + 1u;
 }
 
 }  // namespace augur
@@ -1749,24 +1766,41 @@ int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, v
   return AUGUR_OK;
 }
 
+size_t augur_sur_gate_scratch_bytes(int64_t P) {
+  const int64_t nblk = (P + GATE_THREADS - 1) / GATE_THREADS;
+  return align_up((size_t)(3 * P) * sizeof(float), 16) + (size_t)(2 * (nblk > 0 ? nblk : 1)) * sizeof(double);
+}
+
 int augur_sur_gate(int phase, const float* force, const float* pos, const int64_t* probe, int64_t P, int64_t G,
-                   float L, float* ref, uint32_t* gate, const float* thr, float* log, int nlog, void* stream) {
+                   float L, void* scratch, uint32_t* gate, const float* thr, float* log, int nlog, void* stream) {
   AUGUR_REQUIRE(gate && phase >= 0 && phase <= 2, AUGUR_EINVAL, "sur_gate: bad phase or null gate");
   AUGUR_REQUIRE(is_pow2(G) && G >= 2 && G <= 4096 && L > 0.0f, AUGUR_EINVAL, "sur_gate: bad grid");
-  if (phase >= 1)
-    AUGUR_REQUIRE(force && pos && probe && ref && P >= 1 && P < ((int64_t)1 << 31), AUGUR_EINVAL,
-                  "sur_gate: bad probe arguments");
-  if (phase == 2) AUGUR_REQUIRE(thr && log && nlog >= 1, AUGUR_EINVAL, "sur_gate: phase 2 needs thr, log");
   hipStream_t s = as_stream(stream);
-  const float h = (float)((double)L / (double)G);
   TimedScope ts_("sur_gate", s);
+  if (phase == 0) {
+    hipLaunchKernelGGL(k_sur_decide, dim3(1), dim3(64), 0, s, 0, 0, (const double*)nullptr, gate,
+                       (const float*)nullptr, (float*)nullptr, 0);
+    AUGUR_LAUNCHED("sur_gate");
+    return AUGUR_OK;
+  }
+  AUGUR_REQUIRE(force && pos && probe && scratch && P >= 1 && P < ((int64_t)1 << 31), AUGUR_EINVAL,
+                "sur_gate: bad probe arguments");
+  if (phase == 2) AUGUR_REQUIRE(thr && log && nlog >= 1, AUGUR_EINVAL, "sur_gate: phase 2 needs thr, log");
+  const float h = (float)((double)L / (double)G);
+  const int nblk = (int)((P + GATE_THREADS - 1) / GATE_THREADS);
+  float* ref = static_cast<float*>(scratch);
+  double* part = reinterpret_cast<double*>(static_cast<char*>(scratch) + align_up((size_t)(3 * P) * sizeof(float), 16));
   if (pow2_float(h))
-    hipLaunchKernelGGL(k_sur_gate<true>, dim3(1), dim3(GATE_THREADS), 0, s, phase, force, pos, probe, (int)P, (int)G,
-                       h, 1.0f / h, ref, gate, thr, log, nlog);
+    hipLaunchKernelGGL(k_sur_probe<true>, dim3(nblk), dim3(GATE_THREADS), 0, s, phase, force, pos, probe, (int)P,
+                       (int)G, h, 1.0f / h, ref, part, gate);
   else
-    hipLaunchKernelGGL(k_sur_gate<false>, dim3(1), dim3(GATE_THREADS), 0, s, phase, force, pos, probe, (int)P,
-                       (int)G, h, 0.0f, ref, gate, thr, log, nlog);
+    hipLaunchKernelGGL(k_sur_probe<false>, dim3(nblk), dim3(GATE_THREADS), 0, s, phase, force, pos, probe, (int)P,
+                       (int)G, h, 0.0f, ref, part, gate);
   AUGUR_LAUNCHED("sur_gate");
+  if (phase == 2) {
+    hipLaunchKernelGGL(k_sur_decide, dim3(1), dim3(64), 0, s, 3, nblk, (const double*)part, gate, thr, log, nlog);
+    AUGUR_LAUNCHED("sur_gate_decide");
+  }
   return AUGUR_OK;
 }
 

@@ -520,7 +520,7 @@ class PMSurrogate:
                 assert thr.numel() >= steps, "one threshold per step"
             gate = torch.empty(4, dtype=torch.int32, device=dev)
             log = torch.zeros(max(1, steps), dtype=torch.float32, device=dev)
-            ref = torch.empty(P, 3, dtype=torch.float32, device=dev)
+            ref = _ws(_lib.size("augur_sur_gate_scratch_bytes", P), dev)
             _lib.call("augur_sur_gate", 0, None, None, None, 0, G, self.L, None, _p(gate), None, None, 0, stream)
         mean_density = None
         for s in range(steps):

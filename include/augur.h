@@ -265,14 +265,16 @@ int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* forc
  * host synchronisation, so a k-step call can be captured in a HIP graph).
  * gate: 4 uint32 words [fallback (sticky within a call), revert (this step),
  * step counter, first step that fell back (~0u: none)].
- * augur_sur_gate(phase, ...), one workgroup:
+ * augur_sur_gate(phase, ...):
  *   0  reset the gate (the start of a call);
- *   1  ref[3 p ..] = CIC gather of the force planes (3 x G^3 FP32, the PM
+ *   1  scratch[3 p ..] = CIC gather of the force planes (3 x G^3 FP32, the PM
  *      field of augur_pm_step_begin) at pos[probe[p]], p < P; clear revert;
  *   2  gather again (after the learned field was applied); the residual
- *      sqrt(sum |a - ref|^2 / sum |ref|^2) (FP64 sums) goes to log[step];
- *      above thr[step] (thr[nlog - 1] past the end) it sets fallback and
- *      revert and records the step; the step counter advances.
+ *      sqrt(sum |a - ref|^2 / sum |ref|^2) (FP64 sums, per workgroup and
+ *      then in workgroup order) goes to log[step]; above thr[step]
+ *      (thr[nlog - 1] past the end) it sets fallback and revert and records
+ *      the step; the step counter advances.
+ * scratch: augur_sur_gate_scratch_bytes(P) bytes (16-byte aligned).
  * augur_pm_force_replay(): while gate[1] (pass &gate[1]) is set, re-makes
  * the force planes from the PM spectra the step left in the workspace (the
  * last FFT pass of augur_pm_step_begin again: the same bits); otherwise the
@@ -280,8 +282,9 @@ int augur_pm_grid_offsets(int64_t n, int64_t G, size_t* rho_offset, size_t* forc
  * a residual check against it exceeds a threshold"; the reference has no
  * such check (its nearest is the device-side cell_max_occ overflow test of
  * the pair term, scripts/scatter_field.py:324-334). */
+size_t augur_sur_gate_scratch_bytes(int64_t P);
 int augur_sur_gate(int phase, const float* force, const float* pos, const int64_t* probe, int64_t P, int64_t G,
-                   float L, float* ref, uint32_t* gate, const float* thr, float* log, int nlog, void* stream);
+                   float L, void* scratch, uint32_t* gate, const float* thr, float* log, int nlog, void* stream);
 int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, void* ws, size_t ws_bytes,
                           void* stream);
 
