@@ -74,6 +74,13 @@ SIGNATURES = {
     "augur_pack_cl_bf16": (c_int, [P, c_int, c_int64, c_int64, c_int, c_float, P, P]),
     "augur_pack_features_bf16": (c_int, [P, P, c_int64, c_int64, c_float, c_float, c_float, c_int, P, P]),
     "augur_adam_update": (c_int, [P, P, P, P, c_int64, c_float, c_float, c_float, c_float, c_int, c_float, P, P]),
+    "augur_conv2d_f32": (c_int, [P, c_int, c_int, P, c_int, P, P, c_int64, c_int64, c_int64, c_int, c_int, P, P, P]),
+    "augur_sf_kernel_grid": (c_int, [P, c_int64, c_float, c_float, P, P]),
+    "augur_sf_neg_grad": (c_int, [P, c_int, c_int64, c_int64, c_float, P, P, P]),
+    "augur_pair_knn_f32": (c_int, [P, P, P, c_int64, c_int64, P, P, c_int, c_float, c_float, c_float, c_int, P, P,
+                                   P]),
+    "augur_sf_momfix": (c_int, [P, P, P, c_int64, c_int64, c_int, c_int, P]),
+    "augur_sf_update": (c_int, [P, P, P, P, P, c_int64, c_float, c_int, P]),
     "augur_timing_enable": (c_int, [c_int]),
     "augur_timing_filter": (c_int, [ctypes.c_char_p]),
     "augur_timing_collect": (c_int, [P, c_size_t, P, P, c_int, P]),

@@ -2,6 +2,7 @@
 the tensors' device and launch on the current stream.  Every function needs
 its tensors on a ROCm GPU and raises otherwise (there is no CPU path)."""
 import ctypes
+import math
 
 import torch
 
@@ -274,3 +275,86 @@ def energy_momentum_f64(vel, mass=None, mass_value=1.0, phi=None):
     _lib.call("augur_reduce_energy_momentum_f64", _p(vel), _p(m), float(mass_value), _p(ph), n, _p(out), _p(ws), wsb,
               _stream(dev))
     return out
+
+
+# ------------------------------------------------- 2D scatter-field step
+def conv2d(a, weight, bias=None, act=0, amode=0, b=None, res=None):
+    """FP32 3x3 zero-padded conv (augur_conv2d_f32).  a (B, ca, Ha, Wa) read
+    through amode (0 same size, 1 2x2 average, 2 nearest x2), b (B, cb, H, W)
+    concatenated after it; weight (cout, ca + cb, 3, 3); act 0 none, 1 GELU,
+    2 res + GELU.  -> (B, cout, H, W)."""
+    a = _c(a, F32)
+    dev = _dev(a)
+    B, ca, Ha, Wa = a.shape
+    H, W = (2 * Ha, 2 * Wa) if amode == 2 else ((Ha // 2, Wa // 2) if amode == 1 else (Ha, Wa))
+    b = _c(b, F32)
+    cb = 0 if b is None else b.shape[1]
+    if b is not None:
+        assert b.shape == (B, cb, H, W), (b.shape, (B, cb, H, W))
+    w = _c(weight, F32)
+    cout = w.shape[0]
+    assert w.shape == (cout, ca + cb, 3, 3)
+    res = _c(res, F32)
+    out = torch.empty(B, cout, H, W, dtype=F32, device=dev)
+    _lib.call("augur_conv2d_f32", _p(a), ca, int(amode), _p(b), cb, _p(w), _p(_c(bias, F32)), B, H, W, cout,
+              int(act), _p(res), _p(out), _stream(dev))
+    return out
+
+
+def mlp_params(seq):
+    """Flat FP32 parameters of nn.Sequential(Linear(2, 64), GELU, Linear(64, 64),
+    GELU, Linear(64, 1)) in module order (the layout the kernels read)."""
+    ps = [seq[0].weight, seq[0].bias, seq[2].weight, seq[2].bias, seq[4].weight, seq[4].bias]
+    assert tuple(seq[0].weight.shape) == (64, 2) and tuple(seq[4].weight.shape) == (1, 64), \
+        "the kernels evaluate 2 -> 64 -> 64 -> 1 MLPs"
+    return torch.cat([p.detach().reshape(-1).float() for p in ps]).contiguous()
+
+
+def sf_kernel_grid(mlp_flat, G, h, pp=0.0):
+    """Learned kernel on the (2G)^2 min-image grid (augur_sf_kernel_grid)."""
+    dev = _dev(mlp_flat)
+    K = torch.empty(2 * G, 2 * G, dtype=F32, device=dev)
+    _lib.call("augur_sf_kernel_grid", _p(mlp_flat), int(G), float(h), float(pp), _p(K), _stream(dev))
+    return K
+
+
+def sf_neg_grad(phi, G, h, ext=False, addend=None):
+    """phi (B, 1, G, G) (ext: (B, 1, 2G, 2G)) -> -grad (B, 2, G, G) (+ addend)."""
+    phi = _c(phi, F32)
+    dev = _dev(phi)
+    B = phi.shape[0]
+    out = torch.empty(B, 2, G, G, dtype=F32, device=dev)
+    _lib.call("augur_sf_neg_grad", _p(phi), int(bool(ext)), B, int(G), float(h), _p(_c(addend, F32)), _p(out),
+              _stream(dev))
+    return out
+
+
+def pair_knn(pos, mass, mask, extent, pp, knn, mlp_flat, acc):
+    """Adds the cell-list kNN pair term to acc (B, N, 2) in place."""
+    pos = _c(pos, F32)
+    dev = _dev(pos, acc)
+    B, N, _ = pos.shape
+    C = int(math.ceil(extent / pp))
+    keys = cell_keys(pos, mask, extent / 2, pp, C)
+    perm, starts = sort_by_cell(keys, B * C * C + 1, starts=True)
+    _lib.call("augur_pair_knn_f32", _p(pos), _p(_c(mass, F32)), _p(_c(mask, F32)), B, N, _p(perm), _p(starts), C,
+              float(extent / 2), float(pp), float(pp), int(knn), _p(mlp_flat), _p(acc), _stream(dev))
+    return acc
+
+
+def sf_momfix(acc, mass, mask, fix=True):
+    """acc (B, N, D) in place: minus the mass-weighted mean (fix), times mask."""
+    dev = _dev(acc)
+    B, N, D = acc.shape
+    _lib.call("augur_sf_momfix", _p(acc), _p(_c(mass, F32)), _p(_c(mask, F32)), B, N, D, int(bool(fix)),
+              _stream(dev))
+    return acc
+
+
+def sf_update(pos, vel, a0, a1, aux, dt, phase):
+    """In-place step updates (augur_sf_update), all tensors FP32 contiguous of one shape."""
+    dev = _dev(pos, vel, a0)
+    for t in (pos, vel, a0, a1, aux):
+        assert t is None or (t.dtype == F32 and t.is_contiguous() and t.numel() == pos.numel())
+    _lib.call("augur_sf_update", _p(pos), _p(vel), _p(a0), _p(a1), _p(aux), pos.numel(), float(dt), int(phase),
+              _stream(dev))

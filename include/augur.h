@@ -405,6 +405,67 @@ int augur_pack_features_bf16(const float* rho, const float* force, int64_t B, in
 int augur_adam_update(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                       float eps, int step, float gscale, uint16_t* p_bf16, void* stream);
 
+/* ------------------------------------- 2D reference scatter-field step */
+
+/* FP32 3x3 convolution with zero padding, bias and an epilogue, on
+ * channels-first grids: out[b][co][y][x] = act(bias[co] + sum w[co][ci][dy][dx]
+ * in[b][ci][y + dy - 1][x + dx - 1]) with w in nn.Conv2d's layout [cout][cin][3][3]
+ * (cin = ca + cb), bias may be NULL.  Input channels ci < ca are read from a
+ * (B, ca, ., .) through amode: 0 same size H x W; 1 a is 2H x 2W and is
+ * averaged over 2x2 blocks (F.avg_pool2d(., 2)); 2 a is H/2 x W/2, nearest
+ * upsampling (F.interpolate(scale_factor=2, mode="nearest")); channels
+ * ca .. ca + cb - 1 from b (B, cb, H, W) (torch.cat([a', b], 1)).
+ * act: 0 none; 1 exact (erf) GELU; 2 res + GELU, res (B, cout, H, W).
+ * Replaces the nn.Conv2d / F.gelu / avg_pool2d / interpolate / cat layers of
+ * LocalNet and UNet (scripts/scatter_field.py:15-67). */
+int augur_conv2d_f32(const float* a, int ca, int amode, const float* b, int cb, const float* w, const float* bias,
+                     int64_t B, int64_t H, int64_t W, int cout, int act, const float* res, float* out, void* stream);
+
+/* The learned kernel K on the (2G)^2 min-image distance grid: K[iy][ix] =
+ * mlp(r, log(r + h)) (1 - (1 - min(r/pp, 1)^2)^2 if pp > 0, the split taper),
+ * r = sqrt(dy^2 + dx^2), d = min(i, 2G - i) h.  mlp: the 4417 FP32
+ * parameters of nn.Sequential(Linear(2,64), GELU, Linear(64,64), GELU,
+ * Linear(64,1)) in module order (W0, b0, W1, b1, W2, b2).  Replaces the
+ * kernel construction of ScatterField._kernel_fft (scripts/scatter_field.py:
+ * 263-272); its rfft2 is augur_fft_r2c. */
+int augur_sf_kernel_grid(const float* mlp, int64_t G, float h, float pp, float* K, void* stream);
+
+/* -grad by central differences, out (B, 2, G, G) = [-d/dx, -d/dy] (+ addend,
+ * same shape, may be NULL).  ext = 0: phi (B, G, G) zero padded
+ * (ScatterField.neg_grad, scripts/scatter_field.py:551-555); ext = 1: phi is
+ * the full (B, 2G, 2G) zero-padded convolution, differenced at rows and
+ * columns -1 (= 2G - 1) .. G unpadded (kernel_acc with kpad "ext", :286-288).
+ * The addend is the "acc + ak" of _grid_force (:660-662). */
+int augur_sf_neg_grad(const float* phi, int ext, int64_t B, int64_t G, float h, const float* addend, float* out,
+                      void* stream);
+
+/* Learned short-range pair term through a cell list: cells of side `cell`
+ * (= pp), C per axis, keys and the stable order from augur_cell_keys /
+ * augur_sort_by_cell (perm over B*N, starts over B*C*C + 1).  For every
+ * unmasked particle, the min(knn, .) nearest unmasked others with
+ * r = sqrt(|d|^2 + 1e-8) < pp add ppmlp([r, log(r + 0.05)]) (1 - (r/pp)^2)^2
+ * m_j d / r to acc (B, N, 2) (accumulated).  knn <= 16.  Replaces
+ * ScatterField.pp_acc_cell (scripts/scatter_field.py:324-359) and, with the
+ * same result, the dense pp_acc path (:504-522). */
+int augur_pair_knn_f32(const float* pos, const float* mass, const float* mask, int64_t B, int64_t N,
+                       const int64_t* perm, const int64_t* starts, int C, float off, float cell, float pp, int knn,
+                       const float* mlp, float* acc, void* stream);
+
+/* acc (B, N, D) -= sum(m acc) / max(sum m, 1e-9) per scene (fix != 0;
+ * m = mass * mask, FP64 sums in a fixed order), then acc *= mask.  mass and
+ * mask may be NULL (ones).  Replaces the momfix and masking of
+ * ScatterField._force (scripts/scatter_field.py:612-615). */
+int augur_sf_momfix(float* acc, const float* mass, const float* mask, int64_t B, int64_t N, int D, int fix,
+                    void* stream);
+
+/* Elementwise step updates over n floats, in the reference's operation order:
+ * phase 0 (Verlet drift): pos += dt vel + (dt^2/2) a0, aux = vel + dt a0;
+ * phase 1 (Verlet close): aux = (dt/2)(a0 + a1) (aux may be NULL), vel += it;
+ * phase 2 (plain step): aux = a0 dt, pos += dt (vel + aux/2), vel += aux.
+ * Replaces ScatterField._step (scripts/scatter_field.py:677-688). */
+int augur_sf_update(float* pos, float* vel, const float* a0, const float* a1, float* aux, int64_t n, float dt,
+                    int phase, void* stream);
+
 /* ------------------------------------------------------------ measurement */
 
 /* Per-launch timing: when enabled, each kernel launch of the library records

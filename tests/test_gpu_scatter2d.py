@@ -1,0 +1,166 @@
+"""GPU parity of the 2D reference scatter-field step (augur_amd.scatter_field
+on the library's kernels) with the reference's own FP64 rollouts of its
+shipped model (web/scatter/tests/vectors.json, ms_kp_pot_v_g128: UNet,
+learned kernel with the split taper, kNN pair term, momfix, Verlet), and of
+each kernel with the FP64 oracle (oracle/scatter2d.py) on seeded inputs.
+Tolerances are FP32 against FP64."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "scatter2d")
+F64 = torch.float64
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def ref_model(dev):
+    from augur_amd.scatter_field import ScatterField
+    from oracle import scatter2d as S
+    params, cfg = S.load_web_weights(os.path.join(GOLD, "weights.json"), os.path.join(GOLD, "weights.bin"))
+    m = ScatterField(grid=cfg["grid"], extent=cfg["extent"], net="unet", dt=cfg["dt"], kernel=True, pp=cfg["pp"],
+                     split=True, potential=True, verlet=True, levels=cfg["levels"], knn=cfg["knn"]).to(dev)
+    m.load_state_dict({k: v.float() for k, v in params.items()})
+    return m, params, S.default_cfg(**cfg)
+
+
+@pytest.mark.parametrize("name", ["n8", "n30", "n100"])
+def test_reference_rollouts(dev, ref_model, name):
+    from augur_amd.scatter_field import rollout
+    model = ref_model[0]
+    vec = json.load(open(os.path.join(GOLD, "vectors.json")))
+    sc = vec["scenes"][name]
+    pos = torch.tensor(sc["pos"][0], dtype=torch.float32, device=dev)[None]
+    vel = torch.tensor(sc["vel"][0], dtype=torch.float32, device=dev)[None]
+    n = pos.shape[1]
+    one = torch.ones(1, n, device=dev)
+    field = model.init_field(1, dev)
+    errs = []
+    with torch.no_grad():
+        for s in range(vec["steps"]):
+            pos, vel, field, _ = model.step(pos, vel, one, one, field)
+            e = max(float((pos[0].double().cpu() - torch.tensor(sc["pos"][s + 1])).abs().max()),
+                    float((vel[0].double().cpu() - torch.tensor(sc["vel"][s + 1])).abs().max()))
+            errs.append(e)
+    print(name, "max |d| per step:", " ".join(f"{e:.1e}" for e in errs), flush=True)
+    # the reference's own bar for a port (web/scatter/tests/verify.mjs:38) is one step within 1e-4;
+    # FP32 here stays within 1e-5 after one step and 1e-4 after twenty
+    assert errs[0] < 1e-5
+    assert max(errs) < 1e-4
+    if sc["field0"] is not None:
+        f0 = torch.tensor(sc["field0"])
+        assert float((field[0][0, 0].double().cpu() - f0).abs().max()) < 1e-4 * float(f0.abs().max()) + 1e-5
+    # rollout() gives the same trajectory as stepping by hand
+    p0 = torch.tensor(sc["pos"][0], dtype=torch.float32, device=dev)[None]
+    v0 = torch.tensor(sc["vel"][0], dtype=torch.float32, device=dev)[None]
+    ps, vs = rollout(model, p0, v0, one, one, 3)
+    assert ps.shape == (1, 4, n, 2)
+    assert float((ps[0, 1].double().cpu() - torch.tensor(sc["pos"][1])).abs().max()) < 1e-4
+
+
+@pytest.mark.parametrize("amode,act,cb", [(0, 1, 0), (0, 0, 0), (1, 1, 0), (2, 1, 32), (0, 2, 0)])
+def test_conv2d_matches_fp64(dev, amode, act, cb):
+    from augur_amd import ops
+    g = torch.Generator().manual_seed(10 * amode + act + cb)
+    B, H, W, ca, cout = 2, 32, 48, 32, 32 if act == 2 else 27
+    Ha, Wa = (2 * H, 2 * W) if amode == 1 else ((H // 2, W // 2) if amode == 2 else (H, W))
+    a = torch.randn(B, ca, Ha, Wa, generator=g)
+    b = torch.randn(B, cb, H, W, generator=g) if cb else None
+    w = 0.1 * torch.randn(cout, ca + cb, 3, 3, generator=g)
+    bias = 0.1 * torch.randn(cout, generator=g)
+    res = torch.randn(B, cout, H, W, generator=g) if act == 2 else None
+    x = a.double()
+    if amode == 1:
+        x = F.avg_pool2d(x, 2)
+    elif amode == 2:
+        x = F.interpolate(x, scale_factor=2, mode="nearest")
+    if cb:
+        x = torch.cat([x, b.double()], 1)
+    ref = F.conv2d(x, w.double(), bias.double(), padding=1)
+    if act >= 1:
+        ref = F.gelu(ref)
+    if act == 2:
+        ref = res.double() + ref
+    y = ops.conv2d(a.to(dev), w.to(dev), bias.to(dev), act=act, amode=amode, b=None if b is None else b.to(dev),
+                   res=None if res is None else res.to(dev))
+    assert float((y.double().cpu() - ref).abs().max()) < 2e-5 * float(ref.abs().max())
+
+
+@pytest.mark.parametrize("kpad", ["zero", "ext"])
+def test_kernel_acc_matches_oracle(dev, ref_model, kpad):
+    from oracle import scatter2d as S
+    model, params, cfg = ref_model
+    G = model.grid
+    rng = np.random.default_rng(4)
+    rho = torch.from_numpy(rng.uniform(0, 1, (2, 1, G, G)))
+    model.kpad = kpad
+    try:
+        ak = model.kernel_acc(rho.float().to(dev)).double().cpu()
+    finally:
+        model.kpad = "zero"
+    ref = S.kernel_acc(rho, params, dict(cfg, kpad=kpad))
+    K = S.kernel_grid(params, cfg)
+    from augur_amd import ops
+    Kg = ops.sf_kernel_grid(ops.mlp_params(model.kmlp), G, model.h, cfg["pp"]).double().cpu()
+    assert float((Kg - K).abs().max()) < 1e-5 * float(K.abs().max())
+    # FP32 FFT convolution, then a difference of neighbouring potentials: ~1e-4 relative
+    assert float((ak - ref).abs().max()) < 3e-4 * float(ref.abs().max())
+
+
+@pytest.mark.parametrize("n,masked", [(100, False), (600, True), (3000, False)])
+def test_pair_term_matches_dense_oracle(dev, ref_model, n, masked):
+    # the cell-list pair term equals the reference's dense kNN sum, also past
+    # the dense path's N <= 512 and with masked particles
+    from oracle import scatter2d as S
+    model, params, cfg = ref_model
+    rng = np.random.default_rng(n)
+    spread = 2.0 * np.sqrt(n / 8)
+    pos = torch.from_numpy(rng.uniform(-spread, spread, (1, n, 2)))
+    mass = torch.from_numpy(rng.uniform(0.5, 2.0, (1, n)))
+    mask = torch.from_numpy((rng.uniform(0, 1, (1, n)) > (0.2 if masked else -1)).astype(np.float64))
+    ref = S.pair_acc(pos, mass, mask, params, cfg)
+    got = model.pp_acc(pos.float().to(dev), mass.float().to(dev), mask.float().to(dev), cfg["knn"]).double().cpu()
+    assert float((got - ref).abs().max()) < 1e-4 * float(ref.abs().max()) + 1e-7
+    assert float(ref.abs().max()) > 0
+
+
+def test_localnet_plain_step_matches_oracle(dev):
+    """A LocalNet model with random non-zero weights, no kernel / pair term,
+    the non-Verlet step with momfix, against the oracle."""
+    from augur_amd.scatter_field import ScatterField
+    from oracle import scatter2d as S
+    torch.manual_seed(7)
+    model = ScatterField(grid=64, extent=32.0, net="local", layers=3, width=32, dt=0.05).to(dev)
+    with torch.no_grad():
+        for p in model.parameters():
+            p.normal_(0.0, 0.05)
+    params = {k: v.double().cpu() for k, v in model.state_dict().items()}
+    cfg = S.default_cfg(grid=64, extent=32.0, net="local", layers=3, dt=0.05)
+    rng = np.random.default_rng(5)
+    pos = torch.from_numpy(rng.uniform(-12, 12, (2, 200, 2)))
+    vel = torch.from_numpy(rng.normal(size=(2, 200, 2)))
+    mass = torch.from_numpy(rng.uniform(0.5, 2, (2, 200)))
+    mask = torch.ones(2, 200, dtype=F64)
+    mask[1, 150:] = 0
+    f0 = S.init_field(2, cfg)
+    p1, v1, f1, dv1 = S.step(pos, vel, mass, mask, f0, params, cfg)
+    p2, v2, f2, dv2 = S.step(p1, v1, mass, mask, f1, params, cfg)
+    d = lambda t: t.float().to(dev)
+    q1, w1, g1, _ = model.step(d(pos), d(vel), d(mass), d(mask), model.init_field(2, dev))
+    q2, w2, g2, e2 = model.step(q1, w1, d(mass), d(mask), g1)
+    for got, ref in ((q2, p2), (w2, v2), (g2, f2), (e2, dv2)):
+        assert float((got.double().cpu() - ref).abs().max()) < 1e-4 * float(ref.abs().max()) + 1e-6
+    # momfix: the mass-weighted mean of dv is zero in each scene
+    mm = (mass * mask)[..., None]
+    assert float((mm * e2.double().cpu()).sum(1).abs().max()) < 1e-5 * float((mm * e2.double().cpu().abs()).sum())
