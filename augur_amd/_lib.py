@@ -81,6 +81,9 @@ SIGNATURES = {
                                    P]),
     "augur_sf_momfix": (c_int, [P, P, P, c_int64, c_int64, c_int, c_int, P]),
     "augur_sf_update": (c_int, [P, P, P, P, P, c_int64, c_float, c_int, P]),
+    "augur_grad_sumsq": (c_int, [P, c_int64, P, P]),
+    "augur_grad_clip_decide": (c_int, [P, P, c_float, P, P, P]),
+    "augur_adam_update_gated": (c_int, [P, P, P, P, c_int64, c_float, c_float, c_float, c_float, P, P, P, P]),
     "augur_timing_enable": (c_int, [c_int]),
     "augur_timing_filter": (c_int, [ctypes.c_char_p]),
     "augur_timing_collect": (c_int, [P, c_size_t, P, P, c_int, P]),

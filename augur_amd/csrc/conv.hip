@@ -1100,6 +1100,71 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   }
 }
 
+// Adam gated on the device: the step counter and the skip flag live in
+// opt[0] (steps taken) and opt[2] (1: this step skipped); the gradient scale
+// (the clip factor) in *gscale.  Nothing is written when the step is skipped.
+__global__ void k_adam_gated(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                             float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                             const int* __restrict__ opt, const float* __restrict__ gscale,
+                             uint16_t* __restrict__ p_bf16) {
+  if (opt[2] != 0) return;
+  const int step = opt[0];
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  const float gs = *gscale;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const float gj = g[j] * gs;
+    const float mj = b1 * m[j] + (1.0f - b1) * gj;
+    const float vj = b2 * v[j] + (1.0f - b2) * gj * gj;
+    m[j] = mj;
+    v[j] = vj;
+    const float pj = p[j] - lr * (mj / c1) / (sqrtf(vj / c2) + eps);
+    p[j] = pj;
+    if (p_bf16) p_bf16[j] = f2bf(pj);
+  }
+}
+
+// acc[0] += sum g^2 over n values (FP64, one workgroup, fixed order)
+constexpr int GN_T = 1024;
+__global__ __launch_bounds__(GN_T) void k_grad_sumsq(const float* __restrict__ g, int64_t n, double* __restrict__ acc) {
+  double s = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += GN_T) s += (double)g[j] * (double)g[j];
+  __shared__ double red[GN_T / 64];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < GN_T / 64; ++w) t += red[w];
+    acc[0] += t;
+  }
+}
+
+// clip_grad_norm_(max_norm) and the non-finite skip: norm = sqrt(acc[0]);
+// skip when the norm or the loss is not finite; scale = min(1, max_norm /
+// (norm + 1e-6)); the step counter advances only on a step taken.  acc[0] is
+// cleared for the next step; acc[1] keeps the last norm.
+__global__ void k_grad_clip_decide(double* __restrict__ acc, const float* __restrict__ loss, float max_norm,
+                                   float* __restrict__ gscale, int* __restrict__ opt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double norm = sqrt(acc[0]);
+  const bool ok = isfinite(norm) && (loss == nullptr || isfinite(*loss));
+  float sc = 1.0f;
+  if (max_norm > 0.0f) {
+    const float c = max_norm / ((float)norm + 1e-6f);
+    sc = c < 1.0f ? c : 1.0f;
+  }
+  *gscale = sc;
+  acc[1] = norm;
+  acc[0] = 0.0;
+  if (ok) {
+    opt[0] += 1;
+    opt[2] = 0;
+  } else {
+    opt[1] += 1;
+    opt[2] = 1;
+  }
+}
+
 int conv_tiles_ok(int64_t G) { return is_pow2(G) && G >= 32 && G <= 1024; }
 
 #ifndef AUGUR_CONV_SPLIT
@@ -1347,6 +1412,31 @@ int augur_adam_update(float* p, const float* g, float* m, float* v, int64_t n, f
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
   hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p, g, m, v, n, lr, b1, b2, eps,
                      c1, c2, gscale, p_bf16);
+  AUGUR_LAUNCHED("adam");
+  return AUGUR_OK;
+}
+
+int augur_grad_sumsq(const float* g, int64_t n, double* acc, void* stream) {
+  AUGUR_REQUIRE(g && acc && n >= 0, AUGUR_EINVAL, "grad_sumsq: bad arguments");
+  if (n == 0) return AUGUR_OK;
+  hipLaunchKernelGGL(k_grad_sumsq, dim3(1), dim3(GN_T), 0, as_stream(stream), g, n, acc);
+  AUGUR_LAUNCHED("grad_sumsq");
+  return AUGUR_OK;
+}
+
+int augur_grad_clip_decide(double* acc, const float* loss, float max_norm, float* gscale, int* opt, void* stream) {
+  AUGUR_REQUIRE(acc && gscale && opt, AUGUR_EINVAL, "grad_clip_decide: bad arguments");
+  hipLaunchKernelGGL(k_grad_clip_decide, dim3(1), dim3(64), 0, as_stream(stream), acc, loss, max_norm, gscale, opt);
+  AUGUR_LAUNCHED("grad_clip_decide");
+  return AUGUR_OK;
+}
+
+int augur_adam_update_gated(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                            float eps, const int* opt, const float* gscale, uint16_t* p_bf16, void* stream) {
+  AUGUR_REQUIRE(p && g && m && v && opt && gscale && n >= 0, AUGUR_EINVAL, "adam_gated: bad arguments");
+  if (n == 0) return AUGUR_OK;
+  hipLaunchKernelGGL(k_adam_gated, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p, g, m, v, n, lr, b1,
+                     b2, eps, opt, gscale, p_bf16);
   AUGUR_LAUNCHED("adam");
   return AUGUR_OK;
 }

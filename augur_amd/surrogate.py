@@ -177,7 +177,7 @@ class LocalNet3D:
 
     def state_dict(self):
         return {"w": [w.cpu() for w in self.w], "b": [b.cpu() for b in self.b], "cin": self.cin,
-                "cout": self.cout, "layers": self.layers, "step": self.step_count}
+                "cout": self.cout, "layers": self.layers, "step": self.optimizer_steps[0]}
 
     def load_state_dict(self, sd):
         # in place when the shapes match, so a captured graph sees the new values
@@ -319,19 +319,68 @@ class LocalNet3D:
         self._saved = None
         return grads
 
-    def adam_step(self, grads, lr=3e-4, gscale=1.0):
-        """Adam on the FP32 masters (scripts/train_scatter_field.py uses torch.optim.Adam)."""
+    def adam_step(self, grads, lr=3e-4, loss=None, clip=1.0):
+        """Adam on the FP32 masters after the reference's gradient clip and
+        non-finite skip (scripts/train_scatter_field.py:225-235: clip_grad_norm_
+        at 1.0, no step when the loss or the norm is not finite), all decided on
+        the device: the step counter, the skip flag and the clip factor never
+        reach the host, so a captured training step keeps no host sync.
+        ``loss`` (a 1-element FP32/FP64 device tensor, optional) joins the
+        finiteness test; ``clip`` <= 0 disables clipping."""
+        dev = self.w[0].device
         if self.adam is None:
             self.adam = [((torch.zeros_like(w), torch.zeros_like(w)), (torch.zeros_like(b), torch.zeros_like(b)))
                          for w, b in zip(self.w, self.b)]
-        self.step_count += 1
-        for k, ((dw, db), w, b) in enumerate(zip(grads, self.w, self.b)):
+            # [steps taken, steps skipped, last step skipped]; [sum g^2, last norm]; clip factor
+            self.opt_state = torch.tensor([self.step_count, 0, 0], dtype=torch.int32, device=dev)
+            self.gacc = torch.zeros(2, dtype=torch.float64, device=dev)
+            self.gscale = torch.ones(1, dtype=torch.float32, device=dev)
+        stream = _stream(dev)
+        gl = []
+        for (dw, db), b in zip(grads, self.b):
             gb = torch.zeros_like(b)
             gb[: db.shape[0]] = db
+            gl.append((dw.contiguous(), gb))
+            for g in gl[-1]:
+                _lib.call("augur_grad_sumsq", _p(g), g.numel(), _p(self.gacc), stream)
+        lz = None if loss is None else loss.reshape(-1)[:1].to(torch.float32).contiguous()
+        _lib.call("augur_grad_clip_decide", _p(self.gacc), _p(lz), float(clip), _p(self.gscale), _p(self.opt_state),
+                  stream)
+        for k, ((dw, gb), w, b) in enumerate(zip(gl, self.w, self.b)):
             (mw, vw), (mb, vb) = self.adam[k]
-            adam_update(w, dw.contiguous(), mw, vw, lr, self.step_count, gscale=gscale)
-            adam_update(b, gb, mb, vb, lr, self.step_count, gscale=gscale)
+            for p, g, m, v in ((w, dw, mw, vw), (b, gb, mb, vb)):
+                _lib.call("augur_adam_update_gated", _p(p), _p(g), _p(m), _p(v), p.numel(), float(lr), 0.9, 0.999,
+                          1e-8, _p(self.opt_state), _p(self.gscale), None, stream)
         self.sync_bf16()
+
+    @property
+    def optimizer_steps(self):
+        """(steps taken, steps skipped) so far (reads the device counters)."""
+        if self.adam is None:
+            return self.step_count, 0
+        st = self.opt_state.cpu().tolist()
+        return st[0], st[1]
+
+    def opt_state_dict(self):
+        """The optimizer state: Adam moments per parameter and the counters."""
+        if self.adam is None:
+            return {"m": None, "v": None, "counters": [self.step_count, 0, 0]}
+        return {"m": [t.cpu() for mv in self.adam for t in (mv[0][0], mv[1][0])],
+                "v": [t.cpu() for mv in self.adam for t in (mv[0][1], mv[1][1])],
+                "counters": self.opt_state.cpu().tolist()}
+
+    def load_opt_state_dict(self, sd):
+        dev = self.w[0].device
+        if sd.get("m") is None:
+            self.adam = None
+            self.step_count = int(sd["counters"][0])
+            return
+        ms, vs = [t.to(dev) for t in sd["m"]], [t.to(dev) for t in sd["v"]]
+        self.adam = [((ms[2 * k], vs[2 * k]), (ms[2 * k + 1], vs[2 * k + 1])) for k in range(len(self.w))]
+        self.opt_state = torch.tensor(sd["counters"], dtype=torch.int32, device=dev)
+        self.gacc = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.gscale = torch.ones(1, dtype=torch.float32, device=dev)
+        self.step_count = int(sd["counters"][0])
 
 
 # --------------------------------------------------------------- PM surrogate
@@ -574,7 +623,8 @@ class PMSurrogate:
         at = a_true.to(self.device, torch.float64)
         norm = float((at ** 2).sum())
         r = a - at
-        loss = float((r ** 2).sum()) / norm
+        loss_t = (r ** 2).sum() / norm
+        loss = float(loss_t)
         if self.mode == "correction":
             res = rel_rms(a, self.gather(force, pt))
             self.train_residual = res if self.train_residual is None else 0.9 * self.train_residual + 0.1 * res
@@ -583,7 +633,7 @@ class PMSurrogate:
                                  mass_value=1.0)[0, 1:4]
         dy = (dfield * self.force_scale)[None].contiguous()
         grads = self.net.backward(dy)
-        self.net.adam_step(grads, lr=lr)
+        self.net.adam_step(grads, lr=lr, loss=loss_t)
         return loss
 
 
@@ -660,5 +710,5 @@ class BatchGraph:
 
     def step(self, *batch, lr=3e-4):
         loss, grads = self.grads_for(*batch)
-        self.sur.net.adam_step(grads, lr=lr)
+        self.sur.net.adam_step(grads, lr=lr, loss=loss)
         return loss

@@ -405,6 +405,23 @@ int augur_pack_features_bf16(const float* rho, const float* force, int64_t B, in
 int augur_adam_update(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                       float eps, int step, float gscale, uint16_t* p_bf16, void* stream);
 
+/* The training step's gradient-norm clip and non-finite skip, decided on the
+ * device (no host read, so a captured step stays capturable):
+ * augur_grad_sumsq() adds sum g^2 (FP64) of one gradient tensor to acc[0];
+ * augur_grad_clip_decide() takes norm = sqrt(acc[0]), skips the step when
+ * the norm or *loss (may be NULL) is not finite, sets *gscale = min(1,
+ * max_norm / (norm + 1e-6)) (max_norm <= 0: 1), advances the step counter
+ * opt[0] on a step taken or the skip counter opt[1] on a skip, sets opt[2]
+ * to 1 on a skip (0 otherwise), keeps the norm in acc[1] and clears acc[0];
+ * augur_adam_update_gated() is augur_adam_update with the step from opt[0],
+ * the gradient times *gscale, and no write at all while opt[2] != 0.
+ * Replaces clip_grad_norm_(1.0) and the isfinite skip of
+ * scripts/train_scatter_field.py:225-235. */
+int augur_grad_sumsq(const float* g, int64_t n, double* acc, void* stream);
+int augur_grad_clip_decide(double* acc, const float* loss, float max_norm, float* gscale, int* opt, void* stream);
+int augur_adam_update_gated(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                            float eps, const int* opt, const float* gscale, uint16_t* p_bf16, void* stream);
+
 /* ------------------------------------- 2D reference scatter-field step */
 
 /* FP32 3x3 convolution with zero padding, bias and an epilogue, on

@@ -97,7 +97,7 @@ def test_batch_graph_replays_match_eager(dev):
     for k in range(6):
         b = batches[k % 3]
         l1, g1 = batch_grads(s1, *b)
-        s1.net.adam_step(g1, lr=1e-3)
+        s1.net.adam_step(g1, lr=1e-3, loss=l1)
         l2 = gr.step(*b, lr=1e-3)
         torch.cuda.synchronize()
         assert torch.equal(l1, l2), k
