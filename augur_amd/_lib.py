@@ -39,6 +39,7 @@ SIGNATURES = {
     "augur_neg_grad_fd": (c_int, [P, c_int64, c_int64, c_float, P, P]),
     "augur_direct_sum_workspace_bytes": (c_size_t, [c_int64, c_int64, c_int]),
     "augur_direct_sum_f64": (c_int, [P, P, P, c_int64, c_int, P, c_int64, c_double, P, P, c_size_t, P]),
+    "augur_direct_sum_f64_noalloc": (c_int, [P, P, P, c_int64, c_int, P, c_int64, c_double, P, P]),
     "augur_direct_potential_f64": (c_int, [P, P, P, c_int64, c_int, P, c_int64, c_double, P, P, c_size_t, P]),
     "augur_ewald_workspace_bytes": (c_size_t, [c_int64, c_int64, c_int]),
     "augur_ewald_f64": (c_int, [P, P, c_double, c_int64, P, c_int64, c_double, c_double, c_double, c_int, P, P,

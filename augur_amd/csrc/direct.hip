@@ -2,6 +2,8 @@
 // sum (periodic cube, mean density removed).  Work is split over
 // (target block, source slice); each block writes a partial sum and a second
 // pass adds the slices in a fixed order, so results are run-to-run identical.
+#include <mutex>
+
 #include "common.hpp"
 
 using namespace augur;
@@ -275,6 +277,56 @@ int augur_direct_sum_f64(const double* pos64, const float* pos32, const double* 
                          const int64_t* targets, int64_t t, double eps, double* out, void* ws, size_t ws_bytes,
                          void* stream) {
   return direct_impl(false, pos64, pos32, mass, n, D, targets, t, eps, out, ws, ws_bytes, stream);
+}
+
+// ------------------------------------------------ no-allocation variant
+// The partial sums of augur_direct_sum_f64 in a static device array instead
+// of a caller workspace, for callers that cannot allocate.  Every call on a
+// device shares that device's copy; each partial it reads is written earlier
+// in the same call, so nothing needs clearing, but calls that overlap on one
+// device race.
+constexpr size_t DS_SCRATCH_DOUBLES = (size_t)1 << 21;  // 16 MiB per device
+static __device__ __align__(256) double scratch[DS_SCRATCH_DOUBLES];
+
+constexpr int DS_MAX_DEVICES = 64;
+static void* g_scratch_ptr[DS_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// the device address of this device's `scratch` (looked up once per device,
+// with that device current); host code never uses the array's name itself
+static int scratch_address(double** out) {
+  int dev = -1;
+  AUGUR_HIP(hipGetDevice(&dev));
+  AUGUR_REQUIRE(dev >= 0 && dev < DS_MAX_DEVICES, AUGUR_EUNSUPPORTED, "direct_sum_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("direct_sum_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return AUGUR_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<double*>(g_scratch_ptr[dev]);
+  return AUGUR_OK;
+}
+
+int augur_direct_sum_f64_noalloc(const double* pos64, const float* pos32, const double* mass, int64_t n, int D,
+                                 const int64_t* targets, int64_t t, double eps, double* out, void* stream) {
+  AUGUR_REQUIRE(D == 2 || D == 3, AUGUR_EINVAL, "direct_sum_noalloc: D must be 2 or 3");
+  AUGUR_REQUIRE(n >= 1 && t >= 0, AUGUR_EINVAL, "direct_sum_noalloc: bad sizes");
+  if (t == 0) return AUGUR_OK;
+  const size_t need = (size_t)ds_splits(t, n) * (size_t)t * (size_t)D;
+  AUGUR_REQUIRE(need <= DS_SCRATCH_DOUBLES, AUGUR_EINVAL,
+                "direct_sum_noalloc: %zu partial sums exceed the static scratch of %zu doubles (at most 699050 "
+                "targets in 3D); use augur_direct_sum_f64 with a workspace",
+                need, DS_SCRATCH_DOUBLES);
+  double* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return direct_impl(false, pos64, pos32, mass, n, D, targets, t, eps, out, ws,
+                     DS_SCRATCH_DOUBLES * sizeof(double), stream);
 }
 
 int augur_direct_potential_f64(const double* pos64, const float* pos32, const double* mass, int64_t n, int D,

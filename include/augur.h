@@ -161,8 +161,19 @@ int augur_direct_sum_f64(const double* pos64, const float* pos32, const double* 
                          const int64_t* targets, int64_t t, double eps, double* out, void* ws, size_t ws_bytes,
                          void* stream);
 
+/* augur_direct_sum_f64 with its partial sums in a static device array of the
+ * library (16 MiB per GPU, held until the process ends) instead of a caller
+ * workspace: no allocation of any kind.  No counterpart in the reference; it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Refuses (AUGUR_EINVAL)
+ * sizes whose partials exceed the array (above 699050 targets in 3D); every
+ * call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * race. */
+int augur_direct_sum_f64_noalloc(const double* pos64, const float* pos32, const double* mass, int64_t n, int D,
+                                 const int64_t* targets, int64_t t, double eps, double* out, void* stream);
+
 /* FP64 softened potential sums phi_t = sum_{j != idx_t} m_j (|x_j - x_t|^2 +
- * eps^2)^-1/2, out (t) FP64, same arguments and workspace as above.  The
+ * eps^2)^-1/2, out (t) FP64, same arguments and workspace as augur_direct_sum_f64.  The
  * pair-potential part of exact_energy (scripts/scatter_scaling_ops.py:90-103)
  * and gravity_sim.energy (scripts/gravity_sim.py:14-18) is -0.5 sum m_t phi_t. */
 int augur_direct_potential_f64(const double* pos64, const float* pos32, const double* mass, int64_t n, int D,
