@@ -85,6 +85,15 @@ SIGNATURES = {
     "augur_grad_sumsq": (c_int, [P, c_int64, P, P]),
     "augur_grad_clip_decide": (c_int, [P, P, c_float, P, P, P]),
     "augur_adam_update_gated": (c_int, [P, P, P, P, c_int64, c_float, c_float, c_float, c_float, P, P, P, P]),
+    "augur_lp_encode": (c_int, [P, c_int, P, c_int, c_int64, c_float, P, P, P]),
+    "augur_lp_decode": (c_int, [P, c_int, P, c_int, c_int64, c_float, P, P, P]),
+    "augur_lp_drift_deposit": (c_int, [P, c_int, P, c_int, c_int64, c_int64, c_int64, c_float, c_float, c_int64,
+                                       c_float, P, P]),
+    "augur_lp_finish": (c_int, [P, c_int64, c_int64, c_float, c_float, P, P]),
+    "augur_pm_solve_grid": (c_int, [P, P, c_int64, c_float, c_float, P, c_size_t, P]),
+    "augur_grid_mean_workspace_bytes": (c_size_t, []),
+    "augur_grid_mean_acc_f64": (c_int, [P, P, c_int64, P, P, c_size_t, P]),
+    "augur_lp_kick": (c_int, [P, c_int, P, c_int, c_int64, c_int64, c_float, P, P, c_int64, c_float, P]),
     "augur_timing_enable": (c_int, [c_int]),
     "augur_timing_filter": (c_int, [ctypes.c_char_p]),
     "augur_timing_collect": (c_int, [P, c_size_t, P, P, c_int, P]),

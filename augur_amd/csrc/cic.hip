@@ -430,38 +430,6 @@ __device__ __forceinline__ int64_t scene_of(const int64_t* __restrict__ starts, 
   return lo;
 }
 
-// acc[key] += q for every active lane of the wave, with runs of equal keys in
-// adjacent lanes summed first (a segmented scan over the wave) so that only
-// the run's last lane issues the atomic.  Cell-sorted particles put a dense
-// cell's visits in adjacent lanes, where same-address atomics would
-// serialise; the integer sums are exact, so merging changes no bits.  Every
-// lane of the wave must call it (inactive lanes with active = false).
-__device__ __forceinline__ void wave_merged_add(unsigned long long* __restrict__ acc, int64_t key, long long q,
-                                                bool active) {
-  const int lane = (int)(threadIdx.x & 63);
-  const int64_t k = active ? key : (int64_t)(-1 - lane);  // inactive lanes match nothing
-  const int64_t kprev = __shfl_up(k, 1);
-  const bool head = lane == 0 || kprev != k;
-  if (__ballot(head) == ~0ull) {  // no runs in this wave: one atomic per lane
-    if (active && q) atomicAdd(&acc[key], (unsigned long long)q);
-    return;
-  }
-  long long v = active ? q : 0;
-  int f = head ? 1 : 0;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const long long vo = __shfl_up(v, off);
-    const int fo = __shfl_up(f, off);
-    if (lane >= off && !f) {
-      v += vo;
-      f |= fo;
-    }
-  }
-  const int64_t knext = __shfl_down(k, 1);
-  const bool tail = lane == 63 || knext != k;
-  if (active && tail && v) atomicAdd(&acc[key], (unsigned long long)v);
-}
-
 // starts (may be NULL): ragged batch, scene b holds particles [starts[b], starts[b+1]);
 // otherwise B scenes of N particles each.  The loop runs in whole waves (a
 // lane past the end is inactive) so that wave_merged_add sees every lane.

@@ -121,6 +121,38 @@ __device__ __forceinline__ long long fixed_round(float v, int e) {
   return r < 0.0f ? -(long long)q : (long long)q;
 }
 
+// acc[key] += q for every active lane of the wave, with runs of equal keys in
+// adjacent lanes summed first (a segmented scan over the wave) so that only
+// the run's last lane issues the atomic.  Cell-sorted particles put a dense
+// cell's visits in adjacent lanes, where same-address atomics would
+// serialise; the integer sums are exact, so merging changes no bits.  Every
+// lane of the wave must call it (inactive lanes with active = false).
+__device__ __forceinline__ void wave_merged_add(unsigned long long* __restrict__ acc, int64_t key, long long q,
+                                                bool active) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t k = active ? key : (int64_t)(-1 - lane);  // inactive lanes match nothing
+  const int64_t kprev = __shfl_up(k, 1);
+  const bool head = lane == 0 || kprev != k;
+  if (__ballot(head) == ~0ull) {  // no runs in this wave: one atomic per lane
+    if (active && q) atomicAdd(&acc[key], (unsigned long long)q);
+    return;
+  }
+  long long v = active ? q : 0;
+  int f = head ? 1 : 0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const long long vo = __shfl_up(v, off);
+    const int fo = __shfl_up(f, off);
+    if (lane >= off && !f) {
+      v += vo;
+      f |= fo;
+    }
+  }
+  const int64_t knext = __shfl_down(k, 1);
+  const bool tail = lane == 63 || knext != k;
+  if (active && tail && v) atomicAdd(&acc[key], (unsigned long long)v);
+}
+
 // h = L / G in FP32 is a power of two (then cic_u_pow2 applies)
 inline bool pow2_float(float h) {
   int ex;

@@ -1393,11 +1393,9 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
   return AUGUR_OK;
 }
 
-// deposit -> FFT Poisson + spectral gradient -> force planes
-static int pm_force_grid(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
-                         int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
-  int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s);
-  if (rc) return rc;
+// FFT Poisson + spectral gradient: b.rho -> force planes b.force
+static int pm_solve(int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
+  int rc;
   {
     TimedScope ts_("fft_rows_r2c", s);
     PM_SWITCH_G(G, k_pm_rows_r2c, dim3((unsigned)(G * G / PmRow<GG>::NL)), dim3(FFT_THREADS), s,
@@ -1426,6 +1424,14 @@ static int pm_force_grid(const float* pos, const uint32_t* perm, const float* ma
     AUGUR_LAUNCHED("pm_xc2r");
   }
   return AUGUR_OK;
+}
+
+// deposit -> FFT Poisson + spectral gradient -> force planes
+static int pm_force_grid(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
+                         int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
+  const int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s);
+  if (rc) return rc;
+  return pm_solve(G, L, Gconst, b, s);
 }
 
 // ------------------------------------------------- surrogate residual gate
@@ -1745,6 +1751,20 @@ int augur_pm_kdk_diag(float* pos, float* vel, float* acc, float* mass, int32_t* 
     if (rc) return rc;
   }
   return AUGUR_OK;
+}
+
+int augur_pm_solve_grid(const float* rho, float* force, int64_t G, float L, float Gconst, void* ws, size_t ws_bytes,
+                        void* stream) {
+  int rc = check_pm(1, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(rho && force, AUGUR_EINVAL, "pm_solve_grid: null pointer");
+  Carver c(ws, ws_bytes);
+  PmBufs b;
+  AUGUR_REQUIRE(carve_pm(c, 1, G, b), AUGUR_EWORKSPACE, "pm_solve_grid: workspace too small (%zu < %zu)", ws_bytes,
+                c.used);
+  b.rho = const_cast<float*>(rho);   // read only by the solve
+  b.force = force;
+  return pm_solve(G, L, Gconst, b, as_stream(stream));
 }
 
 int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, void* ws, size_t ws_bytes,

@@ -318,6 +318,45 @@ int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, v
 int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B, int64_t total,
                           const float* force, int64_t G, float L, float dt, int phase, void* stream);
 
+/* ------------------------------------- large-N driver with compact state */
+
+/* Particle state in compact encodings, 3 values per particle: positions
+ * penc 0 FP32, 1 FIX32 (x = q L / 2^32, uint32), 2 FIX16 (x = q L / 2^16,
+ * uint16) -- fixed point wraps the periodic box [0, L) by integer overflow --
+ * and velocities venc 0 FP32, 1 FP16, 2 BF16 (round to nearest even).
+ * augur_lp_encode / augur_lp_decode convert n particles from / to FP32 (either
+ * array may be NULL).  The state is used in the merged KDK form M = v +
+ * (dt/2) a of the reference's billion-body loop (scripts/scatter_bh_1b.py:
+ * 1-9,107-146, with its fp16/bf16 state, :32,102-103):
+ * augur_lp_drift_deposit(): particles [lo, hi) of n_total, x += dt M (dt = 0:
+ *   no drift) stored back in the encoding, then the fixed-point CIC deposit
+ *   of the PM stepper (mass mass_value each, scale 1/h^3) added into img
+ *   (G^3 int64 units, kept zero between steps), so the density is bitwise the
+ *   same for any chunking and order;
+ * augur_lp_finish(): rho (G^3 FP32) from img, img cleared;
+ * augur_pm_solve_grid(): the PM force planes of a density grid (the fused FFT
+ *   passes of augur_pm_step_begin; workspace augur_pm_workspace_bytes(1, G));
+ * augur_grid_mean_acc_f64(): out[0..2] = sum_c rho_c a_c, out[3] = sum_c rho_c
+ *   (FP64, fixed order), the grid-weighted mean acceleration of
+ *   scatter_bh_1b.py:119-120 (workspace augur_grid_mean_workspace_bytes());
+ * augur_lp_kick(): particles [lo, hi): M += kick (a(x) - out[0..2]/out[3])
+ *   with a the CIC gather of the planes (abar NULL: no mean removed). */
+int augur_lp_encode(void* pos, int penc, void* vel, int venc, int64_t n, float L, const float* pin, const float* vin,
+                    void* stream);
+int augur_lp_decode(const void* pos, int penc, const void* vel, int venc, int64_t n, float L, float* pout,
+                    float* vout, void* stream);
+int augur_lp_drift_deposit(void* pos, int penc, const void* vel, int venc, int64_t lo, int64_t hi, int64_t n_total,
+                           float mass_value, float dt, int64_t G, float L, unsigned long long* img, void* stream);
+int augur_lp_finish(unsigned long long* img, int64_t G, int64_t n_total, float mass_value, float L, float* rho,
+                    void* stream);
+int augur_pm_solve_grid(const float* rho, float* force, int64_t G, float L, float Gconst, void* ws, size_t ws_bytes,
+                        void* stream);
+size_t augur_grid_mean_workspace_bytes(void);
+int augur_grid_mean_acc_f64(const float* rho, const float* force, int64_t G, double* out, void* ws, size_t ws_bytes,
+                            void* stream);
+int augur_lp_kick(const void* pos, int penc, void* vel, int venc, int64_t lo, int64_t hi, float kick,
+                  const float* force, const double* abar, int64_t G, float L, void* stream);
+
 /* ------------------------------------------ a11 / a12 surrogate network */
 
 /* BF16 MFMA 3x3x3 periodic convolution, FP32 accumulation, on a G^3 grid
