@@ -29,6 +29,9 @@ SIGNATURES = {
                                   c_float, P, P, c_size_t, P]),
     "augur_cic_deposit_ragged": (c_int, [P, P, P, c_float, P, c_int64, c_int64, c_int, c_int64, c_float, c_float,
                                          c_int, c_float, P, P, c_size_t, P]),
+    "augur_tsc_deposit": (c_int, [P, P, P, c_float, P, c_int64, c_int64, c_int, c_int64, c_float, c_float, c_int,
+                                  c_float, P, P, c_size_t, P]),
+    "augur_tsc_gather": (c_int, [P, P, c_int64, c_int64, c_int, c_int64, c_int, c_float, c_float, c_int, P, P]),
     "augur_cic_gather": (c_int, [P, P, c_int64, c_int64, c_int, c_int64, c_int, c_float, c_float, c_int, P, P]),
     "augur_fft_workspace_bytes": (c_size_t, [c_int, P, c_int64]),
     "augur_fft_r2c": (c_int, [P, P, P, c_int, P, c_int64, P, c_size_t, P]),

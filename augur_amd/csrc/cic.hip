@@ -553,6 +553,118 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ field,
   }
 }
 
+// ------------------------------------------------- TSC ("quad") stencil
+// ScatterField._quad_axis (scripts/scatter_field.py:184-188): i0 = round(u)
+// (half to even), d = u - i0, weights 0.5 (0.5 - d)^2, 0.75 - d^2,
+// 0.5 (0.5 + d)^2 at offsets -1, 0, +1; the corner weight is the product
+// over axes in axis order; off-grid corners weigh 0 (or wrap when periodic).
+__device__ __forceinline__ void tsc_axis(float u, int& i0, float w[3]) {
+  const float r = rintf(u);
+  i0 = (int)r;
+  const float d = __fsub_rn(u, r);
+  const float a = __fsub_rn(0.5f, d), b = __fadd_rn(0.5f, d);
+  w[0] = __fmul_rn(0.5f, __fmul_rn(a, a));
+  w[1] = __fsub_rn(0.75f, __fmul_rn(d, d));
+  w[2] = __fmul_rn(0.5f, __fmul_rn(b, b));
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_dep_tsc(const float* __restrict__ pos, const float* __restrict__ vel,
+                                                 const float* __restrict__ mass, float mass_value,
+                                                 const float* __restrict__ mask, int64_t B, int64_t N, int64_t tot,
+                                                 int G, float off, float h, int periodic, float scale, int C,
+                                                 const unsigned* __restrict__ maxbits,
+                                                 unsigned long long* __restrict__ acc) {
+  int E[4];
+  for (int c = 0; c < C; ++c) E[c] = fixed_exponent(maxbits[c], tot);
+  int64_t GD = 1;
+  for (int d = 0; d < D; ++d) GD *= G;
+  constexpr int NC = D == 2 ? 9 : 27;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < tot; base += stride) {
+    const int64_t ii = base + threadIdx.x;
+    const bool live = ii < tot;
+    const int64_t i = live ? ii : tot - 1;
+    const int64_t b = i / N;
+    float mm = mass ? mass[i] : mass_value;
+    if (mask) mm = __fmul_rn(mm, mask[i]);
+    float valE[4];
+    for (int c = 0; c < C; ++c) valE[c] = ldexpf(chan_value(c, mm, vel, i, D, scale), E[c]);
+    int i0[D];
+    float w[D][3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) tsc_axis(cic_u(pos[i * D + d], off, h), i0[d], w[d]);
+#pragma unroll
+    for (int o = 0; o < NC; ++o) {
+      float wt = 1.0f;
+      bool ok = live;
+      int64_t flat = 0, mul = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int od = (d == 0 ? o : (d == 1 ? o / 3 : o / 9)) % 3;
+        wt = (d == 0) ? w[d][od] : __fmul_rn(wt, w[d][od]);
+        int ix = i0[d] + od - 1;
+        if (periodic) {
+          ix = wrap(ix, G);
+        } else {
+          ok = ok && ix >= 0 && ix < G;
+        }
+        flat += (int64_t)ix * mul;
+        mul *= G;
+      }
+      for (int c = 0; c < C; ++c) {
+        const long long q = fixed_round(__fmul_rn(valE[c], wt), 0);
+        wave_merged_add(acc, ((b * C) + c) * GD + (ok ? flat : 0), q, ok);
+      }
+    }
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_gather_tsc(const float* __restrict__ field, const float* __restrict__ pos,
+                                                    int64_t B, int64_t N, int G, int C, float off, float h,
+                                                    int periodic, float* __restrict__ out) {
+  int64_t GD = 1;
+  for (int d = 0; d < D; ++d) GD *= G;
+  constexpr int NC = D == 2 ? 9 : 27;
+  const int64_t tot = B * N;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / N;
+    int i0[D];
+    float w[D][3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) tsc_axis(cic_u(pos[i * D + d], off, h), i0[d], w[d]);
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // the reference's corner order: x offset outermost (scripts/scatter_field.py:193-201)
+#pragma unroll
+    for (int o = 0; o < NC; ++o) {
+      int od[3];
+      od[0] = o / (NC / 3);
+      od[1] = (o / (D == 3 ? 3 : 1)) % 3;
+      od[2] = o % 3;
+      float wt = 1.0f;
+      bool ok = true;
+      int64_t flat = 0, mul = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        wt = (d == 0) ? w[d][od[d]] : __fmul_rn(wt, w[d][od[d]]);
+        int ix = i0[d] + od[d] - 1;
+        if (periodic) {
+          ix = wrap(ix, G);
+        } else {
+          ok = ok && ix >= 0 && ix < G;
+          ix = min(max(ix, 0), G - 1);
+        }
+        flat += (int64_t)ix * mul;
+        mul *= G;
+      }
+      if (!ok) wt = 0.0f;
+      for (int c = 0; c < C && c < 8; ++c) acc[c] += field[(b * C + c) * GD + flat] * wt;
+    }
+    for (int c = 0; c < C && c < 8; ++c) out[i * C + c] = acc[c];
+  }
+}
+
 // ------------------------------------------------------- FP64 reductions
 constexpr int RED_BLOCKS = 1024, RED_THREADS = 256;
 
@@ -825,6 +937,65 @@ int augur_cic_gather(const float* field, const float* pos, int64_t B, int64_t N,
     hipLaunchKernelGGL(k_gather<3>, dim3(blocks_for(B * N, 256)), dim3(256), 0, s, field, pos, B, N, (int)G, C, off,
                        h, periodic, out);
   AUGUR_LAUNCHED("gather");
+  return AUGUR_OK;
+}
+
+int augur_tsc_deposit(const float* pos, const float* vel, const float* mass, float mass_value, const float* mask,
+                      int64_t B, int64_t N, int D, int64_t G, float off, float h, int periodic, float scale, float* out,
+                      void* ws, size_t ws_bytes, void* stream) {
+  AUGUR_REQUIRE(D == 2 || D == 3, AUGUR_EINVAL, "tsc_deposit: D must be 2 or 3");
+  AUGUR_REQUIRE(B >= 1 && N >= 0 && G >= 2 && h > 0.0f, AUGUR_EINVAL, "tsc_deposit: bad sizes");
+  AUGUR_REQUIRE(!periodic || is_pow2(G), AUGUR_EUNSUPPORTED, "tsc_deposit: periodic needs power-of-two G");
+  AUGUR_REQUIRE(G <= 4096, AUGUR_EUNSUPPORTED, "tsc_deposit: G too large");
+  const int C = vel ? 1 + D : 1;
+  int64_t GD = 1;
+  for (int d = 0; d < D; ++d) GD *= G;
+  hipStream_t s = as_stream(stream);
+  Carver c(ws, ws_bytes);
+  unsigned long long* acc = c.take<unsigned long long>(B * C * GD);
+  unsigned* maxbits = c.take<unsigned>(4);
+  AUGUR_REQUIRE(c.ok(), AUGUR_EWORKSPACE, "tsc_deposit: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  AUGUR_REQUIRE(out && (N == 0 || pos), AUGUR_EINVAL, "tsc_deposit: null pointer");
+  int rc = fill_u64(acc, B * C * GD, 0ull, s);
+  if (rc) return rc;
+  rc = fill_u32(maxbits, 4, 0u, s);
+  if (rc) return rc;
+  const int64_t tot = B * N;
+  if (tot > 0) {
+    hipLaunchKernelGGL(k_dep_absmax, dim3(blocks_for(tot, 256, 2048)), dim3(256), 0, s, vel, mass, mass_value, mask,
+                       tot, D, C, scale, maxbits);
+    AUGUR_LAUNCHED("dep_absmax");
+    TimedScope ts_("dep_tsc", s);
+    if (D == 2)
+      hipLaunchKernelGGL(k_dep_tsc<2>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, pos, vel, mass, mass_value, mask,
+                         B, N, tot, (int)G, off, h, periodic, scale, C, maxbits, acc);
+    else
+      hipLaunchKernelGGL(k_dep_tsc<3>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, pos, vel, mass, mass_value, mask,
+                         B, N, tot, (int)G, off, h, periodic, scale, C, maxbits, acc);
+    AUGUR_LAUNCHED("dep_tsc");
+  }
+  hipLaunchKernelGGL(k_dep_convert, dim3(blocks_for(B * C * GD, 256)), dim3(256), 0, s, acc, out, B, C, GD,
+                     tot > 0 ? tot : 1, maxbits);
+  AUGUR_LAUNCHED("dep_convert");
+  return AUGUR_OK;
+}
+
+int augur_tsc_gather(const float* field, const float* pos, int64_t B, int64_t N, int D, int64_t G, int C, float off,
+                     float h, int periodic, float* out, void* stream) {
+  AUGUR_REQUIRE(D == 2 || D == 3, AUGUR_EINVAL, "tsc_gather: D must be 2 or 3");
+  AUGUR_REQUIRE(B >= 1 && N >= 0 && G >= 2 && C >= 1 && C <= 8 && h > 0.0f, AUGUR_EINVAL,
+                "tsc_gather: bad sizes (C must be 1..8)");
+  AUGUR_REQUIRE(!periodic || is_pow2(G), AUGUR_EUNSUPPORTED, "tsc_gather: periodic needs power-of-two G");
+  if (B * N == 0) return AUGUR_OK;
+  AUGUR_REQUIRE(field && pos && out, AUGUR_EINVAL, "tsc_gather: null pointer");
+  hipStream_t s = as_stream(stream);
+  if (D == 2)
+    hipLaunchKernelGGL(k_gather_tsc<2>, dim3(blocks_for(B * N, 256)), dim3(256), 0, s, field, pos, B, N, (int)G, C,
+                       off, h, periodic, out);
+  else
+    hipLaunchKernelGGL(k_gather_tsc<3>, dim3(blocks_for(B * N, 256)), dim3(256), 0, s, field, pos, B, N, (int)G, C,
+                       off, h, periodic, out);
+  AUGUR_LAUNCHED("tsc_gather");
   return AUGUR_OK;
 }
 

@@ -103,6 +103,35 @@ def cic_deposit(pos, mass, G, h, off, periodic, vel=None, mask=None, scale=None,
     return out
 
 
+def tsc_deposit(pos, mass, G, h, off, periodic, vel=None, mask=None, scale=None, mass_value=1.0):
+    """The TSC ("quad") deposit (augur_tsc_deposit); arguments as cic_deposit."""
+    pos = _c(pos, F32)
+    dev = _dev(pos)
+    B, N, D = pos.shape
+    mass, vel, mask = _c(mass, F32), _c(vel, F32), _c(mask, F32)
+    C = 1 if vel is None else 1 + D
+    if scale is None:
+        scale = 1.0 / h ** D
+    out = torch.empty((B, C) + (G,) * D, dtype=F32, device=dev)
+    wsb = _lib.size("augur_cic_deposit_workspace_bytes", B, D, G, C)
+    ws = _ws(wsb, dev)
+    _lib.call("augur_tsc_deposit", _p(pos), _p(vel), _p(mass), float(mass_value), _p(mask), B, N, D, int(G),
+              float(off), float(h), int(bool(periodic)), float(scale), _p(out), _p(ws), wsb, _stream(dev))
+    return out
+
+
+def tsc_gather(field, pos, h, off, periodic):
+    """The TSC gather (augur_tsc_gather): field (B, C, G..G), pos (B, N, D) -> (B, N, C)."""
+    field, pos = _c(field, F32), _c(pos, F32)
+    dev = _dev(field, pos)
+    B, N, D = pos.shape
+    C, G = field.shape[1], field.shape[-1]
+    out = torch.empty(B, N, C, dtype=F32, device=dev)
+    _lib.call("augur_tsc_gather", _p(field), _p(pos), B, N, D, int(G), int(C), float(off), float(h),
+              int(bool(periodic)), _p(out), _stream(dev))
+    return out
+
+
 def cic_deposit_ragged(pos, starts, G, h, off, periodic, mass=None, vel=None, scale=None, mass_value=1.0):
     """Ragged batch: pos (total, D) with scene b = rows [starts[b], starts[b+1]),
     starts an int64 tensor of B+1 offsets on the device; mass (total,) or

@@ -107,7 +107,7 @@ class ScatterField(nn.Module):
                  deposit="cic", dual=False, dtin=False, pairctx=False, *, dim=2, periodic=False, pm=False,
                  Gconst=1.0, surrogate=None):
         super().__init__()
-        assert deposit == "cic", "only the CIC deposit runs on the GPU path"
+        assert deposit in ("cic", "quad")
         if prior or dual or massfeat or cu or epsin or norm or dtin or pairctx:
             raise NotImplementedError("prior / dual / mass features / norms / pair context are outside this path")
         assert dim in (2, 3)
@@ -175,14 +175,17 @@ class ScatterField(nn.Module):
 
     # ------------------------------------------------------ deposit / gather
     def scatter(self, pos, vel, mass, mask):
-        """CIC deposit of [m, m v_0, .., m v_{D-1}] * in_scale / h^D
-        (scripts/scatter_field.py:237-250) -> (B, 1+D, G, .., G)."""
+        """CIC (or, with deposit="quad", TSC) deposit of [m, m v_0, .., m v_{D-1}]
+        * in_scale / h^D (scripts/scatter_field.py:237-250) -> (B, 1+D, G, .., G)."""
         D = pos.shape[-1]
-        return ops.cic_deposit(pos, mass, self.grid, self.h, self.off, self.periodic, vel=vel, mask=mask,
-                               scale=float(self.in_scale / self.h ** D))
+        dep = ops.tsc_deposit if self.deposit == "quad" else ops.cic_deposit
+        return dep(pos, mass, self.grid, self.h, self.off, self.periodic, vel=vel, mask=mask,
+                   scale=float(self.in_scale / self.h ** D))
 
     def gather(self, field, pos):
-        """CIC gather (scripts/scatter_field.py:252-261) -> (B, N, C)."""
+        """CIC (or, with deposit="quad", TSC) gather (scripts/scatter_field.py:252-261) -> (B, N, C)."""
+        if self.deposit == "quad":
+            return ops.tsc_gather(field, pos, self.h, self.off, self.periodic)
         return ops.cic_gather(field, pos, self.h, self.off, self.periodic)
 
     def neg_grad(self, phi, pad="zero"):

@@ -98,6 +98,20 @@ int augur_cic_deposit_ragged(const float* pos, const float* vel, const float* ma
 int augur_cic_gather(const float* field, const float* pos, int64_t B, int64_t N, int D, int64_t G, int C, float off,
                      float h, int periodic, float* out, void* stream);
 
+/* The TSC ("quad") deposit and gather: the 3-point quadratic-spline stencil of
+ * ScatterField._quad_axis (scripts/scatter_field.py:184-188, used by
+ * _corner_iter / _corners4 when deposit="quad", :190-235) in place of CIC's
+ * 2 points: i0 = round(u) half to even, d = u - i0, axis weights
+ * 0.5 (0.5 - d)^2, 0.75 - d^2, 0.5 (0.5 + d)^2 at i0 - 1, i0, i0 + 1.
+ * Arguments, workspace (augur_cic_deposit_workspace_bytes), channels, the
+ * fixed-point order-independent sums and the open/periodic edges are those
+ * of augur_cic_deposit / augur_cic_gather. */
+int augur_tsc_deposit(const float* pos, const float* vel, const float* mass, float mass_value, const float* mask,
+                      int64_t B, int64_t N, int D, int64_t G, float off, float h, int periodic, float scale, float* out,
+                      void* ws, size_t ws_bytes, void* stream);
+int augur_tsc_gather(const float* field, const float* pos, int64_t B, int64_t N, int D, int64_t G, int C, float off,
+                     float h, int periodic, float* out, void* stream);
+
 /* ---------------------------------------------------------------- a4 FFT */
 
 /* Batched real-to-complex FFT over the last ndim (2 or 3) axes; shape =
