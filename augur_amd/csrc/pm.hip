@@ -422,47 +422,46 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
         int64_t sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
         // slots first (LDS walk), then each kind of load for all visits
         // together, so the position loads are in flight at the same time
-        // (a load that depends on another, the perm lookup, is its own pass)
-        int64_t sl[TD_K];
+        // (a load that depends on another, the perm lookup, is its own pass).
+        // A visit past the lane's range reads the lane's first slot (always
+        // valid) and adds nothing: no branches around the loads.
+        uint32_t sl[TD_K];
+        bool vok[TD_K];
 #pragma unroll
         for (int u = 0; u < TD_K; ++u) {
           const uint32_t ww = l0 + u;
-          sl[u] = -1;
-          if (ww < l1) {
+          vok[u] = ww < l1;
+          if (vok[u]) {
             while (ww >= snext) {
               ++seg;
               snext = seg_pre[seg + 1];
               sbase = (int64_t)seg_lo[seg] - seg_pre[seg];
             }
-            sl[u] = sbase + ww;
+            sl[u] = (uint32_t)(sbase + ww);
+          } else {
+            sl[u] = sl[0];  // visit 0 is always in range (l0 < phi)
           }
         }
         if (perm) {
 #pragma unroll
-          for (int u = 0; u < TD_K; ++u)
-            if (sl[u] >= 0) sl[u] = (int64_t)perm[sl[u]];
+          for (int u = 0; u < TD_K; ++u) sl[u] = perm[sl[u]];
         }
         float px[TD_K][3], pm[TD_K];
 #pragma unroll
         for (int u = 0; u < TD_K; ++u) {
-          px[u][0] = px[u][1] = px[u][2] = 0.0f;
-          if (sl[u] >= 0) {
-            px[u][0] = pos[3 * sl[u]];
-            px[u][1] = pos[3 * sl[u] + 1];
-            px[u][2] = pos[3 * sl[u] + 2];
-          }
+          const float* pp = pos + 3 * (uint64_t)sl[u];
+          px[u][0] = pp[0];
+          px[u][1] = pp[1];
+          px[u][2] = pp[2];
         }
-#pragma unroll
-        for (int u = 0; u < TD_K; ++u) pm[u] = mass_value;
         if (mass) {
 #pragma unroll
-          for (int u = 0; u < TD_K; ++u)
-            if (sl[u] >= 0) pm[u] = mass[sl[u]];
+          for (int u = 0; u < TD_K; ++u) pm[u] = mass[sl[u]];
         }
         int rkey = -1;
         double run[8];
 #pragma unroll
-        for (int o = 0; o < 8; ++o) run[o] = 0.0;
+        for (int o = 0; o < 8; ++o) run[o] = 0;
         // straight-line per visit: a visit outside the grown tile (or past
         // the lane's range) keeps the current run and adds zero units, so
         // only the flush on a change of base cell is a branch
@@ -480,26 +479,30 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
           // base cell as offsets from the tile origin (wrapped into [0, G));
           // a corner is inside when its offset is below the tile size
           const int ax = (i0[0] - t.x0) & (G - 1), ay = (i0[1] - t.y0) & (G - 1), az = (i0[2] - t.z0) & (G - 1);
-          const bool in = (sl[u] >= 0) & ((ax < TD_X) | (ax == G - 1)) & ((ay < TD_Y) | (ay == G - 1)) &
+          const bool in = vok[u] & ((ax < TD_X) | (ax == G - 1)) & ((ay < TD_Y) | (ay == G - 1)) &
                           ((az < TD_Z) | (az == G - 1));
           const int key = in ? (az << 20) | (ay << 10) | ax : rkey;
-          if (key != rkey) {
-            if (u > 0) td_flush(acc, run, rkey, G, lane);  // nothing to add before the first visit
-#pragma unroll
-            for (int o = 0; o < 8; ++o) run[o] = 0.0;
-            rkey = key;
-          }
           // units rint(m * scale * w * 2^E), the power-of-two scaling applied to m * scale
           const float valE = in ? (mass ? ldexpf(__fmul_rn(pm[u], scale), E) : valE0) : 0.0f;
+          double q[8];
 #pragma unroll
           for (int o = 0; o < 8; ++o) {
             const float wx = (o & 1) ? wv[0] : __fsub_rn(1.0f, wv[0]);
             const float wy = ((o >> 1) & 1) ? wv[1] : __fsub_rn(1.0f, wv[1]);
             const float wz = (o >> 2) ? wv[2] : __fsub_rn(1.0f, wv[2]);
-            run[o] += (double)rintf(__fmul_rn(valE, __fmul_rn(__fmul_rn(wx, wy), wz)));
+            q[o] = (double)rintf(__fmul_rn(valE, __fmul_rn(__fmul_rn(wx, wy), wz)));
+          }
+          if (key != rkey) {
+            if (rkey >= 0) td_flush(acc, run, rkey, G, lane);
+#pragma unroll
+            for (int o = 0; o < 8; ++o) run[o] = q[o];
+            rkey = key;
+          } else {
+#pragma unroll
+            for (int o = 0; o < 8; ++o) run[o] += q[o];
           }
         }
-        td_flush(acc, run, rkey, G, lane);
+        if (rkey >= 0) td_flush(acc, run, rkey, G, lane);
       }
     }
     __syncthreads();
