@@ -993,6 +993,204 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
   if (tid < 32) pw[27 * 32 * CIN + tid] = ((bred[0][tid] + bred[1][tid]) + bred[2][tid]) + bred[3][tid];
 }
 
+// Weight gradient of the 32-channel layers, streamed along z.  A task is a
+// column of 32x2 voxels (x, y) over WZ_ZC planes of one scene; the workgroup
+// walks it plane by plane: x planes (with their one-voxel x/y halo) pass
+// through a three-plane LDS ring, so each is staged once per task instead of
+// once per tap row, and dz through a one-plane buffer.  A plane is 4 K-steps
+// of 16 voxels; each step's dz fragment feeds the wave's taps (as
+// k_conv3_wgrad: two x copies, the second one voxel to the left, so a dx = 0
+// or 1 fragment is one 16-byte read and a dx = 2 fragment four 4-byte reads).
+// 75 KB of LDS: two workgroups per CU.  Persistent workgroups take tasks
+// s, s + nwg, ... (s groups each XCD's workgroups on adjacent columns) and
+// write one partial each, summed in workgroup order by k_sum_partials.
+constexpr int WZ_X = 32, WZ_Y = 2;
+constexpr int WZ_HX = WZ_X + 2, WZ_HY = WZ_Y + 2;
+constexpr int WZ_XS = WZ_HX + 6;        // row stride of a staged x plane (40)
+constexpr int WZ_SLOT = WZ_HY * WZ_XS;  // one plane of one channel (160)
+constexpr int WZ_XCS = 520;             // channel stride of an x copy: 3 planes + pad (260 dwords)
+constexpr int WZ_DS = WZ_X + 8;         // row stride of the dz plane (40)
+constexpr int WZ_DCS = 136;             // channel stride of the dz plane: 2 rows + pad (68 dwords)
+constexpr int WZ_ZC = 64;               // planes per task (fewer when G is smaller)
+static_assert(WZ_XCS >= 3 * WZ_SLOT && (WZ_XCS / 2) % 64 == 4 && (WZ_DCS / 2) % 64 == 4 &&
+                  WZ_DCS >= WZ_Y * WZ_DS,
+              "16-byte fragment reads of 16 consecutive channels hit distinct banks");
+
+__global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* __restrict__ x,
+                                                                const uint16_t* __restrict__ dz, int B, int G,
+                                                                int zc, int64_t ntasks, float* __restrict__ part) {
+  constexpr int XCH = WZ_HY * (WZ_HX / 2) * 4;  // voxel pairs x 16-byte chunks of an x plane (272)
+  constexpr int XPER = (XCH + CONV_THREADS - 1) / CONV_THREADS;
+  constexpr int DCH = WZ_Y * WZ_X * 4;          // 16-byte chunks of a dz plane (256)
+  static_assert(DCH == CONV_THREADS, "one dz chunk per thread");
+  __shared__ __attribute__((aligned(16))) uint16_t xl[2][32 * WZ_XCS];  // [0]: voxel tx at tx; [1]: at tx - 1
+  __shared__ __attribute__((aligned(16))) uint16_t dl[32 * WZ_DCS];
+  __shared__ float bred[4][32];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i = lane & 31, kh = lane >> 5;
+  const int ntx = G / WZ_X, nty = G / WZ_Y, ntz = G / zc;
+  const int64_t G3 = (int64_t)G * G * G;
+  f32x16 acc[DW_TAPS_PER_WAVE];
+#pragma unroll
+  for (int k = 0; k < DW_TAPS_PER_WAVE; ++k)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[k][q] = 0.0f;
+  f32x16 accb;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) accb[q] = 0.0f;
+  bf16x8 ones;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
+
+  // one x plane (rows y0-1 .. y0+2, voxels x0-1 .. x0+32, in voxel pairs)
+  // and one dz plane travel global -> registers -> LDS (channel-major)
+  u32x4 rx[XPER][2], rd;
+  int cb = 0, cx0 = 0, cy0 = 0;  // the task's scene and column origin
+  auto load_x = [&](int zp) {
+    const int gz = zp & (G - 1);
+#pragma unroll
+    for (int k = 0; k < XPER; ++k) {
+      const int c = tid + k * CONV_THREADS;
+      if (c < XCH) {
+        const int ch = c & 3, tp = (c >> 2) % (WZ_HX / 2), ty = (c >> 2) / (WZ_HX / 2);
+        const int gy = (cy0 + ty - 1) & (G - 1);
+        const int64_t row = (int64_t)cb * G3 + ((int64_t)gz * G + gy) * G;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int gx = (cx0 + 2 * tp + u - 1) & (G - 1);
+          rx[k][u] = *reinterpret_cast<const u32x4*>(&x[(row + gx) * 32 + ch * 8]);
+        }
+      }
+    }
+  };
+  auto store_x = [&](int slot) {
+#pragma unroll
+    for (int k = 0; k < XPER; ++k) {
+      const int c = tid + k * CONV_THREADS;
+      if (c < XCH) {
+        const int ch = c & 3, tp = (c >> 2) % (WZ_HX / 2), ty = (c >> 2) / (WZ_HX / 2);
+        const int o = slot * WZ_SLOT + ty * WZ_XS + 2 * tp;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t a0 = rx[k][0][e], a1 = rx[k][1][e];  // channels 2e (low half), 2e + 1 (high half)
+          const int c0 = (ch * 8 + 2 * e) * WZ_XCS, c1 = c0 + WZ_XCS;
+          *reinterpret_cast<uint32_t*>(&xl[0][c0 + o]) = (a0 & 0xffffu) | (a1 << 16);
+          *reinterpret_cast<uint32_t*>(&xl[0][c1 + o]) = (a0 >> 16) | (a1 & 0xffff0000u);
+          xl[1][c0 + o] = (uint16_t)(a1 & 0xffffu);
+          xl[1][c1 + o] = (uint16_t)(a1 >> 16);
+          if (tp > 0) {
+            xl[1][c0 + o - 1] = (uint16_t)(a0 & 0xffffu);
+            xl[1][c1 + o - 1] = (uint16_t)(a0 >> 16);
+          }
+        }
+      }
+    }
+  };
+  const int dch = tid & 3, dtx = (tid >> 2) % WZ_X, dty = (tid >> 2) / WZ_X;
+  auto load_dz = [&](int zp) {
+    const int64_t gv = ((int64_t)zp * G + (cy0 + dty)) * G + cx0 + dtx;
+    rd = *reinterpret_cast<const u32x4*>(&dz[((int64_t)cb * G3 + gv) * 32 + dch * 8]);
+  };
+  auto store_dz = [&]() {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      dl[(dch * 8 + 2 * e) * WZ_DCS + dty * WZ_DS + dtx] = (uint16_t)(rd[e] & 0xffffu);
+      dl[(dch * 8 + 2 * e + 1) * WZ_DCS + dty * WZ_DS + dtx] = (uint16_t)(rd[e] >> 16);
+    }
+  };
+  // the MFMAs of one plane for wave WV: x planes z-1, z, z+1 in ring slots
+  // (s0, s0+1, s0+2) mod 3; 4 K-steps (2 rows x 2 halves of 16 voxels)
+  auto mfma_plane = [&](auto wvc, int s0) __attribute__((always_inline)) {
+    constexpr int WV = decltype(wvc)::value;
+#pragma unroll
+    for (int st = 0; st < 2 * WZ_Y; ++st) {
+      const int r = st >> 1, half = st & 1;
+      const int vb = half * 16 + kh * 8;
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&dl[i * WZ_DCS + r * WZ_DS + vb]);
+      if ((st & 3) == WV) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ones, accb, 0, 0, 0);
+#pragma unroll
+      for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+        const int tap = WV + 4 * k;
+        if (tap < 27) {
+          const int dzo = tap / 9, dyo = (tap / 3) % 3, dxo = tap % 3;
+          int sl = s0 + dzo;
+          sl = sl >= 3 ? sl - 3 : sl;
+          const int o = i * WZ_XCS + sl * WZ_SLOT + (r + dyo) * WZ_XS + vb;
+          bf16x8 bv;
+          if (dxo == 2) {  // voxels vb + 2 .. vb + 9 of the unshifted copy: 4-byte aligned
+            const uint32_t* p = reinterpret_cast<const uint32_t*>(&xl[0][o + 2]);
+            const u32x4 u = {p[0], p[1], p[2], p[3]};
+            bv = __builtin_bit_cast(bf16x8, u);
+          } else {
+            bv = *reinterpret_cast<const bf16x8*>(&xl[dxo][o]);
+          }
+          acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[k], 0, 0, 0);
+        }
+      }
+    }
+  };
+
+  const int64_t nwg = gridDim.x;
+  const int64_t s = (nwg % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nwg / 8) + blockIdx.x / 8 : (int64_t)blockIdx.x;
+  for (int64_t task = s; task < ntasks; task += nwg) {
+    int64_t t = task;
+    const int bx = (int)(t % ntx);
+    t /= ntx;
+    const int by = (int)(t % nty);
+    t /= nty;
+    const int bz = (int)(t % ntz);
+    cb = (int)(t / ntz);
+    cx0 = bx * WZ_X;
+    cy0 = by * WZ_Y;
+    const int z0 = bz * zc;
+    // prologue: x planes z0 - 1 and z0 into slots 0 and 1
+    load_x(z0 - 1);
+    __syncthreads();  // the previous task's MFMAs are done with LDS
+    store_x(0);
+    load_x(z0);
+    store_x(1);
+    load_x(z0 + 1);
+    load_dz(z0);
+    for (int zi = 0; zi < zc; ++zi) {
+      __syncthreads();  // the previous plane's MFMAs are done with slot (zi + 2) % 3 and the dz plane
+      store_x((zi + 2) % 3);
+      store_dz();
+      __syncthreads();
+      if (zi + 1 < zc) {
+        load_x(z0 + zi + 2);
+        load_dz(z0 + zi + 1);
+      }
+      const int s0 = zi % 3;
+      switch (wv) {  // wave-uniform: each wave's tap list is compile-time
+        case 0: mfma_plane(std::integral_constant<int, 0>{}, s0); break;
+        case 1: mfma_plane(std::integral_constant<int, 1>{}, s0); break;
+        case 2: mfma_plane(std::integral_constant<int, 2>{}, s0); break;
+        default: mfma_plane(std::integral_constant<int, 3>{}, s0); break;
+      }
+    }
+  }
+  if (i == 0)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) bred[wv][8 * (q / 4) + 4 * kh + (q % 4)] = accb[q];
+  __syncthreads();
+  float* pw = part + (int64_t)blockIdx.x * (27 * 32 * 32 + 32);
+#pragma unroll
+  for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
+    const int tap = wv + 4 * k;
+    if (tap >= 27) break;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int co = 8 * (q / 4) + 4 * kh + (q % 4);
+      pw[(tap * 32 + co) * 32 + i] = acc[k][q];
+    }
+  }
+  if (tid < 32) pw[27 * 32 * 32 + tid] = ((bred[0][tid] + bred[1][tid]) + bred[2][tid]) + bred[3][tid];
+}
+
+#ifndef AUGUR_WGRAD_Z
+#define AUGUR_WGRAD_Z 1
+#endif
+
 // out_w[j] (j < mw) and out_b[j - mw] = sum over partials in workgroup order
 __global__ void k_sum_partials(const float* __restrict__ part, int nparts, int64_t m, int64_t mw,
                                float* __restrict__ out_w, float* __restrict__ out_b) {
@@ -1347,7 +1545,12 @@ int augur_conv3_bf16_wgrad(const uint16_t* x, int cin, const uint16_t* dz, int64
     if (cin == 16)
       hipLaunchKernelGGL((k_conv3_wgrad<16>), dim3(nwg), dim3(CONV_THREADS), 0, s, x, dz, (int)B, (int)G, ntiles,
                          (float*)ws);
-    else
+    else if (AUGUR_WGRAD_Z) {
+      const int zc = (int)(G < WZ_ZC ? G : WZ_ZC);
+      const int64_t ntasks = B * (G / WZ_X) * (G / WZ_Y) * (G / zc);
+      hipLaunchKernelGGL(k_conv3_wgrad_z, dim3(nwg), dim3(CONV_THREADS), 0, s, x, dz, (int)B, (int)G, zc, ntasks,
+                         (float*)ws);
+    } else
       hipLaunchKernelGGL((k_conv3_wgrad<32>), dim3(nwg), dim3(CONV_THREADS), 0, s, x, dz, (int)B, (int)G, ntiles,
                          (float*)ws);
     AUGUR_LAUNCHED("conv_wgrad");
