@@ -8,7 +8,8 @@ of launches with no host synchronisation:
 1. the fixed-point CIC deposit of every scene in one launch
    (``augur_cic_deposit_ragged``);
 2. the batched FFT Poisson solve with Green's function -4 pi G / k^2 and the
-   spectral gradient (B grids per launch);
+   spectral gradient: the single-scene stepper's fused passes, B grids per
+   launch (``augur_pm_solve_grids``);
 3. optionally the scatter-field correction net on the B grids
    (``PMSurrogate`` in correction mode, BF16 MFMA convolutions at batch B);
 4. one persistent launch of the CIC gather of each scene's force planes with
@@ -68,9 +69,7 @@ class RaggedStepper:
         """(rho (B, G, G, G), PM force (B, 3, G, G, G)) of every scene."""
         G = self.G
         rho = ops.cic_deposit_ragged(pos, starts, G, self.h, 0.0, True, mass=mass)[:, 0]
-        spec = ops.rfftn(rho, s=(G, G, G))
-        acc = ops.poisson_green_grad(spec, G, self.L, self.Gconst, scale=1.0 / G ** 3)
-        return rho, ops.irfftn(acc, s=(G, G, G), scale=1.0)
+        return rho, ops.pm_solve_grids(rho, self.L, self.Gconst)
 
     def force_field(self, pos, starts, mass, scene_density=None):
         """(B, 3, G, G, G) force field of every scene: plain PM, plus the

@@ -201,8 +201,9 @@ int fft_r2c_xy(const float* in, float2* out, int64_t G, int64_t pitch, hipStream
 }
 
 // y-column forward transforms of a (G, G, pitch) half spectrum (G/2 + 1 columns)
-int fft_cols_y(float2* data, int64_t G, int64_t pitch, hipStream_t s) {
-  ColMap my{G, 1, G * pitch, 0, pitch, (int)(G / 2 + 1)};
+int fft_cols_y(float2* data, int64_t G, int64_t pitch, hipStream_t s, int64_t nb) {
+  // nb grids of G planes each, consecutive: every plane is one outer index
+  ColMap my{nb * G, 1, G * pitch, 0, pitch, (int)(G / 2 + 1)};
   return launch_cols<false>((int)G, data, my, s);
 }
 

@@ -20,7 +20,7 @@ int fft_r2c_xy(const float* in, float2* out, int64_t G, int64_t pitch, hipStream
 
 // compute units of the current device (cached per device; 256 if the query fails)
 int device_cu_count();
-int fft_cols_y(float2* data, int64_t G, int64_t pitch, hipStream_t s);
+int fft_cols_y(float2* data, int64_t G, int64_t pitch, hipStream_t s, int64_t nb = 1);
 
 // Per-launch event timing (see timing.hip).  A TimedScope around a launch
 // records a start/stop event pair on the stream when timing is enabled and

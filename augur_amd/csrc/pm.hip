@@ -581,6 +581,8 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ s
                                                          float kf, float green) {
   using C = PmCol<N>;
   constexpr int R0 = FftPlan<N>::R0, Q = N / R0;
+  spec += blockIdx.y * (int64_t)N * N * C::RP;  // grid blockIdx.y of a batch
+  outB += blockIdx.y * (int64_t)N * N * C::RP;
   __shared__ float2 X[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
@@ -640,6 +642,9 @@ template <int N>
 __global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0, float2* __restrict__ s1,
                                                         float2* __restrict__ s2, float kf) {
   using C = PmCol<N>;
+  s0 += blockIdx.y * (int64_t)N * N * C::RP;  // grid blockIdx.y of a batch
+  s1 += blockIdx.y * (int64_t)N * N * C::RP;
+  s2 += blockIdx.y * (int64_t)N * N * C::RP;
   __shared__ float2 X[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
@@ -693,6 +698,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pm_xc2r(const float2* __restric
   __shared__ float2 X[C::NL * C::LS];
   __shared__ float2 tw[N];
   constexpr int64_t PLANE = (int64_t)N * N * N;
+  s0 += blockIdx.y * (int64_t)N * N * C::RP;  // grid blockIdx.y of a batch
+  s1 += blockIdx.y * (int64_t)N * N * C::RP;
+  s2 += blockIdx.y * (int64_t)N * N * C::RP;
+  force += blockIdx.y * 3 * PLANE;
   make_twiddles<N>(tw);
   __syncthreads();
   const int64_t row0 = (int64_t)blockIdx.x * C::NL;
@@ -731,6 +740,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pm_rows_r2c(const float* __rest
                                                              float2* __restrict__ out) {
   using C = PmRow<N>;
   constexpr int H = C::NL / 2;
+  rho += blockIdx.y * (int64_t)N * N * N;  // grid blockIdx.y of a batch
+  out += blockIdx.y * (int64_t)N * N * C::RP;
   __shared__ float2 X[H * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
@@ -1397,33 +1408,36 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
 }
 
 // FFT Poisson + spectral gradient: b.rho -> force planes b.force
-static int pm_solve(int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
+// nb grids in a batch: rho, the spectra and force are nb consecutive blocks
+static int pm_solve(int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s, int nb = 1) {
   int rc;
   {
     TimedScope ts_("fft_rows_r2c", s);
-    PM_SWITCH_G(G, k_pm_rows_r2c, dim3((unsigned)(G * G / PmRow<GG>::NL)), dim3(FFT_THREADS), s,
+    PM_SWITCH_G(G, k_pm_rows_r2c, dim3((unsigned)(G * G / PmRow<GG>::NL), (unsigned)nb), dim3(FFT_THREADS), s,
                 (const float*)b.rho, b.s0);
     AUGUR_LAUNCHED("fft_rows_r2c");
   }
-  rc = fft_cols_y(b.s0, G, pm_pitch(G), s);  // y columns forward
+  rc = fft_cols_y(b.s0, G, pm_pitch(G), s, nb);  // y columns forward
   if (rc) return rc;
   const double PI = 3.14159265358979323846;
   const float kf = (float)(2.0 * PI / (double)L);
   const float green = (float)(-4.0 * PI * (double)Gconst / ((double)G * G * G));
   {
     TimedScope ts_("pm_zpass", s);
-    PM_SWITCH_G(G, k_pm_zpass, dim3((unsigned)(G * PmCol<GG>::nblk)), dim3(PmCol<GG>::T), s, b.s0, b.s2, kf, green);
+    PM_SWITCH_G(G, k_pm_zpass, dim3((unsigned)(G * PmCol<GG>::nblk), (unsigned)nb), dim3(PmCol<GG>::T), s, b.s0, b.s2,
+                kf, green);
     AUGUR_LAUNCHED("pm_zpass");
   }
   {
     TimedScope ts_("pm_yinv", s);
-    PM_SWITCH_G(G, k_pm_yinv, dim3((unsigned)(G * PmCol<GG>::nblk)), dim3(PmCol<GG>::T), s, b.s0, b.s1, b.s2, kf);
+    PM_SWITCH_G(G, k_pm_yinv, dim3((unsigned)(G * PmCol<GG>::nblk), (unsigned)nb), dim3(PmCol<GG>::T), s, b.s0, b.s1,
+                b.s2, kf);
     AUGUR_LAUNCHED("pm_yinv");
   }
   {
     TimedScope ts_("pm_xc2r", s);
-    PM_SWITCH_G(G, k_pm_xc2r, dim3((unsigned)(G * G / PmRow<GG>::NL)), dim3(FFT_THREADS), s, (const float2*)b.s0,
-                (const float2*)b.s1, (const float2*)b.s2, kf, b.force, (const uint32_t*)nullptr);
+    PM_SWITCH_G(G, k_pm_xc2r, dim3((unsigned)(G * G / PmRow<GG>::NL), (unsigned)nb), dim3(FFT_THREADS), s,
+                (const float2*)b.s0, (const float2*)b.s1, (const float2*)b.s2, kf, b.force, (const uint32_t*)nullptr);
     AUGUR_LAUNCHED("pm_xc2r");
   }
   return AUGUR_OK;
@@ -1768,6 +1782,29 @@ int augur_pm_solve_grid(const float* rho, float* force, int64_t G, float L, floa
   b.rho = const_cast<float*>(rho);   // read only by the solve
   b.force = force;
   return pm_solve(G, L, Gconst, b, as_stream(stream));
+}
+
+size_t augur_pm_solve_grids_workspace_bytes(int64_t B, int64_t G) {
+  if (B < 1 || G < 2) return 0;
+  return 3 * align_up((size_t)(B * G * G * pm_pitch(G)) * sizeof(float2)) + 256;
+}
+
+int augur_pm_solve_grids(const float* rho, float* force, int64_t B, int64_t G, float L, float Gconst, void* ws,
+                         size_t ws_bytes, void* stream) {
+  int rc = check_pm(1, G, L);
+  if (rc) return rc;
+  AUGUR_REQUIRE(B >= 1 && B <= 65535, AUGUR_EINVAL, "pm_solve_grids: B must be in [1, 65535]");
+  AUGUR_REQUIRE(rho && force, AUGUR_EINVAL, "pm_solve_grids: null pointer");
+  Carver c(ws, ws_bytes);
+  PmBufs b{};
+  const int64_t nc = B * G * G * pm_pitch(G);
+  b.s0 = c.take<float2>(nc);
+  b.s1 = c.take<float2>(nc);
+  b.s2 = c.take<float2>(nc);
+  AUGUR_REQUIRE(c.ok(), AUGUR_EWORKSPACE, "pm_solve_grids: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  b.rho = const_cast<float*>(rho);  // read only by the solve
+  b.force = force;
+  return pm_solve(G, L, Gconst, b, as_stream(stream), (int)B);
 }
 
 int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, void* ws, size_t ws_bytes,

@@ -220,6 +220,24 @@ def poisson_green_grad(rho_hat, G, L, Gconst=1.0, scale=1.0, want_phi=False):
     return (acc, phi) if want_phi else acc
 
 
+def pm_solve_grids(rho, L=1.0, Gconst=1.0, out=None):
+    """Periodic PM force planes of B density grids: rho (B, G, G, G) FP32 ->
+    (B, 3, G, G, G) FP32 [a_x, a_y, a_z], the single-scene stepper's fused
+    FFT passes with the batch as a grid dimension (each grid gets the bits it
+    gets alone)."""
+    rho = _c(rho, F32)
+    dev = _dev(rho)
+    B, G = rho.shape[0], rho.shape[-1]
+    assert tuple(rho.shape[1:]) == (G, G, G), rho.shape
+    if out is None:
+        out = torch.empty(B, 3, G, G, G, dtype=F32, device=dev)
+    assert out.dtype == F32 and out.is_contiguous() and tuple(out.shape) == (B, 3, G, G, G)
+    ws = _ws(_lib.size("augur_pm_solve_grids_workspace_bytes", B, G), dev)
+    _lib.call("augur_pm_solve_grids", _p(rho), _p(out), B, G, float(L), float(Gconst), _p(ws), ws.numel(),
+              _stream(dev))
+    return out
+
+
 def open_kernel_2d(G, h, eps, device):
     out = torch.empty(2 * G, 2 * G, dtype=F32, device=device)
     _lib.call("augur_open_kernel_2d", int(G), float(h), float(eps), _p(out), _stream(torch.device(device)))

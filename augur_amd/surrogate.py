@@ -650,13 +650,11 @@ def ragged_batch(scenes, device):
 
 def _batch_grids(sur, pos, starts, mass):
     """PM grids of a ragged batch: every scene's density in one deposit launch,
-    then batched FFT, Green's function and gradient: (rho (B, G, G, G),
-    force (B, 3, G, G, G))."""
+    then the batched fused FFT Poisson solve and spectral gradient: (rho (B,
+    G, G, G), force (B, 3, G, G, G))."""
     G = sur.G
     rho = ops.cic_deposit_ragged(pos, starts, G, sur.h, 0.0, True, mass=mass)[:, 0]
-    spec = ops.rfftn(rho, s=(G, G, G))
-    acc = ops.poisson_green_grad(spec, G, sur.L, sur.pm.Gconst, scale=1.0 / G ** 3)
-    return rho, ops.irfftn(acc, s=(G, G, G), scale=1.0)
+    return rho, ops.pm_solve_grids(rho, sur.L, sur.pm.Gconst)
 
 
 def batch_grads(sur, pos, starts, mass, tpos, a_true, inv_norm, scene_density):

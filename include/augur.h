@@ -332,6 +332,18 @@ int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, v
 int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B, int64_t total,
                           const float* force, int64_t G, float L, float dt, int phase, void* stream);
 
+/* The PM force planes of B density grids in one pass sequence: rho (B, G, G,
+ * G) FP32 -> force (B, 3, G, G, G) FP32 (a_x, a_y, a_z planes per grid), the
+ * fused FFT passes of augur_pm_step_begin with the batch as a second grid
+ * dimension, so every grid gets the bits augur_pm_solve_grid gives it alone.
+ * Periodic box [0, L)^3, G a power of two in [32, 512], 1 <= B <= 65535.
+ * New for the 3D periodic solve (as augur_poisson_green_grad, which it fuses
+ * with the transforms); replaces the kernel FFT convolution + neg_grad of
+ * scripts/pm_baseline_force_error.py:71-75 for a batch of scenes. */
+size_t augur_pm_solve_grids_workspace_bytes(int64_t B, int64_t G);
+int augur_pm_solve_grids(const float* rho, float* force, int64_t B, int64_t G, float L, float Gconst, void* ws,
+                         size_t ws_bytes, void* stream);
+
 /* ------------------------------------- large-N driver with compact state */
 
 /* Particle state in compact encodings, 3 values per particle: positions
