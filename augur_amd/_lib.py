@@ -94,6 +94,9 @@ SIGNATURES = {
                                        c_float, P, P]),
     "augur_lp_finish": (c_int, [P, c_int64, c_int64, c_float, c_float, P, P]),
     "augur_pm_solve_grid": (c_int, [P, P, c_int64, c_float, c_float, P, c_size_t, P]),
+    "augur_cic_deposit_ragged_tiled_workspace_bytes": (c_size_t, [c_int64, c_int64, c_int64]),
+    "augur_cic_deposit_ragged_tiled": (c_int, [P, P, c_float, P, c_int64, c_int64, c_int64, c_float, c_float, P, P,
+                                               c_size_t, P]),
     "augur_pm_solve_grids_workspace_bytes": (c_size_t, [c_int64, c_int64]),
     "augur_pm_solve_grids": (c_int, [P, P, c_int64, c_int64, c_float, c_float, P, c_size_t, P]),
     "augur_grid_mean_workspace_bytes": (c_size_t, []),

@@ -248,8 +248,11 @@ __device__ __forceinline__ int box_runs(const Piece& px, const Piece& py, const 
 // slot starts and an inclusive prefix of their lengths (TD_MAXSEG each; the
 // unused tail has length 0), and the tile's number of work items of at most
 // TD_CHUNK visits; items[ntiles] = 0 so an exclusive scan leaves the total there.
+// A batch of grids has scene-major tiles: tile = scene * tps + tile of the
+// scene (tps tiles per scene, keys scene * G^3 + tile_key), and a region
+// wraps within its own scene's grid.
 __device__ __forceinline__ void dep_plan_body(int bid, const uint32_t* __restrict__ cell_start, int G, int ntiles,
-                                              uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
+                                              int tps, uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
                                               uint32_t* __restrict__ seg_pre) {
   const int tile = bid * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (tile > ntiles) return;
@@ -258,7 +261,8 @@ __device__ __forceinline__ void dep_plan_body(int bid, const uint32_t* __restric
     return;
   }
   const int ntx = G / TD_X, nty = G / TD_Y, ntz = G / TD_Z;
-  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  const int scene = tile / tps, ts = tile - scene * tps;
+  const int tx = ts % ntx, ty = (ts / ntx) % nty, tz = ts / (ntx * nty);
   Piece PX[3], PY[3], PZ[3];
   const int nx = axis_pieces(ntx, TD_X, PX), ny = axis_pieces(nty, TD_Y, PY), nz = axis_pieces(ntz, TD_Z, PZ);
   int pre[28];  // runs before box b (boxes in z, y, x piece order)
@@ -280,7 +284,7 @@ __device__ __forceinline__ void dep_plan_body(int bid, const uint32_t* __restric
       const int bx = bi % nx, by = (bi / nx) % ny, bz = bi / (nx * ny), k = r - pre[bi];
       const Piece &px = PX[bx], &py = PY[by], &pz = PZ[bz];
       const int nt = (((tz + pz.off + ntz) % ntz) * nty + (ty + py.off + nty) % nty) * ntx + (tx + px.off + ntx) % ntx;
-      const uint32_t tb = (uint32_t)nt * TD_CELLS;
+      const uint32_t tb = (uint32_t)(scene * tps + nt) * TD_CELLS;
       uint32_t k0, len;
       const bool fx = px.b - px.a == TD_X, fy = py.b - py.a == TD_Y;
       if (fx && fy) {
@@ -379,7 +383,7 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     const float* __restrict__ pos, const uint32_t* __restrict__ perm, const float* __restrict__ mass,
     float mass_value, int64_t n, int G, float h, float inv_h, float scale, const PmDev* __restrict__ st,
     const uint32_t* __restrict__ item_start, const DepItem* __restrict__ info, const uint16_t* __restrict__ iblk,
-    const uint32_t* __restrict__ tseg_lo, const uint32_t* __restrict__ tseg_pre, int ntiles,
+    const uint32_t* __restrict__ tseg_lo, const uint32_t* __restrict__ tseg_pre, int ntiles, int tps,
     unsigned long long* __restrict__ fix, float* __restrict__ rho) {
   // unit counts of the fixed-point contributions: integers below 2^53 in FP64,
   // so the LDS float atomics and register sums are exact in any order
@@ -395,7 +399,9 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     const DepItem it = info[item];
     const int tile = (int)it.tile;
     const uint32_t nch = item_start[tile + 1] - item_start[tile];
-    const TileOrigin t = tile_origin(tile, G);
+    const int scene = tile / tps;  // scene-major tiles of a batch of grids
+    const TileOrigin t = tile_origin(tile - scene * tps, G);
+    float* const rho_s = rho + (int64_t)scene * G * G * G;
     const uint32_t* lo_t = tseg_lo + (int64_t)tile * TD_MAXSEG;
     const uint32_t* pre_t = tseg_pre + (int64_t)tile * (TD_MAXSEG + 1);
     const uint16_t* blk_t = iblk + (int64_t)item * TD_NBLK;
@@ -510,7 +516,7 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
       for (int i = tid; i < TD_CELLS; i += TD_THREADS) {
         const int lx = i % TD_X, ly = (i / TD_X) % TD_Y, lz = i / (TD_X * TD_Y);
         const int64_t cell = ((int64_t)(t.z0 + lz) * G + (t.y0 + ly)) * G + (t.x0 + lx);
-        rho[cell] = (float)ldexp(acc[i], -E);
+        rho_s[cell] = (float)ldexp(acc[i], -E);
       }
     } else {  // heavy tile: add into its image; k_dep_finish converts it
       unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
@@ -527,12 +533,14 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
 // again for the next launch (one workgroup per tile; others exit)
 __global__ __launch_bounds__(256) void k_dep_finish(const uint32_t* __restrict__ item_start,
                                                     unsigned long long* __restrict__ fix, const PmDev* __restrict__ st,
-                                                    int64_t n, int G, float* __restrict__ rho) {
+                                                    int64_t n, int G, int tps, float* __restrict__ rho) {
   const int tile = blockIdx.x;
   if (item_start[tile + 1] - item_start[tile] <= 1u) return;
   const int E = fixed_exponent(st->maxbits, n);
   const int ntx = G / TD_X, nty = G / TD_Y;
-  const int x0 = (tile % ntx) * TD_X, y0 = ((tile / ntx) % nty) * TD_Y, z0 = (tile / (ntx * nty)) * TD_Z;
+  const int scene = tile / tps, ts = tile - scene * tps;
+  rho += (int64_t)scene * G * G * G;
+  const int x0 = (ts % ntx) * TD_X, y0 = ((ts / ntx) % nty) * TD_Y, z0 = (ts / (ntx * nty)) * TD_Z;
   // plain loads: the atomics of k_dep_tiles are complete and visible at this
   // launch; each thread reads its 32 cells before writing any
   unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
@@ -956,7 +964,7 @@ __global__ __launch_bounds__(256) void k_plans(const uint32_t* __restrict__ cell
                                                uint32_t* __restrict__ gitems, const uint32_t* gate) {
   if (gate && *gate == 0u) return;
   if ((int)blockIdx.x < dep_blocks)
-    dep_plan_body((int)blockIdx.x, cell_start, G, ntiles, items, seg_lo, seg_pre);
+    dep_plan_body((int)blockIdx.x, cell_start, G, ntiles, ntiles, items, seg_lo, seg_pre);
   else
     gather_plan_body((int)blockIdx.x - dep_blocks, cell_start, npieces, gitems);
 }
@@ -1181,6 +1189,51 @@ __global__ __launch_bounds__(DIAG_THREADS) void k_diag_finish(const double* __re
   if (threadIdx.x < 4) out[threadIdx.x] = red[0][threadIdx.x];
 }
 
+// ---------------------------------------- ragged batch on the tiled deposit
+// Keys of a ragged batch (scene b holds particles [starts[b], starts[b+1])):
+// b G^3 + tile_key of the particle's base cell, so that the batch's sort
+// order is scene-major tile-major and its tiles are scene * tps + tile.
+__global__ void k_ragged_keys(const float* __restrict__ pos, const int64_t* __restrict__ starts, int B, int64_t n,
+                              int G, float h, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                              uint32_t* __restrict__ ngaps) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ngaps = 0u;  // for k_cell_starts, later in the same call
+  const uint32_t G3 = (uint32_t)G * G * G;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = B;  // the last scene with starts[scene] <= i
+    while (hi - lo > 1) {
+      const int m = (lo + hi) >> 1;
+      if (starts[m] <= i) lo = m; else hi = m;
+    }
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = wrap((int)floorf(cic_u(pos[3 * i + d], 0.0f, h)), G);
+    keys[i] = (uint32_t)lo * G3 + tile_key(c[0], c[1], c[2], G);
+    vals[i] = (uint32_t)i;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dep_plans(const uint32_t* __restrict__ cell_start, int G, int ntiles, int tps,
+                                                   uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
+                                                   uint32_t* __restrict__ seg_pre) {
+  dep_plan_body((int)blockIdx.x, cell_start, G, ntiles, tps, items, seg_lo, seg_pre);
+}
+
+__global__ __launch_bounds__(256) void k_dep_items(const uint32_t* __restrict__ item_start, int ntiles,
+                                                   const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
+                                                   uint16_t* __restrict__ blk) {
+  tile_items_body<TD_MAXSEG, TD_CHUNK>(blockIdx.x, item_start, ntiles, tseg_pre, info, blk);
+}
+
+// the int64 images of the heavy tiles (more than one item) zeroed before
+// their items add into them (a transient workspace is not kept zero)
+__global__ __launch_bounds__(256) void k_dep_zero_heavy(const uint32_t* __restrict__ item_start,
+                                                        unsigned long long* __restrict__ fix) {
+  const int tile = blockIdx.x;
+  if (item_start[tile + 1] - item_start[tile] <= 1u) return;
+  unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
+  for (int i = threadIdx.x; i < TD_CELLS; i += 256) img[i] = 0ull;
+}
+
 // Every kick and drift is one explicit FMA (v + (dt/2) a, x + dt v) in all the
 // stepping kernels, so a step gives the same bits whether its kicks are fused
 // into the gather or run here.
@@ -1392,16 +1445,16 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
                          mass_value, n, (int)G, h, inv_h, scale, (const PmDev*)b.st, (const uint32_t*)b.items,
                          (const DepItem*)b.info, (const uint16_t*)b.iblk, (const uint32_t*)b.tseg_lo,
                          (const uint32_t*)b.tseg_pre,
-                         (int)ntiles, b.fix, b.rho);
+                         (int)ntiles, (int)ntiles, b.fix, b.rho);
     else
       hipLaunchKernelGGL(k_dep_tiles<false>, dim3((unsigned)grid), dim3(TD_THREADS), 0, s, pos, perm, mass,
                          mass_value, n, (int)G, h, inv_h, scale, (const PmDev*)b.st, (const uint32_t*)b.items,
                          (const DepItem*)b.info, (const uint16_t*)b.iblk, (const uint32_t*)b.tseg_lo,
                          (const uint32_t*)b.tseg_pre,
-                         (int)ntiles, b.fix, b.rho);
+                         (int)ntiles, (int)ntiles, b.fix, b.rho);
     AUGUR_LAUNCHED("dep_tiles");
     hipLaunchKernelGGL(k_dep_finish, dim3((unsigned)ntiles), dim3(256), 0, s, (const uint32_t*)b.items, b.fix,
-                       (const PmDev*)b.st, n, (int)G, b.rho);
+                       (const PmDev*)b.st, n, (int)G, (int)ntiles, b.rho);
     AUGUR_LAUNCHED("dep_finish");
   }
   return AUGUR_OK;
@@ -1805,6 +1858,124 @@ int augur_pm_solve_grids(const float* rho, float* force, int64_t B, int64_t G, f
   b.rho = const_cast<float*>(rho);  // read only by the solve
   b.force = force;
   return pm_solve(G, L, Gconst, b, as_stream(stream), (int)B);
+}
+
+// The workspace of the ragged tiled deposit, carved in this order.
+struct RaggedDepBufs {
+  PmDev* st;
+  uint32_t *keys, *vals, *cell_start, *gaps, *ngaps, *items, *tseg_lo, *tseg_pre;
+  char *rws, *scan_ws;
+  DepItem* info;
+  uint16_t* iblk;
+  unsigned long long* fix;
+};
+static bool carve_ragged_dep(Carver& c, int64_t B, int64_t n, int64_t G, RaggedDepBufs& b) {
+  const int64_t ncells = B * G * G * G, ntiles = B * pm_tiles(G);
+  const int64_t max_items = ntiles + (8 * n + TD_CHUNK - 1) / TD_CHUNK;
+  b.st = c.take<PmDev>(1);
+  b.keys = c.take<uint32_t>(n);
+  b.vals = c.take<uint32_t>(n);
+  b.rws = c.take<char>(radix_workspace_bytes(n));
+  b.cell_start = c.take<uint32_t>(ncells + 1);
+  b.gaps = c.take<uint32_t>(3 * ((ncells + 1) / CS_SPAN + 2));
+  b.ngaps = c.take<uint32_t>(1);
+  b.items = c.take<uint32_t>(ntiles + 1);
+  b.scan_ws = c.take<char>(scan_workspace_bytes(ntiles + 1));
+  b.tseg_lo = c.take<uint32_t>(ntiles * TD_MAXSEG);
+  b.tseg_pre = c.take<uint32_t>(ntiles * (TD_MAXSEG + 1));
+  b.info = c.take<DepItem>(max_items);
+  b.iblk = c.take<uint16_t>(max_items * TD_NBLK);
+  b.fix = c.take<unsigned long long>(ncells);
+  return c.ok();
+}
+
+size_t augur_cic_deposit_ragged_tiled_workspace_bytes(int64_t B, int64_t G, int64_t total) {
+  if (B < 1 || G < 32 || total < 0) return 0;
+  Carver c(nullptr, 0);
+  RaggedDepBufs b;
+  carve_ragged_dep(c, B, total > 0 ? total : 1, G, b);
+  return c.used + 4096;
+}
+
+int augur_cic_deposit_ragged_tiled(const float* pos, const float* mass, float mass_value, const int64_t* starts,
+                                   int64_t B, int64_t total, int64_t G, float h, float scale, float* out, void* ws,
+                                   size_t ws_bytes, void* stream) {
+  AUGUR_REQUIRE(is_pow2(G) && G >= 32 && G <= 512, AUGUR_EUNSUPPORTED,
+                "cic_deposit_ragged_tiled: G must be a power of two in [32, 512]");
+  AUGUR_REQUIRE(B >= 1 && B <= 65535 && total >= 0 && total < ((int64_t)1 << 31) && h > 0.0f && starts && out,
+                AUGUR_EINVAL, "cic_deposit_ragged_tiled: bad arguments");
+  AUGUR_REQUIRE(B * G * G * G < ((int64_t)1 << 32), AUGUR_EUNSUPPORTED,
+                "cic_deposit_ragged_tiled: B G^3 must be below 2^32");
+  hipStream_t s = as_stream(stream);
+  const int64_t G3 = G * G * G;
+  if (total == 0) return fill_u32(out, B * G3, 0u, s);
+  AUGUR_REQUIRE(pos, AUGUR_EINVAL, "cic_deposit_ragged_tiled: null positions");
+  Carver c(ws, ws_bytes);
+  RaggedDepBufs b;
+  AUGUR_REQUIRE(carve_ragged_dep(c, B, total, G, b), AUGUR_EWORKSPACE,
+                "cic_deposit_ragged_tiled: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  const int64_t n = total, tps = pm_tiles(G), ntiles = B * tps;
+  const dim3 g(blocks_for(n, 256, 1024)), bl(256);
+  {
+    TimedScope ts_("ragged_sort", s);
+    hipLaunchKernelGGL(k_ragged_keys, g, bl, 0, s, pos, starts, (int)B, n, (int)G, h, b.keys, b.vals, b.ngaps);
+    AUGUR_LAUNCHED("ragged_keys");
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < B * G3) ++bits;
+    uint32_t *ks, *perm;
+    int rc = radix_sort_pairs(b.keys, b.vals, n, bits, b.rws, radix_workspace_bytes(n), &ks, &perm, s, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cell_starts, dim3(blocks_for(n + 1, 256, 4096)), bl, 0, s, (const uint32_t*)ks, n,
+                       (uint32_t)(B * G3), b.cell_start, b.gaps, b.ngaps, (const uint32_t*)nullptr);
+    AUGUR_LAUNCHED("cell_starts");
+    hipLaunchKernelGGL(k_cell_fill, dim3(1024), bl, 0, s, (const uint32_t*)b.gaps, (const uint32_t*)b.ngaps,
+                       b.cell_start, (const uint32_t*)nullptr);
+    AUGUR_LAUNCHED("cell_fill");
+    hipLaunchKernelGGL(k_dep_plans, dim3((unsigned)((ntiles + 1 + 3) / 4)), dim3(256), 0, s,
+                       (const uint32_t*)b.cell_start, (int)G, (int)ntiles, (int)tps, b.items, b.tseg_lo, b.tseg_pre);
+    AUGUR_LAUNCHED("dep_plans");
+    rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, nullptr);
+    if (rc) return rc;
+    const int64_t max_items = ntiles + (8 * n + TD_CHUNK - 1) / TD_CHUNK;
+    hipLaunchKernelGGL(k_dep_items, dim3((unsigned)max_items), dim3(256), 0, s, (const uint32_t*)b.items, (int)ntiles,
+                       (const uint32_t*)b.tseg_pre, b.info, b.iblk);
+    AUGUR_LAUNCHED("dep_items");
+    hipLaunchKernelGGL(k_dep_zero_heavy, dim3((unsigned)ntiles), dim3(256), 0, s, (const uint32_t*)b.items, b.fix);
+    AUGUR_LAUNCHED("dep_zero_heavy");
+    b.vals = perm;  // sorted slot -> particle
+  }
+  {
+    TimedScope ts_("dep_bound", s);
+    if (mass) {
+      hipLaunchKernelGGL(k_pm_set_maxbits, dim3(1), dim3(64), 0, s, b.st, (const float*)nullptr, 0.0f, scale, n);
+      AUGUR_LAUNCHED("dep_bound_reset");
+      hipLaunchKernelGGL(k_pm_set_maxbits, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, s, b.st, mass, mass_value,
+                         scale, n);
+    } else
+      hipLaunchKernelGGL(k_pm_set_maxbits, dim3(1), dim3(64), 0, s, b.st, (const float*)nullptr, mass_value, scale, n);
+    AUGUR_LAUNCHED("dep_bound");
+  }
+  {
+    TimedScope ts_("dep_tiles", s);
+    const int64_t grid = ntiles + (2 * n + TD_CHUNK - 1) / TD_CHUNK;
+    const bool p2 = pow2_float(h);
+    const float inv_h = p2 ? 1.0f / h : 0.0f;
+    if (p2)
+      hipLaunchKernelGGL(k_dep_tiles<true>, dim3((unsigned)grid), dim3(TD_THREADS), 0, s, pos,
+                         (const uint32_t*)b.vals, mass, mass_value, n, (int)G, h, inv_h, scale, (const PmDev*)b.st,
+                         (const uint32_t*)b.items, (const DepItem*)b.info, (const uint16_t*)b.iblk,
+                         (const uint32_t*)b.tseg_lo, (const uint32_t*)b.tseg_pre, (int)ntiles, (int)tps, b.fix, out);
+    else
+      hipLaunchKernelGGL(k_dep_tiles<false>, dim3((unsigned)grid), dim3(TD_THREADS), 0, s, pos,
+                         (const uint32_t*)b.vals, mass, mass_value, n, (int)G, h, inv_h, scale, (const PmDev*)b.st,
+                         (const uint32_t*)b.items, (const DepItem*)b.info, (const uint16_t*)b.iblk,
+                         (const uint32_t*)b.tseg_lo, (const uint32_t*)b.tseg_pre, (int)ntiles, (int)tps, b.fix, out);
+    AUGUR_LAUNCHED("dep_tiles");
+    hipLaunchKernelGGL(k_dep_finish, dim3((unsigned)ntiles), dim3(256), 0, s, (const uint32_t*)b.items, b.fix,
+                       (const PmDev*)b.st, n, (int)G, (int)tps, out);
+    AUGUR_LAUNCHED("dep_finish");
+  }
+  return AUGUR_OK;
 }
 
 int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, void* ws, size_t ws_bytes,

@@ -132,10 +132,14 @@ def tsc_gather(field, pos, h, off, periodic):
     return out
 
 
-def cic_deposit_ragged(pos, starts, G, h, off, periodic, mass=None, vel=None, scale=None, mass_value=1.0):
+def cic_deposit_ragged(pos, starts, G, h, off, periodic, mass=None, vel=None, scale=None, mass_value=1.0,
+                       tiled=None):
     """Ragged batch: pos (total, D) with scene b = rows [starts[b], starts[b+1]),
     starts an int64 tensor of B+1 offsets on the device; mass (total,) or
-    None (mass_value each) -> (B, C, G, .., G) float32, one launch for all scenes."""
+    None (mass_value each) -> (B, C, G, .., G) float32, every scene at once.
+    A 3D periodic mass deposit (off 0, no velocity channels, G a power of two
+    in [32, 512]) runs on the tiled LDS deposit (augur_cic_deposit_ragged_tiled,
+    the same bits); tiled=False forces the global-atomic deposit."""
     pos = _c(pos, F32)
     dev = _dev(pos)
     total, D = pos.shape
@@ -146,6 +150,17 @@ def cic_deposit_ragged(pos, starts, G, h, off, periodic, mass=None, vel=None, sc
     C = 1 if vel is None else 1 + D
     if scale is None:
         scale = 1.0 / h ** D
+    ok = (D == 3 and bool(periodic) and vel is None and float(off) == 0.0 and G & (G - 1) == 0 and 32 <= G <= 512
+          and B * G ** 3 < 2 ** 32 and B <= 65535 and total < 2 ** 31)
+    if tiled is None:
+        tiled = ok
+    if tiled:
+        assert ok, "the tiled ragged deposit needs D = 3, periodic, off 0, no vel, G a power of two in [32, 512]"
+        out = torch.empty((B, 1, G, G, G), dtype=F32, device=dev)
+        ws = _ws(_lib.size("augur_cic_deposit_ragged_tiled_workspace_bytes", B, G, total), dev)
+        _lib.call("augur_cic_deposit_ragged_tiled", _p(pos), _p(mass), float(mass_value), _p(starts), B, total, int(G),
+                  float(h), float(scale), _p(out), _p(ws), ws.numel(), _stream(dev))
+        return out
     out = torch.empty((B, C) + (G,) * D, dtype=F32, device=dev)
     wsb = _lib.size("augur_cic_deposit_workspace_bytes", B, D, G, C)
     ws = _ws(wsb, dev)

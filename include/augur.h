@@ -340,6 +340,22 @@ int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* sta
  * New for the 3D periodic solve (as augur_poisson_green_grad, which it fuses
  * with the transforms); replaces the kernel FFT convolution + neg_grad of
  * scripts/pm_baseline_force_error.py:71-75 for a batch of scenes. */
+/* The periodic 3D fixed-point CIC deposit of a ragged batch (as
+ * augur_cic_deposit_ragged with D = 3, periodic, off = 0, no velocity
+ * channels: the same units, so the same bits) on the PM stepper's tiled LDS
+ * deposit instead of global atomics: the batch is sorted into scene-major
+ * tile-major order (keys scene G^3 + tile-major cell; the sort permutes an
+ * index array, not the particles), each scene's 32x16x16-cell tiles sum their
+ * grown regions in LDS and write their cells, dense tiles merge through int64
+ * images.  out: (B, G, G, G) FP32.  G a power of two in [32, 512], B G^3 <
+ * 2^32, total < 2^31; h the cell size, scale the deposit scale (1/h^3 for a
+ * density).  Replaces the index_add_ deposit of scripts/scatter_field.py:
+ * 237-250 for a batch of 3D periodic scenes. */
+size_t augur_cic_deposit_ragged_tiled_workspace_bytes(int64_t B, int64_t G, int64_t total);
+int augur_cic_deposit_ragged_tiled(const float* pos, const float* mass, float mass_value, const int64_t* starts,
+                                   int64_t B, int64_t total, int64_t G, float h, float scale, float* out, void* ws,
+                                   size_t ws_bytes, void* stream);
+
 size_t augur_pm_solve_grids_workspace_bytes(int64_t B, int64_t G);
 int augur_pm_solve_grids(const float* rho, float* force, int64_t B, int64_t G, float L, float Gconst, void* ws,
                          size_t ws_bytes, void* stream);

@@ -294,3 +294,30 @@ def test_batched_solve_is_each_grids_solve(dev):
     gen = ops.irfftn(ops.poisson_green_grad(spec, G, L, 1.0, scale=1.0 / G ** 3), s=(G, G, G), scale=1.0)
     rel = float(((f - gen).double().pow(2).sum() / gen.double().pow(2).sum()).sqrt())
     assert rel < 1e-5, rel
+
+
+@pytest.mark.parametrize("G", [32, 64])
+def test_tiled_ragged_deposit_is_the_atomic_deposit_bitwise(dev, G):
+    """The ragged deposit on the tiled LDS kernels (scene-major tiles) gives
+    the global-atomic fixed-point deposit's bits: mixed scenes with per-particle
+    masses, an empty and a one-particle scene, particles on the box faces, and
+    a dense clump whose tiles take several work items (the int64 image path)."""
+    from augur_amd import ops
+    rng = np.random.default_rng(31 + G)
+    sc = scenes_np(7, [3000, 9000])
+    clump = (0.5 + 0.004 * rng.standard_normal((40000, 3))).astype(np.float32) % np.float32(1.0)
+    edge = rng.uniform(0, 1, (500, 3)).astype(np.float32)
+    edge[:100, 0] = 0.0
+    edge[100:200, 1] = np.float32(1.0) - np.float32(2.0 ** -24)
+    sc = [sc[0], np.zeros((0, 3), np.float32), clump, edge, rng.uniform(0, 1, (1, 3)).astype(np.float32), sc[1]]
+    sizes = [len(p) for p in sc]
+    pos = torch.from_numpy(np.concatenate(sc).astype(np.float32)).to(dev)
+    starts = torch.tensor([0] + list(np.cumsum(sizes)), dtype=torch.int64, device=dev)
+    mass = torch.from_numpy(rng.uniform(0.5, 2.0, pos.shape[0]).astype(np.float32) / 1000.0).to(dev)
+    h = 1.0 / G
+    for m in (mass, None):
+        a = ops.cic_deposit_ragged(pos, starts, G, h, 0.0, True, mass=m, mass_value=0.37, tiled=True)
+        b = ops.cic_deposit_ragged(pos, starts, G, h, 0.0, True, mass=m, mass_value=0.37, tiled=False)
+        assert a.shape == b.shape == (len(sc), 1, G, G, G)
+        assert torch.equal(a, b)
+        assert float(a[1].abs().max()) == 0.0
