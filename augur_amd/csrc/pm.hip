@@ -1435,9 +1435,10 @@ static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass,
   const int64_t ntiles = pm_tiles(G);
   {
     TimedScope ts_("dep_tiles", s);
-    // one workgroup per tile plus enough for the items of heavy tiles in a
-    // typical scene; the kernel loops over any items beyond the grid
-    const int64_t grid = ntiles + (2 * n + TD_CHUNK - 1) / TD_CHUNK;
+    // a workgroup for every possible item (a cell lies in at most 8 tile
+    // regions), so no workgroup works through two items of a dense clump in
+    // turn; workgroups past the plan's item count exit at once
+    const int64_t grid = pm_max_items(n, G);
     const bool p2 = pow2_float(h);
     const float inv_h = p2 ? 1.0f / h : 0.0f;
     if (p2)
@@ -1957,7 +1958,7 @@ int augur_cic_deposit_ragged_tiled(const float* pos, const float* mass, float ma
   }
   {
     TimedScope ts_("dep_tiles", s);
-    const int64_t grid = ntiles + (2 * n + TD_CHUNK - 1) / TD_CHUNK;
+    const int64_t grid = ntiles + (8 * n + TD_CHUNK - 1) / TD_CHUNK;  // a workgroup for every possible item
     const bool p2 = pow2_float(h);
     const float inv_h = p2 ? 1.0f / h : 0.0f;
     if (p2)

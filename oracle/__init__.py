@@ -14,8 +14,14 @@ Modules
               solve with spectral gradient, KDK leapfrog, force-error metrics.
 ``direct``    FP64 softened direct sum (isolated, the reference's exact_acc),
               FP64 Ewald sum (periodic), energies and momenta.
-``torch_cpu`` PyTorch-CPU FP32 port of the plain PM step (the CPU baseline)
-              and the FP64 conv3d reference used for the surrogate network.
+``torch_cpu`` PyTorch-CPU FP32 port of the plain PM step (the CPU baseline,
+              3D periodic and the 2D reference-shape pm_acc) and the FP64
+              conv3d / LocalNet3D reference used for the surrogate network.
+``scatter2d`` FP64 restatement of the reference's 2D ScatterField step (UNet
+              and LocalNet grid nets, learned-kernel FFT with zero and ext
+              padding, FD gradient, CIC and TSC stencils, cell-list kNN pair
+              term, momfix, Verlet and plain updates) and the reader of its
+              shipped weights manifest.
 
 Each function cites the reference file:line it restates.  How each one is
 pinned (golden fixture, closed form, brute force) is listed in the test files

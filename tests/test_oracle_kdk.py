@@ -169,3 +169,42 @@ def test_torch_cpu_port_2d_matches_oracle():
     a_port = tc.pm_acc_2d(torch.from_numpy(pos[0]), torch.ones(n), G, ext).double().numpy()
     a_ref = opm.pm_acc_2d_open(pos, np.ones((1, n)), np.ones((1, n)), G, ext)[0]
     assert opm.rel_rms(a_port, a_ref) < 1e-5
+
+
+def test_conv3d_periodic_is_the_brute_force_correlation():
+    # the FP64 reference of every GPU conv test: out[co, z, y, x] = b[co] + sum over
+    # ci and taps (dz, dy, dx) of w[co, ci, dz, dy, dx] x[ci, z+dz-1, y+dy-1, x+dx-1]
+    # (indices mod G); a flipped kernel or a wrong tap axis fails it
+    g = torch.Generator().manual_seed(4)
+    G, ci, co = 4, 2, 3
+    x = torch.randn(1, ci, G, G, G, generator=g, dtype=torch.float64)
+    w = torch.randn(co, ci, 3, 3, 3, generator=g, dtype=torch.float64)
+    b = torch.randn(co, generator=g, dtype=torch.float64)
+    y = tc.conv3d_periodic(x, w, b)
+    ref = torch.zeros_like(y)
+    for z in range(G):
+        for yy in range(G):
+            for xx in range(G):
+                for dz in range(3):
+                    for dy in range(3):
+                        for dx in range(3):
+                            v = x[0, :, (z + dz - 1) % G, (yy + dy - 1) % G, (xx + dx - 1) % G]
+                            ref[0, :, z, yy, xx] += w[:, :, dz, dy, dx] @ v
+    ref += b.view(1, co, 1, 1, 1)
+    torch.testing.assert_close(y, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_localnet3d_composition():
+    # identity input and output convs, zero middle convs with bias 0.3: every
+    # residual block adds gelu(0.3), so y = gelu(x) + (layers - 2) gelu(0.3)
+    # exactly (tanh GELU); a block without its residual, or one too many, fails
+    C, G, layers = 3, 4, 5
+    eye = torch.zeros(C, C, 3, 3, 3, dtype=torch.float64)
+    eye[range(C), range(C), 1, 1, 1] = 1.0
+    ws = [eye] + [torch.zeros(C, C, 3, 3, 3, dtype=torch.float64) for _ in range(layers - 2)] + [eye]
+    bs = [torch.zeros(C, dtype=torch.float64)] + [torch.full((C,), 0.3, dtype=torch.float64)
+                                                  for _ in range(layers - 2)] + [torch.zeros(C, dtype=torch.float64)]
+    x = torch.randn(1, C, G, G, G, generator=torch.Generator().manual_seed(6), dtype=torch.float64)
+    y = tc.localnet3d(x, ws, bs)
+    gel = lambda t: torch.nn.functional.gelu(t, approximate="tanh")
+    torch.testing.assert_close(y, gel(x) + (layers - 2) * gel(torch.tensor(0.3, dtype=torch.float64)))
