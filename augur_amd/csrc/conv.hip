@@ -481,6 +481,18 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   }
   __syncthreads();
 
+  // the into-field epilogue reads and writes this scene's planes through a
+  // raw buffer view: a lane whose channel is past cout gets an offset past
+  // the view (its load gives 0, its store is dropped), so the loads and
+  // stores carry no branches and no wait on earlier stores
+  constexpr bool CF = MODE == CONV_OUT_F32_CF_INTO || MODE == CONV_OUT_F32_CF;
+  const __amdgpu_buffer_rsrc_t rfield = __builtin_amdgcn_make_buffer_rsrc(
+      CF ? (void*)(y_out + (int64_t)b * cout * G3) : (void*)y_out, 0, CF ? (int)((int64_t)cout * G3 * 4) : 0,
+      0x00020000);
+  auto field_off = [&](int co, int64_t sv) -> uint32_t {
+    return co < cout ? (uint32_t)(((int64_t)co * G3 + sv) * 4) : 0xFFFFFFF0u;
+  };
+
 #pragma unroll 1
   for (int zi = 0; zi < zc; ++zi) {
     const int z = zs + zi;
@@ -488,15 +500,13 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
     if (zi + 1 < zc) load_plane(z + 2, pre);
     // the field values this thread updates, fetched before the MFMAs (in place, accumulating)
     float old[ZM_RPW][4];
-    if (MODE == CONV_OUT_F32_CF_INTO && accumulate) {
+    if (MODE == CONV_OUT_F32_CF_INTO) {
 #pragma unroll
       for (int r = 0; r < ZM_RPW; ++r) {
         const int64_t sv = ((int64_t)z * G + y0 + wrow * ZM_RPW + r) * G + x0 + hf * 16 + n16;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int co = 4 * kq + q;
-          old[r][q] = co < cout ? y_out[((int64_t)b * cout + co) * G3 + sv] : 0.0f;
-        }
+        for (int q = 0; q < 4; ++q)
+          old[r][q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rfield, field_off(4 * kq + q, sv), 0, 0));
       }
     }
     f32x4 acc[ZM_RPW];
@@ -525,16 +535,23 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
     for (int r = 0; r < ZM_RPW; ++r) {
       const int ly = wrow * ZM_RPW + r;
       const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + hf * 16 + n16;
+      if (CF) {  // channel-first planes of this scene, through the buffer view
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int co = 4 * kq + q;
+          const float val = acc[r][q] + sbias[co < 32 ? co : 0];
+          const float out = MODE == CONV_OUT_F32_CF ? val
+                            : (accumulate ? fmaf(alpha, val, old[r][q]) : alpha * val);  // uniform select
+          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(out), rfield, field_off(co, sv), 0, 0);
+        }
+        continue;
+      }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int co = 4 * kq + q;
         if (co < cout) {
           const float val = acc[r][q] + sbias[co];
-          if (MODE == CONV_OUT_F32_CF) {
-            y_out[((int64_t)b * cout + co) * G3 + sv] = val;
-          } else if (MODE == CONV_OUT_F32_CF_INTO) {
-            y_out[((int64_t)b * cout + co) * G3 + sv] = accumulate ? fmaf(alpha, val, old[r][q]) : alpha * val;
-          } else {
+          {
             const int64_t o = ((int64_t)b * G3 + sv) * cout + co;
             y_out[o] = addend ? val + addend[o] : val;
           }
@@ -1461,6 +1478,10 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
   if (variant == TILE_DEFAULT) variant = conv_default_variant(mask);
   AUGUR_REQUIRE(variant >= TILE_ZM && variant <= TILE_IN4 && ((mask >> variant) & 1), AUGUR_EUNSUPPORTED,
                 "conv: tile variant cannot run this layer");
+  // the narrow channel-first epilogue addresses one scene's planes through a
+  // 32-bit buffer view
+  if (variant == TILE_NARROW && mode == CONV_OUT_F32_CF && (int64_t)cout * G * G * G * 4 >= ((int64_t)1 << 31))
+    variant = TILE_ZM;
   hipStream_t s = as_stream(stream);
   if (cin == 4) {
     const int64_t blocks = B * (G / ZM_X) * (G / ZM_Y) * (G / zc);
@@ -1511,6 +1532,8 @@ int augur_conv3_bf16_fwd_into(const uint16_t* x, int cin, const uint16_t* w, con
   AUGUR_REQUIRE(conv_tiles_ok(G), AUGUR_EUNSUPPORTED, "conv: G must be a power of two in [32, 1024]");
   AUGUR_REQUIRE(B >= 1 && x && w && bias && field, AUGUR_EINVAL, "conv_into: bad arguments");
   AUGUR_REQUIRE(cin == 32 && cout >= 1 && cout <= 16, AUGUR_EUNSUPPORTED, "conv_into: needs cin 32, cout 1..16");
+  AUGUR_REQUIRE((int64_t)cout * G * G * G * 4 < ((int64_t)1 << 31), AUGUR_EUNSUPPORTED,
+                "conv_into: one scene's field (cout G^3 FP32) must stay below 2 GiB");
   if (zc == 0) zc = (int)(G < ZM_ZC ? G : ZM_ZC);
   AUGUR_REQUIRE(is_pow2(zc) && zc >= 2 && zc <= G, AUGUR_EINVAL, "conv: zc must be a power of two in [2, G]");
   return launch_fwd<32, CONV_OUT_F32_CF_INTO>(x, w, bias, B, G, cout, nullptr, nullptr, field, nullptr, TILE_NARROW,

@@ -451,8 +451,9 @@ int augur_conv3_bf16_fwd_tile(const uint16_t* x, int cin, const uint16_t* w, con
 /* The output layer written straight into a force field: field FP32
  * channels-first [B][cout][G^3] becomes alpha (conv + bias) + field
  * (accumulate != 0, fmaf per voxel) or alpha (conv + bias) (accumulate = 0),
- * cin 32, cout <= 16, on the NARROW family with zc planes per workgroup (0:
- * the default).  While skip (may be NULL) points at a non-zero word the launch
+ * cin 32, cout <= 16, cout G^3 * 4 bytes (one scene's field) below 2 GiB, on
+ * the NARROW family with zc planes per workgroup (0: the default).  While
+ * skip (may be NULL) points at a non-zero word the launch
  * does nothing.  Replaces the separate `force += scale * net(x)` of a
  * correction net (the grid half of ScatterField._grid_force,
  * scripts/scatter_field.py:657-662, where the net's field is added to the
