@@ -97,6 +97,7 @@ SIGNATURES = {
     "augur_cic_deposit_ragged_tiled_workspace_bytes": (c_size_t, [c_int64, c_int64, c_int64]),
     "augur_cic_deposit_ragged_tiled": (c_int, [P, P, c_float, P, c_int64, c_int64, c_int64, c_float, c_float, P, P,
                                                c_size_t, P]),
+    "augur_field_add": (c_int, [P, P, c_float, c_int64, P, P]),
     "augur_pm_solve_grids_workspace_bytes": (c_size_t, [c_int64, c_int64]),
     "augur_pm_solve_grids": (c_int, [P, P, c_int64, c_int64, c_float, c_float, P, c_size_t, P]),
     "augur_grid_mean_workspace_bytes": (c_size_t, []),

@@ -1631,6 +1631,37 @@ int augur_pack_features_bf16(const float* rho, const float* force, int64_t B, in
   return AUGUR_OK;
 }
 
+// y += alpha x over n floats (float4 where aligned); nothing while *skip != 0
+__global__ __launch_bounds__(256) void k_field_add(float* __restrict__ y, const float* __restrict__ x, float alpha,
+                                                   int64_t n, const uint32_t* __restrict__ skip) {
+  if (skip && *skip != 0u) return;
+  const int64_t n4 = n / 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += stride) {
+    float4 a = reinterpret_cast<float4*>(y)[i];
+    const float4 b = reinterpret_cast<const float4*>(x)[i];
+    a.x = fmaf(alpha, b.x, a.x);
+    a.y = fmaf(alpha, b.y, a.y);
+    a.z = fmaf(alpha, b.z, a.z);
+    a.w = fmaf(alpha, b.w, a.w);
+    reinterpret_cast<float4*>(y)[i] = a;
+  }
+  for (int64_t i = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    y[i] = fmaf(alpha, x[i], y[i]);
+}
+
+int augur_field_add(float* y, const float* x, float alpha, int64_t n, const uint32_t* skip, void* stream) {
+  AUGUR_REQUIRE(y && x && n >= 0, AUGUR_EINVAL, "field_add: bad arguments");
+  AUGUR_REQUIRE((reinterpret_cast<uintptr_t>(y) & 15) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0,
+                AUGUR_EINVAL, "field_add: y and x must be 16-byte aligned");
+  if (n == 0) return AUGUR_OK;
+  hipStream_t s = as_stream(stream);
+  TimedScope ts_("field_add", s);
+  hipLaunchKernelGGL(k_field_add, dim3(blocks_for((n + 3) / 4, 256, 4096)), dim3(256), 0, s, y, x, alpha, n, skip);
+  AUGUR_LAUNCHED("field_add");
+  return AUGUR_OK;
+}
+
 int augur_adam_update(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                       float eps, int step, float gscale, uint16_t* p_bf16, void* stream) {
   AUGUR_REQUIRE(p && g && m && v && n >= 0 && step >= 1, AUGUR_EINVAL, "adam: bad arguments");

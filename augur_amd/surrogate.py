@@ -537,12 +537,18 @@ class PMSurrogate:
         """Sorted device state of a scene (PeriodicPM.load; acc from plain PM)."""
         return self.pm.load(pos, vel, mass, mass_value)
 
-    def kdk(self, st, dt, steps, probe=None, threshold=None):
+    def kdk(self, st, dt, steps, probe=None, threshold=None, reuse=1):
         """`steps` KDK steps of the state with the surrogate force.  Each step
         runs the PM half (re-sort, deposit, FFT) into the state's workspace,
         the network on those grids with its output layer writing straight into
         the PM force planes (correction: += force_scale * net; replace:
         = force_scale * net), then the fused gather and kicks.
+
+        reuse = k > 1 (correction mode): the network runs on steps 0, k, 2k, ...
+        of the call only; its correction field is kept (3 G^3 FP32) and added
+        to the PM force planes of the k - 1 steps that follow.  The correction
+        then lags the particles by up to k - 1 steps: cheaper steps, a larger
+        force error (the bench reports both).
 
         With `probe` slot indices every step is checked against plain PM on
         the device: the force planes are gathered at the probes before and
@@ -571,6 +577,9 @@ class PMSurrogate:
             log = torch.zeros(max(1, steps), dtype=torch.float32, device=dev)
             ref = _ws(_lib.size("augur_sur_gate_scratch_bytes", P), dev)
             _lib.call("augur_sur_gate", 0, None, None, None, 0, G, self.L, None, _p(gate), None, None, 0, stream)
+        reuse = int(reuse)
+        assert reuse >= 1 and (reuse == 1 or self.mode == "correction"), "reuse needs a correction net"
+        corr = torch.empty(3, G, G, G, dtype=torch.float32, device=dev) if reuse > 1 else None
         mean_density = None
         for s in range(steps):
             self.pm.step_begin(st, dt, s == 0)
@@ -581,10 +590,16 @@ class PMSurrogate:
             if self.rho_scale is None:
                 self.calibrate(rho, force)
             if self.mode == "correction":
-                if mean_density is None:   # total mass / L^3
-                    mean_density = self._mass_total(st) / self.L ** 3
-                xc = self.packed_features(rho, force, mean_density)
-                self.net.forward_into(xc, force, self.force_scale, accumulate=True, skip=gate)
+                if corr is None or s % reuse == 0:
+                    if mean_density is None:   # total mass / L^3
+                        mean_density = self._mass_total(st) / self.L ** 3
+                    xc = self.packed_features(rho, force, mean_density)
+                    if corr is None:
+                        self.net.forward_into(xc, force, self.force_scale, accumulate=True, skip=gate)
+                    else:
+                        self.net.forward_into(xc, corr, self.force_scale, accumulate=False, skip=gate)
+                if corr is not None:
+                    _lib.call("augur_field_add", _p(force), _p(corr), 1.0, corr.numel(), _p(gate), stream)
             else:
                 xc = pack_cl(self.features_from_grids(rho, force), self.net.cinp)
                 self.net.forward_into(xc, force, self.force_scale, accumulate=False, skip=gate)

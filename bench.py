@@ -137,6 +137,7 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
     print(f"[bench] surrogate {a.steps} steps in {sec * 1e3:.2f} ms -> {value:.4e} particle-steps/s "
           f"(threshold {sur.check_threshold:.3e}, fell back: {fell_back}, max probe residual {res:.3e}; "
           f"a 2e-2 bound would fall back at step {at_2e2})", file=sys.stderr, flush=True)
+    reuse = correction_reuse(a, sur, N, G, pos0, torch.from_numpy(vel_np).to(dev), probe, oracle, pm_value)
     conv = {}
     # algorithmic flop per voxel (2 * 27 * cin * cout of the layer; the kernels pad the 4 input
     # channels of the first layer to 16 and the 3 outputs of the last to 16, which is not counted)
@@ -171,7 +172,65 @@ def surrogate_bench(a, N, G, pos_np, vel_np, dev, oracle, pm_value, pm_rel):
         "conv_kernels": conv,
         "conv_tiles": {"autotune_seconds": t_tune, "chosen": {k: v["best"] for k, v in tuner.report().items()}},
         "kernels": {k: {"us_avg": v[0] / v[1] * 1e3, "launches": v[1]} for k, v in timing.items()},
+        "correction_reuse": reuse,
     }
+
+
+def correction_reuse(a, sur, N, G, pos0, vel0, probe, oracle, pm_value, ks=(1, 2, 4, 8), steps=8, targets=4096):
+    """The correction net run every k-th step only (PMSurrogate.kdk(reuse=k)):
+    speed over `steps` steps of the workload for each k, and the force error at
+    the stalest step (the correction made at step 0, used k - 1 steps later):
+    PM(x_j) + correction(x_0) against FP64 Ewald at x_j for j = k - 1, on the
+    first `targets` of the workload's targets, x_j from plain PM steps."""
+    import torch
+    from augur_amd import ops
+    from augur_amd.surrogate import rel_rms
+
+    out = []
+    for k in ks:
+        st = sur.load(pos0, vel0, mass_value=1.0 / N)
+        sur.kdk(st, a.dt, steps, probe=probe, reuse=k)   # warm-up call of the same shape
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gate = sur.kdk(st, a.dt, steps, probe=probe, reuse=k)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        value = N * steps / sec
+        row = {"k": k, "steps": steps, "ms_per_step": sec * 1e3 / steps, "value": value,
+               "speed_ratio_vs_plain_pm": value / pm_value, "fell_back": gate.fell_back,
+               "max_probe_residual": gate.residual}
+        del st
+        out.append(row)
+    if oracle is not None:
+        tg = oracle[0][:targets].contiguous()
+        rho0, f0 = sur.pm.fields(pos0, mass_value=1.0 / N)
+        corr = torch.empty(3, G, G, G, dtype=torch.float32, device=pos0.device)
+        sur.net.forward_into(sur.packed_features(rho0, f0, (1.0 / N) * N / sur.L ** 3), corr, sur.force_scale,
+                             accumulate=False)
+        del rho0, f0
+        for row in out:
+            lag = row["k"] - 1
+            st = sur.pm.load(pos0, vel0, mass_value=1.0 / N)
+            if lag > 0:
+                sur.pm.kdk(st, a.dt, lag)
+            pj = sur.pm.unload(st)[0].contiguous()
+            del st
+            _, fj = sur.pm.fields(pj, mass_value=1.0 / N)
+            a_ex = ops.ewald_f64(pj, 1.0 / N, tg, L=1.0, eps=1.0 / G)
+            pt = pj[tg].contiguous()
+            row["force_rel_rms_at_lag"] = {"lag_steps": lag, "targets": int(tg.numel()),
+                                           "surrogate": rel_rms(sur.gather(fj + corr, pt), a_ex),
+                                           "plain_pm": rel_rms(sur.gather(fj, pt), a_ex)}
+            del fj, pj
+            torch.cuda.empty_cache()
+        del corr
+    for row in out:
+        e = row.get("force_rel_rms_at_lag", {})
+        print(f"[bench] correction every {row['k']} steps: {row['ms_per_step']:.3f} ms/step "
+              f"({row['speed_ratio_vs_plain_pm']:.3f}x plain PM), force rel-RMS "
+              f"{e.get('surrogate', float('nan')):.4e} after {e.get('lag_steps', '-')} steps "
+              f"(plain PM {e.get('plain_pm', float('nan')):.4e})", file=sys.stderr, flush=True)
+    return out
 
 
 def measured_traffic(kernel, N, G):

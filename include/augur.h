@@ -462,6 +462,13 @@ int augur_conv3_bf16_fwd_into(const uint16_t* x, int cin, const uint16_t* w, con
                               int64_t G, int cout, float alpha, int accumulate, float* field, const uint32_t* skip,
                               int zc, void* stream);
 
+/* y += alpha x over n FP32 values (y, x 16-byte aligned); the launch does
+ * nothing while skip (may be NULL) points at a non-zero word.  A stored
+ * correction field added to later steps' PM force planes (correction reuse
+ * every k steps); no counterpart in the reference, whose net runs every
+ * force evaluation (scripts/scatter_field.py:620-666). */
+int augur_field_add(float* y, const float* x, float alpha, int64_t n, const uint32_t* skip, void* stream);
+
 /* Weight gradient dw[27][32][cin] = sum_v dz[v][co] x[v + o_tap][ci] and
  * db[32] = sum_v dz[v][co] (FP32), dz BF16 channels-last [B][G^3][32].
  * nwg persistent workgroups each write a partial that is summed in a fixed

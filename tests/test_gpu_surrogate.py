@@ -199,3 +199,33 @@ def test_kdk_with_gate_is_capturable(dev):
         for x, y in zip((st.pos, st.vel, st.acc), want):
             assert torch.equal(x, y)
         assert cap.residuals == want_res
+
+
+@pytest.mark.parametrize("reuse,steps", [(3, 3), (2, 4)])
+def test_correction_reuse_adds_the_stored_correction(dev, reuse, steps):
+    """kdk(reuse=k): the net runs on steps 0, k, 2k, ... only and its stored
+    correction is added to the PM force planes of the steps in between; the
+    call equals that sequence done step by step with the stepper's pieces, bit
+    for bit, and the device residual check still logs every step."""
+    from augur_amd.surrogate import PMSurrogate
+    n, G, dt = 20000, 64, 1e-3
+    pos = clumpy_scene(8, n, dev)
+    vel = 0.3 * torch.randn(n, 3, device=dev)
+    sur = PMSurrogate(G, device=dev, mode="correction", init_std=0.02)
+    rho, force = sur.pm.fields(pos, mass_value=1.0 / n)
+    sur.calibrate(rho, force)
+    a, b = sur.load(pos, vel), sur.load(pos, vel)
+    probe = torch.arange(0, n, 3, device=dev)
+    gate = sur.kdk(b, dt, steps, probe=probe, threshold=float("inf"), reuse=reuse)
+    corr = torch.empty(3, G, G, G, dtype=torch.float32, device=dev)
+    md = sur._mass_total(a) / sur.L ** 3
+    for s in range(steps):
+        sur.pm.step_begin(a, dt, s == 0)
+        rho, force = sur.pm.grids(a)
+        if s % reuse == 0:
+            sur.net.forward_into(sur.packed_features(rho, force, md), corr, sur.force_scale, accumulate=False)
+        force.add_(corr)
+        sur.pm.step_end(a, dt, s == steps - 1)
+    assert not gate.fell_back and all(r > 0 for r in gate.residuals)
+    for x, y in ((a.pos, b.pos), (a.vel, b.vel), (a.acc, b.acc)):
+        assert torch.equal(x, y)
