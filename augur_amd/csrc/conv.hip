@@ -1036,10 +1036,10 @@ static_assert(WZ_XCS >= 3 * WZ_SLOT && (WZ_XCS / 2) % 64 == 4 && (WZ_DCS / 2) % 
 __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* __restrict__ x,
                                                                 const uint16_t* __restrict__ dz, int B, int G,
                                                                 int zc, int64_t ntasks, float* __restrict__ part) {
-  constexpr int XCH = WZ_HY * (WZ_HX / 2) * 4;  // voxel pairs x 16-byte chunks of an x plane (272)
-  constexpr int XPER = (XCH + CONV_THREADS - 1) / CONV_THREADS;
-  constexpr int DCH = WZ_Y * WZ_X * 4;          // 16-byte chunks of a dz plane (256)
-  static_assert(DCH == CONV_THREADS, "one dz chunk per thread");
+  static_assert(CONV_THREADS == 256 && WZ_HY == 4 && WZ_Y == 2 && WZ_X == 32,
+                "staging threads: 16 channel pairs x 4 runs of 8 voxels x 4 x rows (2 dz rows)");
+  static_assert(WZ_XCS % 8 == 0 && WZ_SLOT % 8 == 0 && WZ_XS % 8 == 0 && WZ_DCS % 8 == 0 && WZ_DS % 8 == 0,
+                "16-byte aligned runs");
   __shared__ __attribute__((aligned(16))) uint16_t xl[2][32 * WZ_XCS];  // [0]: voxel tx at tx; [1]: at tx - 1
   __shared__ __attribute__((aligned(16))) uint16_t dl[32 * WZ_DCS];
   __shared__ float bred[4][32];
@@ -1059,60 +1059,71 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* 
 #pragma unroll
   for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
 
-  // one x plane (rows y0-1 .. y0+2, voxels x0-1 .. x0+32, in voxel pairs)
-  // and one dz plane travel global -> registers -> LDS (channel-major)
-  u32x4 rx[XPER][2], rd;
+  // one x plane (rows y0-1 .. y0+2, halo voxels 0 .. 33 = x0-1 .. x0+32) and
+  // one dz plane travel global -> registers -> LDS (channel-major).  Thread
+  // (cp, vg, ty) of the x plane holds channels 2cp, 2cp+1 of halo voxels
+  // 8vg .. 8vg+8 (and 8vg+9 for vg = 3) as 4-byte words; byte permutes pair
+  // neighbouring voxels, so each copy is written as 16-byte runs of 8 voxels
+  // of one channel (copy 0 runs at halo voxel 8vg, copy 1 at 8vg+1), plus
+  // copy 0's last two voxels from vg = 3.  Lanes cp = 0..7 of a 16-byte store
+  // land on banks 8cp .. 8cp+3: conflict-free.  The dz plane is staged the
+  // same way by threads 0..127 (channels 2cp, 2cp+1 of voxels 8vg .. 8vg+7
+  // of row r).
+  uint32_t rx[10], rd[8];
   int cb = 0, cx0 = 0, cy0 = 0;  // the task's scene and column origin
+  const int xcp = tid & 15, xvg = (tid >> 4) & 3, xty = tid >> 6;
   auto load_x = [&](int zp) {
     const int gz = zp & (G - 1);
+    const int gy = (cy0 + xty - 1) & (G - 1);
+    const uint32_t* row =
+        reinterpret_cast<const uint32_t*>(x) + ((int64_t)cb * G3 + ((int64_t)gz * G + gy) * G) * 16 + xcp;
 #pragma unroll
-    for (int k = 0; k < XPER; ++k) {
-      const int c = tid + k * CONV_THREADS;
-      if (c < XCH) {
-        const int ch = c & 3, tp = (c >> 2) % (WZ_HX / 2), ty = (c >> 2) / (WZ_HX / 2);
-        const int gy = (cy0 + ty - 1) & (G - 1);
-        const int64_t row = (int64_t)cb * G3 + ((int64_t)gz * G + gy) * G;
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int gx = (cx0 + 2 * tp + u - 1) & (G - 1);
-          rx[k][u] = *reinterpret_cast<const u32x4*>(&x[(row + gx) * 32 + ch * 8]);
-        }
+    for (int u = 0; u < 10; ++u) {
+      if (u < 9 || xvg == 3) {
+        const int gx = (cx0 + 8 * xvg + u - 1) & (G - 1);
+        rx[u] = row[(int64_t)gx * 16];
       }
     }
   };
   auto store_x = [&](int slot) {
+    const int o = 2 * xcp * WZ_XCS + slot * WZ_SLOT + xty * WZ_XS + 8 * xvg;
+    u32x4 lo0, hi0, lo1, hi1;  // copies 0 and 1 of channels 2cp (lo) and 2cp + 1 (hi)
 #pragma unroll
-    for (int k = 0; k < XPER; ++k) {
-      const int c = tid + k * CONV_THREADS;
-      if (c < XCH) {
-        const int ch = c & 3, tp = (c >> 2) % (WZ_HX / 2), ty = (c >> 2) / (WZ_HX / 2);
-        const int o = slot * WZ_SLOT + ty * WZ_XS + 2 * tp;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const uint32_t a0 = rx[k][0][e], a1 = rx[k][1][e];  // channels 2e (low half), 2e + 1 (high half)
-          const int c0 = (ch * 8 + 2 * e) * WZ_XCS, c1 = c0 + WZ_XCS;
-          *reinterpret_cast<uint32_t*>(&xl[0][c0 + o]) = (a0 & 0xffffu) | (a1 << 16);
-          *reinterpret_cast<uint32_t*>(&xl[0][c1 + o]) = (a0 >> 16) | (a1 & 0xffff0000u);
-          xl[1][c0 + o] = (uint16_t)(a1 & 0xffffu);
-          xl[1][c1 + o] = (uint16_t)(a1 >> 16);
-          if (tp > 0) {
-            xl[1][c0 + o - 1] = (uint16_t)(a0 & 0xffffu);
-            xl[1][c1 + o - 1] = (uint16_t)(a0 >> 16);
-          }
-        }
-      }
+    for (int j = 0; j < 4; ++j) {
+      lo0[j] = (rx[2 * j] & 0xffffu) | (rx[2 * j + 1] << 16);
+      hi0[j] = (rx[2 * j] >> 16) | (rx[2 * j + 1] & 0xffff0000u);
+      lo1[j] = (rx[2 * j + 1] & 0xffffu) | (rx[2 * j + 2] << 16);
+      hi1[j] = (rx[2 * j + 1] >> 16) | (rx[2 * j + 2] & 0xffff0000u);
+    }
+    *reinterpret_cast<u32x4*>(&xl[0][o]) = lo0;
+    *reinterpret_cast<u32x4*>(&xl[0][o + WZ_XCS]) = hi0;
+    *reinterpret_cast<u32x4*>(&xl[1][o]) = lo1;
+    *reinterpret_cast<u32x4*>(&xl[1][o + WZ_XCS]) = hi1;
+    if (xvg == 3) {
+      *reinterpret_cast<uint32_t*>(&xl[0][o + 8]) = (rx[8] & 0xffffu) | (rx[9] << 16);
+      *reinterpret_cast<uint32_t*>(&xl[0][o + 8 + WZ_XCS]) = (rx[8] >> 16) | (rx[9] & 0xffff0000u);
     }
   };
-  const int dch = tid & 3, dtx = (tid >> 2) % WZ_X, dty = (tid >> 2) / WZ_X;
+  const int dr = (tid >> 6) & 1;
   auto load_dz = [&](int zp) {
-    const int64_t gv = ((int64_t)zp * G + (cy0 + dty)) * G + cx0 + dtx;
-    rd = *reinterpret_cast<const u32x4*>(&dz[((int64_t)cb * G3 + gv) * 32 + dch * 8]);
+    if (tid < 128) {
+      const uint32_t* row = reinterpret_cast<const uint32_t*>(dz) +
+                            ((int64_t)cb * G3 + ((int64_t)zp * G + (cy0 + dr)) * G + cx0 + 8 * xvg) * 16 + xcp;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) rd[u] = row[u * 16];
+    }
   };
   auto store_dz = [&]() {
+    if (tid < 128) {
+      const int o = 2 * xcp * WZ_DCS + dr * WZ_DS + 8 * xvg;
+      u32x4 lo, hi;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      dl[(dch * 8 + 2 * e) * WZ_DCS + dty * WZ_DS + dtx] = (uint16_t)(rd[e] & 0xffffu);
-      dl[(dch * 8 + 2 * e + 1) * WZ_DCS + dty * WZ_DS + dtx] = (uint16_t)(rd[e] >> 16);
+      for (int j = 0; j < 4; ++j) {
+        lo[j] = (rd[2 * j] & 0xffffu) | (rd[2 * j + 1] << 16);
+        hi[j] = (rd[2 * j] >> 16) | (rd[2 * j + 1] & 0xffff0000u);
+      }
+      *reinterpret_cast<u32x4*>(&dl[o]) = lo;
+      *reinterpret_cast<u32x4*>(&dl[o + WZ_DCS]) = hi;
     }
   };
   // the MFMAs of one plane for wave WV: x planes z-1, z, z+1 in ring slots
