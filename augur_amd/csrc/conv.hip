@@ -19,6 +19,7 @@ namespace augur {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+typedef __attribute__((ext_vector_type(2))) uint32_t u32x2;
 
 constexpr int CT_X = 32, CT_Y = 4, CT_Z = 2;
 constexpr int CONV_THREADS = 256;
@@ -438,7 +439,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
   const int64_t G3 = (int64_t)G * G * G;
   const uint16_t* xb = x + (int64_t)b * G3 * CIN;
   const int lane = tid & 63, wv = tid >> 6;
-  const int hf = wv >> 2, wrow = wv & 3;  // voxels 16 hf .. + 15 of rows 4 wrow .. + 3
+  const int hf = wv >> 2, wrow = wv & 3;  // voxels 2 n16 + hf (even or odd) of rows 4 wrow .. + 3
   const int n16 = lane & 15, kq = lane >> 4;
   if (tid < 32) sbias[tid] = (MODE == CONV_OUT_F32_CL || tid >= cout) ? 0.0f : bias[tid];
 
@@ -471,6 +472,9 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
       }
     }
   };
+  // the plane ring as in k_conv3_zm_split: one barrier per plane, after the
+  // dz = 0 MFMAs, then plane z + 2 goes into the slot of plane z - 1
+  u32x4 pre[PER];
   {
     u32x4 r[PER];
 #pragma unroll 1
@@ -479,6 +483,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
       store_plane(s, r);
     }
   }
+  load_plane(zs + 2, pre);  // zc >= 2
   __syncthreads();
 
   // the into-field epilogue reads and writes this scene's planes through a
@@ -496,14 +501,12 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
 #pragma unroll 1
   for (int zi = 0; zi < zc; ++zi) {
     const int z = zs + zi;
-    u32x4 pre[PER];
-    if (zi + 1 < zc) load_plane(z + 2, pre);
     // the field values this thread updates, fetched before the MFMAs (in place, accumulating)
     float old[ZM_RPW][4];
     if (MODE == CONV_OUT_F32_CF_INTO) {
 #pragma unroll
       for (int r = 0; r < ZM_RPW; ++r) {
-        const int64_t sv = ((int64_t)z * G + y0 + wrow * ZM_RPW + r) * G + x0 + hf * 16 + n16;
+        const int64_t sv = ((int64_t)z * G + y0 + wrow * ZM_RPW + r) * G + x0 + 2 * n16 + hf;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           old[r][q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rfield, field_off(4 * kq + q, sv), 0, 0));
@@ -514,14 +517,19 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
     for (int r = 0; r < ZM_RPW; ++r) acc[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int dz = 0; dz < 3; ++dz) {
+      if (dz == 1) {
+        __syncthreads();
+        if (zi + 1 < zc) store_plane(zi % 3, pre);
+        if (zi + 2 < zc) load_plane(z + 3, pre);
+      }
       const uint16_t* P = ring + ((zi + dz) % 3) * PLANE;
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
         for (int yr = 0; yr < ZM_RPW + 2; ++yr) {
-          // B operand: voxel 16 hf + n16 + dx - 1 of the row, ci = 8 kq .. 8 kq + 7
+          // B operand: voxel 2 n16 + hf + dx - 1 of the row, ci = 8 kq .. 8 kq + 7 (40-dword lane stride)
           const bf16x8 av =
-              *reinterpret_cast<const bf16x8*>(&P[((wrow * ZM_RPW + yr) * ZM_PX + hf * 16 + n16 + dx) * CP + kq * 8]);
+              *reinterpret_cast<const bf16x8*>(&P[((wrow * ZM_RPW + yr) * ZM_PX + 2 * n16 + hf + dx) * CP + kq * 8]);
 #pragma unroll
           for (int r = 0; r < ZM_RPW; ++r) {
             const int dy = yr - r;
@@ -530,11 +538,11 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
           }
         }
     }
-    // acc[r][q] holds output channel 4 kq + q of voxel (x0 + 16 hf + n16, y0 + row, z)
+    // acc[r][q] holds output channel 4 kq + q of voxel (x0 + 2 n16 + hf, y0 + row, z)
 #pragma unroll
     for (int r = 0; r < ZM_RPW; ++r) {
       const int ly = wrow * ZM_RPW + r;
-      const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + hf * 16 + n16;
+      const int64_t sv = ((int64_t)z * G + y0 + ly) * G + x0 + 2 * n16 + hf;
       if (CF) {  // channel-first planes of this scene, through the buffer view
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -557,11 +565,6 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_narrow(const uint16_t* 
           }
         }
       }
-    }
-    __syncthreads();
-    if (zi + 1 < zc) {
-      store_plane(zi % 3, pre);
-      __syncthreads();
     }
   }
 }
@@ -634,6 +637,13 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
       }
     }
   };
+  // slot zi % 3 holds plane z - 1 at step zi (z = zs + zi), slot (zi + 1) % 3
+  // plane z and slot (zi + 2) % 3 plane z + 1.  One barrier per plane, after
+  // the dz = 0 MFMAs: every wave is then done with plane z - 1, so plane z + 2
+  // (fetched into registers a step earlier) goes into its slot, and the next
+  // barrier publishes it before step zi + 1 reads it as dz = 2.  A wave's
+  // epilogue runs while the other waves start the next plane.
+  u32x4 pre[PER];
   {
     u32x4 r[PER];
 #pragma unroll 1
@@ -642,13 +652,12 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
       store_plane(s, r);
     }
   }
+  load_plane(zs + 2, pre);  // zc >= 2
   __syncthreads();
 
 #pragma unroll 1
   for (int zi = 0; zi < zc; ++zi) {
     const int z = zs + zi;
-    u32x4 pre[PER];
-    if (zi + 1 < zc) load_plane(z + 2, pre);
     f32x4 acc[ZM_RPW][2];
 #pragma unroll
     for (int r = 0; r < ZM_RPW; ++r)
@@ -656,6 +665,11 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
       for (int hf = 0; hf < 2; ++hf) acc[r][hf] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int dz = 0; dz < 3; ++dz) {
+      if (dz == 1) {
+        __syncthreads();
+        if (zi + 1 < zc) store_plane(zi % 3, pre);
+        if (zi + 2 < zc) load_plane(z + 3, pre);
+      }
       const uint16_t* P = ring + ((zi + dz) % 3) * PLANE;
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx)
@@ -663,9 +677,11 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
         for (int yr = 0; yr < ZM_RPW + 2; ++yr)
 #pragma unroll
           for (int hf = 0; hf < 2; ++hf) {
-            // B operand: voxel 16 hf + n16 + dx - 1 of the row, ci = 8 kq .. 8 kq + 7
+            // B operand: voxel 2 n16 + hf + dx - 1 of the row, ci = 8 kq .. 8 kq + 7 (the even
+            // or odd voxels: a 40-dword lane stride, so the 32 lanes served together hit
+            // distinct banks)
             const bf16x8 av = *reinterpret_cast<const bf16x8*>(
-                &P[((wrow * ZM_RPW + yr) * ZM_PX + hf * 16 + n16 + dx) * CP + kq * 8]);
+                &P[((wrow * ZM_RPW + yr) * ZM_PX + 2 * n16 + hf + dx) * CP + kq * 8]);
 #pragma unroll
             for (int r = 0; r < ZM_RPW; ++r) {
               const int dy = yr - r;
@@ -674,64 +690,91 @@ __global__ __launch_bounds__(ZS_THREADS) void k_conv3_zm_split(const uint16_t* _
             }
           }
     }
-    // acc[r][hf][q] holds output channel 16 half + 4 kq + q of voxel (x0 + 16 hf + n16, y0 + row, z)
+    // acc[r][hf][q] holds output channel 16 half + 4 kq + q of voxel (x0 + 2 n16 + hf, y0 + row, z)
     const uint16_t* C = ring + ((zi + 1) % 3) * PLANE;  // input plane z (residual centre)
     const int co0 = 16 * half + 4 * kq;
     const float4 bj = *reinterpret_cast<const float4*>(&sbias[co0]);
+    // plane z of each channels-last tensor through a raw buffer view (byte
+    // offsets below G^2 x 32 x 4 B <= 2^27); a null tensor gets an empty view,
+    // so its loads give 0 and its stores are dropped.  All of the plane's reads
+    // are issued before any is used: no branch or wait per row.
+    const int64_t pe = ((int64_t)b * G + z) * G * G * 32;  // first element of plane z
+    const uint32_t nb16 = (uint32_t)G * G * 64, nb32 = 2 * nb16;
+    const auto view = [](const void* p, uint32_t bytes) {
+      return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, p ? bytes : 0, 0x00020000);
+    };
+    constexpr bool CL = MODE == CONV_OUT_F32_CL || MODE == CONV_OUT_F32_CL_DGELU;
+    const __amdgpu_buffer_rsrc_t rz = view(z_out ? z_out + pe : nullptr, nb16);  // z_out, or z_in (mode 4)
+    const __amdgpu_buffer_rsrc_t rh = view(h_out ? h_out + pe : nullptr, nb16);  // h_out, or dz (mode 4)
+    const __amdgpu_buffer_rsrc_t ry = view(CL && y_out ? y_out + pe : nullptr, nb32);
+    const __amdgpu_buffer_rsrc_t ra = view(CL && addend ? addend + pe : nullptr, nb32);
+    // element offset of the lane's first channel at (row r, hf): e0 + 32 hf + r G 32.
+    // Loads take the row part as the scalar offset; stores take the whole
+    // offset in the vector register (a store with a scalar offset whose data
+    // registers are rewritten by the next instruction lost data on this GPU)
+    const uint32_t e0 = ((uint32_t)(y0 + wrow * ZM_RPW) * (uint32_t)G + (uint32_t)(x0 + 2 * n16)) * 32u + (uint32_t)co0;
+    const uint32_t rowe = (uint32_t)G * 32u;
+    // two rows at a time: their reads are issued together, then used
 #pragma unroll
-    for (int r = 0; r < ZM_RPW; ++r) {
-      const int ly = wrow * ZM_RPW + r;
+    for (int rp = 0; rp < ZM_RPW; rp += 2) {
+      u32x4 adr[2][2];   // addend (modes 3, 4)
+      uint2 side[2][2];  // residual centre (mode 2) or z_in (mode 4)
 #pragma unroll
-      for (int hf = 0; hf < 2; ++hf) {
-        const int lx = hf * 16 + n16;
-        const int64_t o = ((int64_t)b * G3 + ((int64_t)z * G + y0 + ly) * G + x0 + lx) * 32 + co0;
-        const float v0 = acc[r][hf][0] + bj.x, v1 = acc[r][hf][1] + bj.y, v2 = acc[r][hf][2] + bj.z,
-                    v3 = acc[r][hf][3] + bj.w;
-        if (MODE == CONV_OUT_F32_CL || MODE == CONV_OUT_F32_CL_DGELU) {  // FP32 channels-last, 32 channels
-          float4 val = {v0, v1, v2, v3};
-          if (addend) {
-            const float4 ad = *reinterpret_cast<const float4*>(&addend[o]);
-            val.x += ad.x;
-            val.y += ad.y;
-            val.z += ad.z;
-            val.w += ad.w;
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+          const int r = rp + i, ly = wrow * ZM_RPW + r, lx = 2 * n16 + hf;
+          if (CL) adr[i][hf] = __builtin_amdgcn_raw_buffer_load_b128(ra, (e0 + 32u * hf) * 4u, r * rowe * 4u, 0);
+          if (MODE == CONV_OUT_F32_CL_DGELU) {
+            const u32x2 zr = __builtin_amdgcn_raw_buffer_load_b64(rz, (e0 + 32u * hf) * 2u, r * rowe * 2u, 0);
+            side[i][hf] = make_uint2(zr[0], zr[1]);
           }
-          if (MODE == CONV_OUT_F32_CL || y_out) *reinterpret_cast<float4*>(&y_out[o]) = val;
-          if (MODE == CONV_OUT_F32_CL_DGELU) {  // dz = y * gelu'(z_in), BF16 (k_gelu_bwd's arithmetic)
-            const uint2 zr = *reinterpret_cast<const uint2*>(&z_out[o]);
-            uint2 dp;
-            dp.x = (uint32_t)f2bf(val.x * gelu_grad(__uint_as_float(zr.x << 16))) |
-                   ((uint32_t)f2bf(val.y * gelu_grad(__uint_as_float(zr.x & 0xffff0000u))) << 16);
-            dp.y = (uint32_t)f2bf(val.z * gelu_grad(__uint_as_float(zr.y << 16))) |
-                   ((uint32_t)f2bf(val.w * gelu_grad(__uint_as_float(zr.y & 0xffff0000u))) << 16);
-            *reinterpret_cast<uint2*>(&h_out[o]) = dp;
+          if (MODE == CONV_RES_GELU)
+            side[i][hf] = *reinterpret_cast<const uint2*>(&C[((ly + 1) * ZM_PX + 1 + lx) * CP + co0]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+          const int r = rp + i;
+          const uint32_t ev = e0 + 32u * hf + r * rowe;
+          const float v0 = acc[r][hf][0] + bj.x, v1 = acc[r][hf][1] + bj.y, v2 = acc[r][hf][2] + bj.z,
+                      v3 = acc[r][hf][3] + bj.w;
+          if (CL) {  // FP32 channels-last, 32 channels (+ addend: uniform select, no branch)
+            const u32x4 ad = adr[i][hf];
+            float4 val = {v0, v1, v2, v3};
+            if (addend) {
+              val.x += __uint_as_float(ad[0]);
+              val.y += __uint_as_float(ad[1]);
+              val.z += __uint_as_float(ad[2]);
+              val.w += __uint_as_float(ad[3]);
+            }
+            const u32x4 yv = {__float_as_uint(val.x), __float_as_uint(val.y), __float_as_uint(val.z),
+                              __float_as_uint(val.w)};
+            __builtin_amdgcn_raw_buffer_store_b128(yv, ry, ev * 4u, 0, 0);
+            if (MODE == CONV_OUT_F32_CL_DGELU) {  // dz = y * gelu'(z_in), BF16 (k_gelu_bwd's arithmetic)
+              const uint2 zr = side[i][hf];
+              const u32x2 dp = {(uint32_t)f2bf(val.x * gelu_grad(__uint_as_float(zr.x << 16))) |
+                                    ((uint32_t)f2bf(val.y * gelu_grad(__uint_as_float(zr.x & 0xffff0000u))) << 16),
+                                (uint32_t)f2bf(val.z * gelu_grad(__uint_as_float(zr.y << 16))) |
+                                    ((uint32_t)f2bf(val.w * gelu_grad(__uint_as_float(zr.y & 0xffff0000u))) << 16)};
+              __builtin_amdgcn_raw_buffer_store_b64(dp, rh, ev * 2u, 0, 0);
+            }
+            continue;
           }
-          continue;
+          const u32x2 zp = {pack_bf16x2(v0, v1), pack_bf16x2(v2, v3)};
+          __builtin_amdgcn_raw_buffer_store_b64(zp, rz, ev * 2u, 0, 0);
+          float h0 = gelu(v0), h1 = gelu(v1), h2 = gelu(v2), h3 = gelu(v3);
+          if (MODE == CONV_RES_GELU) {
+            const uint2 xr = side[i][hf];
+            h0 += __uint_as_float(xr.x << 16);
+            h1 += __uint_as_float(xr.x & 0xffff0000u);
+            h2 += __uint_as_float(xr.y << 16);
+            h3 += __uint_as_float(xr.y & 0xffff0000u);
+          }
+          const u32x2 hp = {pack_bf16x2(h0, h1), pack_bf16x2(h2, h3)};
+          __builtin_amdgcn_raw_buffer_store_b64(hp, rh, ev * 2u, 0, 0);
         }
-        if (z_out) {
-          uint2 zp;
-          zp.x = pack_bf16x2(v0, v1);
-          zp.y = pack_bf16x2(v2, v3);
-          *reinterpret_cast<uint2*>(&z_out[o]) = zp;
-        }
-        float h0 = gelu(v0), h1 = gelu(v1), h2 = gelu(v2), h3 = gelu(v3);
-        if (MODE == CONV_RES_GELU) {
-          const uint2 xr = *reinterpret_cast<const uint2*>(&C[((ly + 1) * ZM_PX + 1 + lx) * CP + co0]);
-          h0 += __uint_as_float(xr.x << 16);
-          h1 += __uint_as_float(xr.x & 0xffff0000u);
-          h2 += __uint_as_float(xr.y << 16);
-          h3 += __uint_as_float(xr.y & 0xffff0000u);
-        }
-        uint2 hp;
-        hp.x = pack_bf16x2(h0, h1);
-        hp.y = pack_bf16x2(h2, h3);
-        *reinterpret_cast<uint2*>(&h_out[o]) = hp;
-      }
-    }
-    __syncthreads();
-    if (zi + 1 < zc) {
-      store_plane(zi % 3, pre);
-      __syncthreads();
     }
   }
 }
