@@ -1055,26 +1055,30 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad(const uint16_t* __
 
 // Weight gradient of the 32-channel layers, streamed along z.  A task is a
 // column of 32x2 voxels (x, y) over WZ_ZC planes of one scene; the workgroup
-// walks it plane by plane: x planes (with their one-voxel x/y halo) pass
-// through a three-plane LDS ring, so each is staged once per task instead of
-// once per tap row, and dz through a one-plane buffer.  A plane is 4 K-steps
-// of 16 voxels; each step's dz fragment feeds the wave's taps (as
-// k_conv3_wgrad: two x copies, the second one voxel to the left, so a dx = 0
-// or 1 fragment is one 16-byte read and a dx = 2 fragment four 4-byte reads).
-// 75 KB of LDS: two workgroups per CU.  Persistent workgroups take tasks
-// s, s + nwg, ... (s groups each XCD's workgroups on adjacent columns) and
-// write one partial each, summed in workgroup order by k_sum_partials.
+// walks it plane by plane: x planes (with their one-row y halo) pass through a
+// three-plane LDS ring, so each is staged once per task instead of once per
+// tap row, and dz through a one-plane buffer.  A plane is 4 K-steps of 16
+// voxels; each step's dz fragment feeds the wave's taps.  Every fragment is
+// one aligned 16-byte read: x is held twice (copy 0 at voxel x0 + p - 1,
+// copy 1 at x0 + p), dz twice (copy 0 at x0 + p, copy 1 at x0 + p - 1).  A
+// dx = 0 or 1 tap pairs dz copy 0 with x copy 0 or 1; a dx = 2 tap sums its
+// column over voxels x0 - 1 .. x0 + 30 instead (every voxel is still summed
+// once over the periodic grid) and pairs dz copy 1 with x copy 1.  60 KB of
+// LDS: two workgroups per CU.  Persistent workgroups take tasks s, s + nwg,
+// ... (s groups each XCD's workgroups on adjacent columns) and write one
+// partial each, summed in workgroup order by k_sum_partials.
 constexpr int WZ_X = 32, WZ_Y = 2;
-constexpr int WZ_HX = WZ_X + 2, WZ_HY = WZ_Y + 2;
-constexpr int WZ_XS = WZ_HX + 6;        // row stride of a staged x plane (40)
-constexpr int WZ_SLOT = WZ_HY * WZ_XS;  // one plane of one channel (160)
-constexpr int WZ_XCS = 520;             // channel stride of an x copy: 3 planes + pad (260 dwords)
-constexpr int WZ_DS = WZ_X + 8;         // row stride of the dz plane (40)
-constexpr int WZ_DCS = 136;             // channel stride of the dz plane: 2 rows + pad (68 dwords)
+constexpr int WZ_HY = WZ_Y + 2;
+constexpr int WZ_XS = WZ_X;             // row stride of a staged x plane (32 voxels)
+constexpr int WZ_SLOT = WZ_HY * WZ_XS;  // one plane of one channel (128)
+constexpr int WZ_XCS = 392;             // channel stride of an x copy: 3 planes + pad (196 dwords)
+constexpr int WZ_DS = WZ_X;             // row stride of the dz plane (32)
+constexpr int WZ_DCS = 72;              // channel stride of a dz copy: 2 rows + pad (36 dwords)
 constexpr int WZ_ZC = 64;               // planes per task (fewer when G is smaller)
-static_assert(WZ_XCS >= 3 * WZ_SLOT && (WZ_XCS / 2) % 64 == 4 && (WZ_DCS / 2) % 64 == 4 &&
-                  WZ_DCS >= WZ_Y * WZ_DS,
-              "16-byte fragment reads of 16 consecutive channels hit distinct banks");
+// 32 lanes (consecutive channels) of a 16-byte read are served together: a
+// channel stride of 4 x odd dwords (mod 128) puts them on distinct banks
+static_assert(WZ_XCS >= 3 * WZ_SLOT && (WZ_XCS / 2) % 8 == 4 && (WZ_DCS / 2) % 8 == 4 && WZ_DCS >= WZ_Y * WZ_DS,
+              "16-byte fragment reads of 32 consecutive channels hit distinct banks");
 
 __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* __restrict__ x,
                                                                 const uint16_t* __restrict__ dz, int B, int G,
@@ -1083,8 +1087,8 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* 
                 "staging threads: 16 channel pairs x 4 runs of 8 voxels x 4 x rows (2 dz rows)");
   static_assert(WZ_XCS % 8 == 0 && WZ_SLOT % 8 == 0 && WZ_XS % 8 == 0 && WZ_DCS % 8 == 0 && WZ_DS % 8 == 0,
                 "16-byte aligned runs");
-  __shared__ __attribute__((aligned(16))) uint16_t xl[2][32 * WZ_XCS];  // [0]: voxel tx at tx; [1]: at tx - 1
-  __shared__ __attribute__((aligned(16))) uint16_t dl[32 * WZ_DCS];
+  __shared__ __attribute__((aligned(16))) uint16_t xl[2][32 * WZ_XCS];  // [0]: voxel x0 + p - 1 at p; [1]: x0 + p
+  __shared__ __attribute__((aligned(16))) uint16_t dl[2][32 * WZ_DCS];  // [0]: voxel x0 + p at p; [1]: x0 + p - 1
   __shared__ float bred[4][32];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i = lane & 31, kh = lane >> 5;
@@ -1102,17 +1106,15 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* 
 #pragma unroll
   for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
 
-  // one x plane (rows y0-1 .. y0+2, halo voxels 0 .. 33 = x0-1 .. x0+32) and
-  // one dz plane travel global -> registers -> LDS (channel-major).  Thread
-  // (cp, vg, ty) of the x plane holds channels 2cp, 2cp+1 of halo voxels
-  // 8vg .. 8vg+8 (and 8vg+9 for vg = 3) as 4-byte words; byte permutes pair
+  // one x plane (rows y0-1 .. y0+2, voxels x0-1 .. x0+31) and one dz plane
+  // (rows y0, y0+1, voxels x0-1 .. x0+31) travel global -> registers -> LDS
+  // (channel-major).  Thread (cp, vg, ty) holds channels 2cp, 2cp+1 of voxels
+  // x0 + 8vg - 1 .. x0 + 8vg + 7 as 4-byte words; byte permutes pair
   // neighbouring voxels, so each copy is written as 16-byte runs of 8 voxels
-  // of one channel (copy 0 runs at halo voxel 8vg, copy 1 at 8vg+1), plus
-  // copy 0's last two voxels from vg = 3.  Lanes cp = 0..7 of a 16-byte store
-  // land on banks 8cp .. 8cp+3: conflict-free.  The dz plane is staged the
-  // same way by threads 0..127 (channels 2cp, 2cp+1 of voxels 8vg .. 8vg+7
-  // of row r).
-  uint32_t rx[10], rd[8];
+  // of one channel at position 8vg.  Lanes cp = 0..7 of a 16-byte store land
+  // on distinct banks.  Threads 0..127 stage the dz plane the same way (row r
+  // for ty = r).
+  uint32_t rx[9], rd[9];
   int cb = 0, cx0 = 0, cy0 = 0;  // the task's scene and column origin
   const int xcp = tid & 15, xvg = (tid >> 4) & 3, xty = tid >> 6;
   auto load_x = [&](int zp) {
@@ -1121,52 +1123,50 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* 
     const uint32_t* row =
         reinterpret_cast<const uint32_t*>(x) + ((int64_t)cb * G3 + ((int64_t)gz * G + gy) * G) * 16 + xcp;
 #pragma unroll
-    for (int u = 0; u < 10; ++u) {
-      if (u < 9 || xvg == 3) {
-        const int gx = (cx0 + 8 * xvg + u - 1) & (G - 1);
-        rx[u] = row[(int64_t)gx * 16];
-      }
+    for (int u = 0; u < 9; ++u) {
+      const int gx = (cx0 + 8 * xvg + u - 1) & (G - 1);
+      rx[u] = row[(int64_t)gx * 16];
     }
+  };
+  // 16-byte runs of channels 2cp (lo) and 2cp + 1 (hi): words w[j0 .. j0 + 8)
+  auto run_lo = [](const uint32_t (&w)[9], int j0) {
+    u32x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (w[j0 + 2 * j] & 0xffffu) | (w[j0 + 2 * j + 1] << 16);
+    return v;
+  };
+  auto run_hi = [](const uint32_t (&w)[9], int j0) {
+    u32x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (w[j0 + 2 * j] >> 16) | (w[j0 + 2 * j + 1] & 0xffff0000u);
+    return v;
   };
   auto store_x = [&](int slot) {
     const int o = 2 * xcp * WZ_XCS + slot * WZ_SLOT + xty * WZ_XS + 8 * xvg;
-    u32x4 lo0, hi0, lo1, hi1;  // copies 0 and 1 of channels 2cp (lo) and 2cp + 1 (hi)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      lo0[j] = (rx[2 * j] & 0xffffu) | (rx[2 * j + 1] << 16);
-      hi0[j] = (rx[2 * j] >> 16) | (rx[2 * j + 1] & 0xffff0000u);
-      lo1[j] = (rx[2 * j + 1] & 0xffffu) | (rx[2 * j + 2] << 16);
-      hi1[j] = (rx[2 * j + 1] >> 16) | (rx[2 * j + 2] & 0xffff0000u);
-    }
-    *reinterpret_cast<u32x4*>(&xl[0][o]) = lo0;
-    *reinterpret_cast<u32x4*>(&xl[0][o + WZ_XCS]) = hi0;
-    *reinterpret_cast<u32x4*>(&xl[1][o]) = lo1;
-    *reinterpret_cast<u32x4*>(&xl[1][o + WZ_XCS]) = hi1;
-    if (xvg == 3) {
-      *reinterpret_cast<uint32_t*>(&xl[0][o + 8]) = (rx[8] & 0xffffu) | (rx[9] << 16);
-      *reinterpret_cast<uint32_t*>(&xl[0][o + 8 + WZ_XCS]) = (rx[8] >> 16) | (rx[9] & 0xffff0000u);
-    }
+    *reinterpret_cast<u32x4*>(&xl[0][o]) = run_lo(rx, 0);
+    *reinterpret_cast<u32x4*>(&xl[0][o + WZ_XCS]) = run_hi(rx, 0);
+    *reinterpret_cast<u32x4*>(&xl[1][o]) = run_lo(rx, 1);
+    *reinterpret_cast<u32x4*>(&xl[1][o + WZ_XCS]) = run_hi(rx, 1);
   };
   const int dr = (tid >> 6) & 1;
   auto load_dz = [&](int zp) {
     if (tid < 128) {
-      const uint32_t* row = reinterpret_cast<const uint32_t*>(dz) +
-                            ((int64_t)cb * G3 + ((int64_t)zp * G + (cy0 + dr)) * G + cx0 + 8 * xvg) * 16 + xcp;
+      const uint32_t* row =
+          reinterpret_cast<const uint32_t*>(dz) + ((int64_t)cb * G3 + ((int64_t)zp * G + (cy0 + dr)) * G) * 16 + xcp;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) rd[u] = row[u * 16];
+      for (int u = 0; u < 9; ++u) {
+        const int gx = (cx0 + 8 * xvg + u - 1) & (G - 1);
+        rd[u] = row[(int64_t)gx * 16];
+      }
     }
   };
   auto store_dz = [&]() {
     if (tid < 128) {
       const int o = 2 * xcp * WZ_DCS + dr * WZ_DS + 8 * xvg;
-      u32x4 lo, hi;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        lo[j] = (rd[2 * j] & 0xffffu) | (rd[2 * j + 1] << 16);
-        hi[j] = (rd[2 * j] >> 16) | (rd[2 * j + 1] & 0xffff0000u);
-      }
-      *reinterpret_cast<u32x4*>(&dl[o]) = lo;
-      *reinterpret_cast<u32x4*>(&dl[o + WZ_DCS]) = hi;
+      *reinterpret_cast<u32x4*>(&dl[0][o]) = run_lo(rd, 1);
+      *reinterpret_cast<u32x4*>(&dl[0][o + WZ_DCS]) = run_hi(rd, 1);
+      *reinterpret_cast<u32x4*>(&dl[1][o]) = run_lo(rd, 0);
+      *reinterpret_cast<u32x4*>(&dl[1][o + WZ_DCS]) = run_hi(rd, 0);
     }
   };
   // the MFMAs of one plane for wave WV: x planes z-1, z, z+1 in ring slots
@@ -1177,8 +1177,9 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* 
     for (int st = 0; st < 2 * WZ_Y; ++st) {
       const int r = st >> 1, half = st & 1;
       const int vb = half * 16 + kh * 8;
-      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&dl[i * WZ_DCS + r * WZ_DS + vb]);
-      if ((st & 3) == WV) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ones, accb, 0, 0, 0);
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&dl[0][i * WZ_DCS + r * WZ_DS + vb]);
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&dl[1][i * WZ_DCS + r * WZ_DS + vb]);
+      if ((st & 3) == WV) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, ones, accb, 0, 0, 0);
 #pragma unroll
       for (int k = 0; k < DW_TAPS_PER_WAVE; ++k) {
         const int tap = WV + 4 * k;
@@ -1187,15 +1188,8 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* 
           int sl = s0 + dzo;
           sl = sl >= 3 ? sl - 3 : sl;
           const int o = i * WZ_XCS + sl * WZ_SLOT + (r + dyo) * WZ_XS + vb;
-          bf16x8 bv;
-          if (dxo == 2) {  // voxels vb + 2 .. vb + 9 of the unshifted copy: 4-byte aligned
-            const uint32_t* p = reinterpret_cast<const uint32_t*>(&xl[0][o + 2]);
-            const u32x4 u = {p[0], p[1], p[2], p[3]};
-            bv = __builtin_bit_cast(bf16x8, u);
-          } else {
-            bv = *reinterpret_cast<const bf16x8*>(&xl[dxo][o]);
-          }
-          acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[k], 0, 0, 0);
+          const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&xl[dxo == 0 ? 0 : 1][o]);
+          acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(dxo == 2 ? a1 : a0, bv, acc[k], 0, 0, 0);
         }
       }
     }
