@@ -84,7 +84,7 @@ class RaggedStepper:
         contrast = (rho / scene_density.view(-1, 1, 1, 1) - 1.0) * sur.rho_scale
         x = torch.cat([contrast[:, None], force / sur.force_scale], 1).contiguous()
         # the output layer adds force_scale * net(x) into the PM field in its epilogue
-        return sur.net.forward_into(pack_cl(x, sur.net.cinp), force, sur.force_scale, accumulate=True)
+        return sur.net.forward_into(pack_cl(x, sur.net.cpack), force, sur.force_scale, accumulate=True)
 
     def scene_density(self, starts, mass, total):
         """(B,) mass / L^3 of every scene, on the device: differences of the

@@ -1252,6 +1252,206 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_z(const uint16_t* 
   if (tid < 32) pw[27 * 32 * 32 + tid] = ((bred[0][tid] + bred[1][tid]) + bred[2][tid]) + bred[3][tid];
 }
 
+// Weight gradient of the 4-channel input layer: dw[tap][co][ci], ci < 4.  The
+// MFMA's 32 columns are (tap, channel) pairs, 8 taps x 4 channels, so the 27
+// taps take 5 MFMAs per K-step of 16 voxels: three for the 18 taps with
+// dx = 0 or 1 (dz copy 0 against x copy dx) and two for the 9 taps with
+// dx = 2 (dz copy 1 against x copy 1, the column of k_conv3_wgrad_z shifted
+// by one voxel); one more against a ones operand sums the bias.  Columns past
+// a group's taps read a zero fragment.  A task is a column of 32x8 voxels
+// over WZ_ZC planes of one scene; wave w takes rows 2w and 2w + 1 of each
+// plane (4 K-steps) and writes a partial of its own.  The input is 8 bytes a
+// voxel and dz 64: the kernel streams dz.
+constexpr int W4_Y = 8;                    // dz rows per task
+constexpr int W4_HY = W4_Y + 2;            // x rows (one-row halo each side)
+constexpr int W4_SLOT = W4_HY * 32;        // one x plane of one channel (320)
+constexpr int W4_XCS = 968;                // channel stride of an x copy: 3 planes + pad (484 dwords)
+constexpr int W4_DCS = 264;                // channel stride of a dz copy: 8 rows + pad (132 dwords)
+constexpr int W4_PART = 27 * 32 * 4 + 32;  // floats of one partial
+static_assert(W4_XCS >= 3 * W4_SLOT && (W4_XCS / 2) % 8 == 4 && (W4_DCS / 2) % 8 == 4 && W4_DCS >= W4_Y * 32,
+              "16-byte fragment reads of 32 consecutive channels hit distinct banks");
+
+__global__ __launch_bounds__(CONV_THREADS) void k_conv3_wgrad_in4(const uint16_t* __restrict__ x,
+                                                                  const uint16_t* __restrict__ dz, int B, int G,
+                                                                  int zc, int64_t ntasks, float* __restrict__ part) {
+  static_assert(CONV_THREADS == 256, "staging threads: x 2 channel pairs x 4 runs x 10 rows; dz 16 x 4 x 4 (2 rows)");
+  __shared__ __attribute__((aligned(16))) uint16_t xl[2][4 * W4_XCS];   // [0]: voxel x0 + p - 1 at p; [1]: x0 + p
+  __shared__ __attribute__((aligned(16))) uint16_t dl[2][32 * W4_DCS];  // [0]: voxel x0 + p at p; [1]: x0 + p - 1
+  __shared__ __attribute__((aligned(16))) uint16_t zero8[8];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i = lane & 31, kh = lane >> 5;
+  const int ntx = G / 32, nty = G / W4_Y, ntz = G / zc;
+  const int64_t G3 = (int64_t)G * G * G;
+  if (tid < 8) zero8[tid] = 0;
+  f32x16 acc[5], accb;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    accb[q] = 0.0f;
+#pragma unroll
+    for (int g = 0; g < 5; ++g) acc[g][q] = 0.0f;
+  }
+  bf16x8 ones;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
+  // this lane's column in group g: channel ci = i & 3 of tap (dzo, dyo, dx)
+  const int ci = i & 3, tc = i >> 2;
+  bool cval[5];
+  int ccopy[5], cdzo[5], cdyo[5], ctap[5];
+#pragma unroll
+  for (int g = 0; g < 5; ++g) {
+    if (g < 3) {
+      const int t = 8 * g + tc;  // the 18 taps with dx = 0, 1 in (dz, dy, dx) order
+      cval[g] = t < 18;
+      cdzo[g] = t / 6;
+      cdyo[g] = (t / 2) % 3;
+      ccopy[g] = t % 2;
+      ctap[g] = cdzo[g] * 9 + cdyo[g] * 3 + ccopy[g];
+    } else {
+      const int t = 8 * (g - 3) + tc;  // the 9 taps with dx = 2
+      cval[g] = t < 9;
+      cdzo[g] = t / 3;
+      cdyo[g] = t % 3;
+      ccopy[g] = 1;
+      ctap[g] = cdzo[g] * 9 + cdyo[g] * 3 + 2;
+    }
+  }
+
+  // x plane: rows y0-1 .. y0+8, voxels x0-1 .. x0+31, 4 channels; thread
+  // (cp, vg, ty) < 80 holds channels 2cp, 2cp+1 of voxels x0+8vg-1 .. x0+8vg+7.
+  // dz plane: rows y0 .. y0+7, thread (cp, vg, q) rows 2q, 2q+1, channels 2cp,
+  // 2cp+1.  Both written as 16-byte channel runs (byte permutes of voxel pairs).
+  uint32_t rx[9], rd[2][9];
+  int cb = 0, cx0 = 0, cy0 = 0;
+  const int xcp = tid & 1, xvg = (tid >> 1) & 3, xty = tid >> 3;
+  const int dcp = tid & 15, dvg = (tid >> 4) & 3, dq = tid >> 6;
+  auto run_lo = [](const uint32_t (&w)[9], int j0) {
+    u32x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (w[j0 + 2 * j] & 0xffffu) | (w[j0 + 2 * j + 1] << 16);
+    return v;
+  };
+  auto run_hi = [](const uint32_t (&w)[9], int j0) {
+    u32x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (w[j0 + 2 * j] >> 16) | (w[j0 + 2 * j + 1] & 0xffff0000u);
+    return v;
+  };
+  auto load_x = [&](int zp) {
+    if (tid < 2 * 4 * W4_HY) {
+      const int gz = zp & (G - 1);
+      const int gy = (cy0 + xty - 1) & (G - 1);
+      const uint32_t* row =
+          reinterpret_cast<const uint32_t*>(x) + ((int64_t)cb * G3 + ((int64_t)gz * G + gy) * G) * 2 + xcp;
+#pragma unroll
+      for (int u = 0; u < 9; ++u) {
+        const int gx = (cx0 + 8 * xvg + u - 1) & (G - 1);
+        rx[u] = row[(int64_t)gx * 2];
+      }
+    }
+  };
+  auto store_x = [&](int slot) {
+    if (tid < 2 * 4 * W4_HY) {
+      const int o = 2 * xcp * W4_XCS + slot * W4_SLOT + xty * 32 + 8 * xvg;
+      *reinterpret_cast<u32x4*>(&xl[0][o]) = run_lo(rx, 0);
+      *reinterpret_cast<u32x4*>(&xl[0][o + W4_XCS]) = run_hi(rx, 0);
+      *reinterpret_cast<u32x4*>(&xl[1][o]) = run_lo(rx, 1);
+      *reinterpret_cast<u32x4*>(&xl[1][o + W4_XCS]) = run_hi(rx, 1);
+    }
+  };
+  auto load_dz = [&](int zp) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t* row = reinterpret_cast<const uint32_t*>(dz) +
+                            ((int64_t)cb * G3 + ((int64_t)zp * G + (cy0 + 2 * dq + k)) * G) * 16 + dcp;
+#pragma unroll
+      for (int u = 0; u < 9; ++u) {
+        const int gx = (cx0 + 8 * dvg + u - 1) & (G - 1);
+        rd[k][u] = row[(int64_t)gx * 16];
+      }
+    }
+  };
+  auto store_dz = [&]() {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int o = 2 * dcp * W4_DCS + (2 * dq + k) * 32 + 8 * dvg;
+      *reinterpret_cast<u32x4*>(&dl[0][o]) = run_lo(rd[k], 1);
+      *reinterpret_cast<u32x4*>(&dl[0][o + W4_DCS]) = run_hi(rd[k], 1);
+      *reinterpret_cast<u32x4*>(&dl[1][o]) = run_lo(rd[k], 0);
+      *reinterpret_cast<u32x4*>(&dl[1][o + W4_DCS]) = run_hi(rd[k], 0);
+    }
+  };
+  // the MFMAs of one plane for this wave: x planes z-1, z, z+1 in ring slots
+  // (s0, s0+1, s0+2) mod 3; rows 2 wv and 2 wv + 1, two halves of 16 voxels
+  auto mfma_plane = [&](int s0) __attribute__((always_inline)) {
+    int xo[5];  // element offset of each group's fragment without the row and voxel terms
+#pragma unroll
+    for (int g = 0; g < 5; ++g) {
+      int sl = s0 + cdzo[g];
+      sl = sl >= 3 ? sl - 3 : sl;
+      xo[g] = ci * W4_XCS + sl * W4_SLOT + cdyo[g] * 32;
+    }
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int r = 2 * wv + (st >> 1), vb = (st & 1) * 16 + kh * 8;
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&dl[0][i * W4_DCS + r * 32 + vb]);
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&dl[1][i * W4_DCS + r * 32 + vb]);
+      accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, ones, accb, 0, 0, 0);
+#pragma unroll
+      for (int g = 0; g < 5; ++g) {
+        const uint16_t* src = cval[g] ? &xl[ccopy[g]][xo[g] + r * 32 + vb] : zero8;
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(src);
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g < 3 ? a0 : a1, bv, acc[g], 0, 0, 0);
+      }
+    }
+  };
+
+  const int64_t nwg = gridDim.x;
+  const int64_t s = (nwg % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nwg / 8) + blockIdx.x / 8 : (int64_t)blockIdx.x;
+  for (int64_t task = s; task < ntasks; task += nwg) {
+    int64_t t = task;
+    const int bx = (int)(t % ntx);
+    t /= ntx;
+    const int by = (int)(t % nty);
+    t /= nty;
+    const int bz = (int)(t % ntz);
+    cb = (int)(t / ntz);
+    cx0 = bx * 32;
+    cy0 = by * W4_Y;
+    const int z0 = bz * zc;
+    load_x(z0 - 1);
+    __syncthreads();  // the previous task's MFMAs are done with LDS (and zero8 is written)
+    store_x(0);
+    load_x(z0);
+    store_x(1);
+    load_x(z0 + 1);
+    load_dz(z0);
+    for (int zi = 0; zi < zc; ++zi) {
+      __syncthreads();  // the previous plane's MFMAs are done with slot (zi + 2) % 3 and the dz plane
+      store_x((zi + 2) % 3);
+      store_dz();
+      __syncthreads();
+      if (zi + 1 < zc) {
+        load_x(z0 + zi + 2);
+        load_dz(z0 + zi + 1);
+      }
+      mfma_plane(zi % 3);
+    }
+  }
+  float* pw = part + ((int64_t)blockIdx.x * 4 + wv) * W4_PART;
+#pragma unroll
+  for (int g = 0; g < 5; ++g) {
+    if (cval[g])
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int co = 8 * (q / 4) + 4 * kh + (q % 4);
+        pw[(ctap[g] * 32 + co) * 4 + ci] = acc[g][q];
+      }
+  }
+  if (i == 0)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) pw[27 * 32 * 4 + 8 * (q / 4) + 4 * kh + (q % 4)] = accb[q];
+}
+
 #ifndef AUGUR_WGRAD_Z
 #define AUGUR_WGRAD_Z 1
 #endif
@@ -1309,6 +1509,20 @@ __global__ void k_pack_cl8(const float* __restrict__ in, int C, int64_t B, int64
       for (int k = 0; k < 4; ++k) w[k] = pack_bf16x2(f[2 * k], f[2 * k + 1]);
       *reinterpret_cast<u32x4*>(&out[t * CP + g * 8]) = w;
     }
+  }
+}
+
+// the same for a 4-channel image: one voxel per thread, 8 bytes written
+__global__ void k_pack_cl4(const float* __restrict__ in, int C, int64_t B, int64_t G3, float scale,
+                           uint16_t* __restrict__ out) {
+  const int64_t nv = B * G3;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nv; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / G3, v = t - b * G3;
+    const float* src = in + b * C * G3 + v;
+    float f[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) f[c] = c < C ? src[(int64_t)c * G3] * scale : 0.0f;
+    *reinterpret_cast<uint2*>(&out[t * 4]) = uint2{pack_bf16x2(f[0], f[1]), pack_bf16x2(f[2], f[3])};
   }
 }
 
@@ -1599,21 +1813,29 @@ int augur_conv3_bf16_fwd(const uint16_t* x, int cin, const uint16_t* w, const fl
 size_t augur_conv3_wgrad_workspace_bytes(int cin, int64_t B, int64_t G, int nwg) {
   (void)B;
   (void)G;
-  return align_up((size_t)nwg * (27 * 32 * cin + 32) * sizeof(float)) + 256;
+  const size_t parts = cin == 4 ? (size_t)nwg * 4 : (size_t)nwg;  // cin 4: one partial per wave
+  return align_up(parts * (27 * 32 * cin + 32) * sizeof(float)) + 256;
 }
 
 int augur_conv3_bf16_wgrad(const uint16_t* x, int cin, const uint16_t* dz, int64_t B, int64_t G, float* dw, float* db,
                            int nwg, void* ws, size_t ws_bytes, void* stream) {
   AUGUR_REQUIRE(conv_tiles_ok(G), AUGUR_EUNSUPPORTED, "conv: G must be a power of two in [32, 1024]");
-  AUGUR_REQUIRE(cin == 16 || cin == 32, AUGUR_EUNSUPPORTED, "conv wgrad: cin must be 16 or 32");
+  AUGUR_REQUIRE(cin == 4 || cin == 16 || cin == 32, AUGUR_EUNSUPPORTED, "conv wgrad: cin must be 4, 16 or 32");
   AUGUR_REQUIRE(B >= 1 && x && dz && dw && db && nwg >= 1 && nwg <= 65536, AUGUR_EINVAL, "conv wgrad: bad arguments");
   const int64_t m = 27 * 32 * cin + 32;
-  AUGUR_REQUIRE(ws && ws_bytes >= (size_t)nwg * m * sizeof(float), AUGUR_EWORKSPACE, "conv wgrad: workspace too small");
+  const int nparts = cin == 4 ? 4 * nwg : nwg;
+  AUGUR_REQUIRE(ws && ws_bytes >= (size_t)nparts * m * sizeof(float), AUGUR_EWORKSPACE,
+                "conv wgrad: workspace too small");
   hipStream_t s = as_stream(stream);
   const int64_t ntiles = B * (G / CT_X) * (G / CT_Y) * (G / CT_Z);
   {
     TimedScope ts_("conv_wgrad", s);
-    if (cin == 16)
+    if (cin == 4) {
+      const int zc = (int)(G < WZ_ZC ? G : WZ_ZC);
+      const int64_t ntasks = B * (G / 32) * (G / W4_Y) * (G / zc);
+      hipLaunchKernelGGL(k_conv3_wgrad_in4, dim3(nwg), dim3(CONV_THREADS), 0, s, x, dz, (int)B, (int)G, zc, ntasks,
+                         (float*)ws);
+    } else if (cin == 16)
       hipLaunchKernelGGL((k_conv3_wgrad<16>), dim3(nwg), dim3(CONV_THREADS), 0, s, x, dz, (int)B, (int)G, ntiles,
                          (float*)ws);
     else if (AUGUR_WGRAD_Z) {
@@ -1626,7 +1848,7 @@ int augur_conv3_bf16_wgrad(const uint16_t* x, int cin, const uint16_t* dz, int64
                          (float*)ws);
     AUGUR_LAUNCHED("conv_wgrad");
   }
-  hipLaunchKernelGGL(k_sum_partials, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const float*)ws, nwg, m,
+  hipLaunchKernelGGL(k_sum_partials, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const float*)ws, nparts, m,
                      (int64_t)27 * 32 * cin, dw, db);
   AUGUR_LAUNCHED("sum_partials");
   return AUGUR_OK;
@@ -1659,6 +1881,9 @@ int augur_pack_cl_bf16(const float* in, int C, int64_t B, int64_t G, int CP, flo
   TimedScope ts_("pack_cl", as_stream(stream));
   if (CP % 8 == 0)
     hipLaunchKernelGGL(k_pack_cl8, dim3(blocks_for(B * G3, 256)), dim3(256), 0, as_stream(stream), in, C, B, G3, CP,
+                       scale, out);
+  else if (CP == 4)
+    hipLaunchKernelGGL(k_pack_cl4, dim3(blocks_for(B * G3, 256)), dim3(256), 0, as_stream(stream), in, C, B, G3,
                        scale, out);
   else
     hipLaunchKernelGGL(k_pack_cl, dim3(blocks_for(B * G3 * CP, 256)), dim3(256), 0, as_stream(stream), in, C, B, G3,

@@ -135,6 +135,8 @@ class LocalNet3D:
         assert width == 32, "the MFMA kernels are built for width 32"
         self.cin, self.cout, self.layers = cin, cout, layers
         self.cinp = pad16(cin)
+        # channels of the packed input image: 4 (the packed-tap input layer) for up to 4 inputs
+        self.cpack = 4 if cin <= 4 else self.cinp
         self.device = torch.device(device)
         g = torch.Generator().manual_seed(seed)
         shapes = [(cin, 32)] + [(32, 32)] * (layers - 1) + [(32, cout)]
@@ -223,16 +225,13 @@ class LocalNet3D:
         self.tuner = tuner if tuner is not None else (self.tuner or ConvTuner(reps=reps))
         dev = self.w[0].device
         g = torch.Generator(device=dev).manual_seed(seed)
-        x = torch.randn(B, G, G, G, self.cinp, device=dev, generator=g).to(BF16)
+        x = torch.randn(B, G, G, G, self.cpack, device=dev, generator=g).to(BF16)
         self._tuning = True
         try:
             self.forward_packed(x, save=backward)
             if backward:
                 dy = 1e-3 * torch.randn(B, self.cout, G, G, G, device=dev, generator=g)
                 self.backward(dy)
-            if self.wk_in4 is not None:   # the 4-channel input layer of the inference path
-                x4 = torch.randn(B, G, G, G, 4, device=dev, generator=g).to(BF16)
-                self._conv(x4, self.wk_in4, self.b[0], MODE_GELU, 32)
         finally:
             self._tuning = False
             self._saved = None
@@ -245,13 +244,15 @@ class LocalNet3D:
 
     def forward(self, x, save=False):
         """x: FP32 (B, cin, G, G, G) -> FP32 (B, cout, G, G, G)."""
-        return self.forward_packed(pack_cl(x, self.cinp), save=save)
+        return self.forward_packed(pack_cl(x, self.cpack), save=save)
 
     def forward_packed(self, xc, save=False):
-        """xc: BF16 channels-last (B, G, G, G, cinp) input -> FP32 (B, cout, G, G, G).
-        A 4-channel input (inference only, cin <= 4) runs the input layer with
-        the taps packed into the MFMA K dimension (k_conv3_in4); training
-        (save=True) needs the cinp-channel image, whose weight gradient it reads."""
+        """xc: BF16 channels-last (B, G, G, G, cpack) input -> FP32 (B, cout, G, G, G).
+        With cin <= 4 the image has 4 channels and the input layer packs 4
+        taps x 4 channels into each MFMA's K dimension (k_conv3_in4); its
+        weight gradient runs on the same 4 channels (cin 4 of
+        augur_conv3_bf16_wgrad).  A cinp-channel image (zero padded) is also
+        accepted."""
         h = self._hidden(xc, save)
         y = self._conv(h, self.wk[-1], self.b[-1], MODE_OUT_CF, self.cout)
         return y
@@ -276,9 +277,8 @@ class LocalNet3D:
         return field
 
     def _hidden(self, xc, save):
-        if save and xc.shape[-1] == 4 and self.cinp != 4:
-            raise ValueError("the 4-channel input layer is inference-only: train on the "
-                             f"{self.cinp}-channel image (pack_cl)")
+        if xc.shape[-1] == 4 and self.wk_in4 is None:
+            raise ValueError(f"a 4-channel input image needs cin <= 4 (this net has cin = {self.cin})")
         zs, hs = [], [xc]
         need_z = save
         z = torch.empty(xc.shape[:-1] + (32,), dtype=BF16, device=xc.device) if need_z else None
@@ -601,7 +601,7 @@ class PMSurrogate:
                 if corr is not None:
                     _lib.call("augur_field_add", _p(force), _p(corr), 1.0, corr.numel(), _p(gate), stream)
             else:
-                xc = pack_cl(self.features_from_grids(rho, force), self.net.cinp)
+                xc = pack_cl(self.features_from_grids(rho, force), self.net.cpack)
                 self.net.forward_into(xc, force, self.force_scale, accumulate=False, skip=gate)
             if gate is not None:
                 _lib.call("augur_sur_gate", 2, _p(force), _p(st.pos), _p(probe), P, G, self.L, _p(ref), _p(gate),

@@ -470,10 +470,11 @@ int augur_conv3_bf16_fwd_into(const uint16_t* x, int cin, const uint16_t* w, con
 int augur_field_add(float* y, const float* x, float alpha, int64_t n, const uint32_t* skip, void* stream);
 
 /* Weight gradient dw[27][32][cin] = sum_v dz[v][co] x[v + o_tap][ci] and
- * db[32] = sum_v dz[v][co] (FP32), dz BF16 channels-last [B][G^3][32].
- * nwg persistent workgroups each write a partial that is summed in a fixed
- * order (run-to-run identical).  Replaces autograd's conv weight gradient
- * (scripts/train_scatter_field.py:225 loss.backward()). */
+ * db[32] = sum_v dz[v][co] (FP32), dz BF16 channels-last [B][G^3][32],
+ * cin 4 (the input layer on its 4 real channels), 16 or 32.  nwg persistent
+ * workgroups each write a partial (cin 4: one per wave) that is summed in a
+ * fixed order (run-to-run identical).  Replaces autograd's conv weight
+ * gradient (scripts/train_scatter_field.py:225 loss.backward()). */
 size_t augur_conv3_wgrad_workspace_bytes(int cin, int64_t B, int64_t G, int nwg);
 int augur_conv3_bf16_wgrad(const uint16_t* x, int cin, const uint16_t* dz, int64_t B, int64_t G, float* dw, float* db,
                            int nwg, void* ws, size_t ws_bytes, void* stream);

@@ -86,11 +86,12 @@ def test_conv_data_gradient_is_transposed_conv(dev):
     assert (dx - x.grad).abs().max() < 1e-4 * x.grad.abs().max()
 
 
-@pytest.mark.parametrize("cin", [16, 32])
-def test_conv_weight_gradient(dev, cin):
+@pytest.mark.parametrize("cin,B,G", [(4, 2, 32), (4, 1, 64), (16, 2, 32), (32, 2, 32)])
+def test_conv_weight_gradient(dev, cin, B, G):
+    """Weight and bias gradients against float64 autograd; cin 4 is the input
+    layer's own kernel (columns = tap x channel pairs, one partial per wave)."""
     from augur_amd import surrogate as S
-    g = torch.Generator().manual_seed(11 + cin)
-    B, G = 2, 32
+    g = torch.Generator().manual_seed(11 + cin + G)
     x = bf16_round(torch.randn(B, cin, G, G, G, generator=g))
     dz = bf16_round(torch.randn(B, 32, G, G, G, generator=g))
     wt = torch.zeros(32, cin, 3, 3, 3, dtype=torch.float64, requires_grad=True)
@@ -226,12 +227,13 @@ def test_autotuned_localnet_matches_default_tiles(dev, tmp_path):
     g_def = net.backward(dy)
     tuner = net.autotune(B, G, reps=2)
     rep = tuner.report()
-    # 7 forward calls: conv_in, 4 residual layers with one shape, conv_out; backward: the output
-    # layer's data gradient, the residual layers' (with addend), the last of them writing dz only;
-    # the 4-channel input layer of the inference path; two weight-gradient shapes
+    # 6 forward calls: conv_in (the 4-channel packed-tap layer), 4 residual layers with one shape,
+    # conv_out; backward: the output layer's data gradient, the residual layers' (with addend), the
+    # last of them writing dz only; two weight-gradient shapes (4 and 32 input channels)
     fwd = [k for k in rep if k.startswith("fwd:")]
     wg = [k for k in rep if k.startswith("wgrad:")]
-    assert len(fwd) == 7 and len(wg) == 2, sorted(rep)
+    assert len(fwd) == 6 and len(wg) == 2, sorted(rep)
+    assert any(k.startswith("fwd:cin4:") for k in fwd) and any(k.startswith("wgrad:cin4:") for k in wg)
     assert any(k.endswith(":noy") for k in fwd)
     for k, v in rep.items():
         assert v["best"] in v["ms"] and min(v["ms"].values()) == v["ms"][v["best"]], (k, v)
@@ -245,6 +247,35 @@ def test_autotuned_localnet_matches_default_tiles(dev, tmp_path):
     tuner.save(path)
     t2 = A.ConvTuner(path=path)
     assert t2.table == tuner.table
+
+
+def test_four_channel_training_path_matches_padded_image(dev):
+    """Training on the 4-channel image (packed-tap input layer, 4-channel weight
+    gradient) and on the zero-padded 16-channel image give the same loss
+    gradients within BF16 rounding of the activations."""
+    from augur_amd import surrogate as S
+    B, G = 2, 32
+    net = S.LocalNet3D(cin=4, cout=3, device=dev, init_std=0.05)
+    g = torch.Generator(device=dev).manual_seed(17)
+    x = torch.randn(B, 4, G, G, G, device=dev, generator=g)
+    dy = torch.randn(B, 3, G, G, G, device=dev, generator=g)
+    assert net.cpack == 4
+    assert torch.equal(S.pack_cl(x, 4), S.pack_cl(x, 16)[..., :4])   # the 4-channel pack kernel
+    y4 = net.forward_packed(S.pack_cl(x, 4), save=True)
+    g4 = net.backward(dy)
+    y16 = net.forward_packed(S.pack_cl(x, 16), save=True)
+    g16 = net.backward(dy)
+    assert (y4 - y16).abs().max() <= 1e-2 * y16.abs().max()
+    for (dw4, db4), (dw16, db16) in zip(g4, g16):
+        assert dw4.shape == dw16.shape and db4.shape == db16.shape
+        assert (dw4 - dw16).abs().max() <= 2e-2 * dw16.abs().max() + 1e-6
+        assert (db4 - db16).abs().max() <= 2e-2 * db16.abs().max() + 1e-6
+    # the input layer's weight gradient itself: same saved activations' gradient, both kernels
+    dz = torch.randn(B, G, G, G, 32, device=dev, generator=g).to(torch.bfloat16)
+    dwa, dba = S.conv3_wgrad(S.pack_cl(x, 4), dz)
+    dwb, dbb = S.conv3_wgrad(S.pack_cl(x, 16), dz)
+    assert (dwa - dwb[:, :, :4]).abs().max() <= 1e-5 * dwb.abs().max()
+    assert (dba - dbb).abs().max() <= 1e-5 * dbb.abs().max()
 
 
 @pytest.mark.parametrize("with_add", [False, True])
