@@ -2,9 +2,11 @@
 
 * the Plummer cold collapse (N = 65,536, 64^3, 100 KDK steps of dt = 1e-3)
   against the FP64 oracle's rollout;
-* the surrogate network's conv tiles at 256^3 (every family and every zc up to
-  128, the tiles the autotuner picks there), each against the float64 conv on
-  slabs of output planes, bits equal across zc within a family;
+* the clumpy N = 2^24, 256^3 scene, ten KDK steps of dt = 5e-4, against the
+  same steps of the pinned PyTorch-CPU port;
+* the surrogate network's conv tiles at 256^3 and 512^3 (every family and
+  every zc up to 128, the tiles the autotuner picks there), each against the
+  float64 conv on slabs of output planes, bits equal across zc within a family;
 * the surrogate on that Plummer scene's 256^3 grid: the network step against
   its unfused form, and the per-step fallback to plain PM at the 2e-2
   residual bound.
@@ -60,6 +62,40 @@ def test_plummer_cold_collapse_100_steps(dev):
     assert np.abs(v.sum(0)).max() / n < 1e-5 * np.sqrt((V ** 2).sum(1).mean())
 
 
+def test_cfg2_shape_ten_kdk_steps(dev):
+    """cfg2 end to end: N = 2^24 (50% uniform, 64 Gaussian clumps), 256^3, ten KDK
+    steps of dt = 5e-4 through the stepper, against the same steps of the
+    PyTorch-CPU FP32 port (pinned against the FP64 oracle's force and step);
+    the two differ only by FP32 summation order in the deposit (measured: max
+    |dx| 1.2e-7, velocity rel-RMS 1.9e-7).  The fused FP64 momentum sums drift
+    by 2e-11 against a mean |v| of 2.4e-2."""
+    from augur_amd import scenes
+    from augur_amd.pm import PeriodicPM
+    from oracle import torch_cpu as tc
+    pos, vel = scenes.cfg2()
+    n, G, dt, steps = len(pos), 256, 5e-4, 10
+    pm = PeriodicPM(G, 1.0, 1.0, dev)
+    st = pm.load(torch.from_numpy(pos).to(dev), torch.from_numpy(vel).to(dev), mass_value=1.0 / n)
+    diag = pm.kdk(st, dt, steps, diag=True).cpu().numpy()
+    p, v, _ = (t.cpu().numpy() for t in pm.unload(st))
+    P, V = torch.from_numpy(pos), torch.from_numpy(vel)
+    m = torch.full((n,), 1.0 / n)
+    A = tc.pm_acc(P, m, G)
+    for _ in range(steps):
+        P, V, A = tc.kdk_step(P, V, A, m, G, dt)
+    P, V = P.numpy(), V.numpy()
+    dx = _wrapdiff(p, P).max()
+    relv = np.sqrt(((v.astype(np.float64) - V) ** 2).sum() / (V.astype(np.float64) ** 2).sum())
+    print(f"cfg2 after {steps} steps: max |dx| {dx:.2e}, velocity rel-RMS {relv:.2e}", flush=True)
+    assert dx < 1e-6 and relv < 2e-6
+    # momentum: sum m v over all particles after each step, against sum m |v|
+    pmag = (np.abs(vel.astype(np.float64)).sum() / n)
+    drift = np.abs(diag[:, :3] - diag[0, :3]).max()
+    print(f"momentum drift over the steps {drift:.2e} (mean |v| {pmag:.2e})", flush=True)
+    assert drift < 1e-8 * pmag
+    assert np.isfinite(diag).all() and (diag[:, 3] > 0).all()
+
+
 # ------------------------------------------------------------ conv at 256^3
 def _slab_ref(x_cl, w_taps, bias, z):
     """float64 conv of output plane z from input planes z-1..z+1 (periodic in
@@ -107,6 +143,45 @@ def test_conv_tiles_at_256(dev, cin, mode, cout):
         else:
             first[fam] = y.clone()
         del y
+
+
+@pytest.mark.parametrize("cin,mode,cout", [(4, 1, 32), (32, 2, 32), (32, 0, 3)])
+def test_conv_tiles_at_512(dev, cin, mode, cout):
+    """The large workload's 512^3 grid (the surrogate leg of N = 2^30): the
+    input, residual and output layers on every candidate tile, two planes each
+    against the float64 conv, the checked planes bitwise equal across zc within
+    a family."""
+    from augur_amd import autotune as A
+    from augur_amd import surrogate as S
+    B, G = 1, 512
+    g = torch.Generator(device=dev).manual_seed(11 * cin + mode)
+    x = torch.randn(B, G, G, G, cin, device=dev, generator=g).to(torch.bfloat16)
+    w = (0.1 * torch.randn(27, 32, cin, device=dev, generator=g)).to(torch.bfloat16).float()
+    w[:, cout:] = 0
+    b = 0.1 * torch.randn(32, device=dev, generator=g)
+    wk = S.kernel_weights(w, cin)
+    planes = (0, 301)
+    refs = {}
+    for z in planes:
+        r = _slab_ref(x, w, b, z)[:cout]
+        if mode in (1, 2):
+            r = F.gelu(r, approximate="tanh")
+            if mode == 2:
+                r = r + x[0, z].double().cpu().permute(2, 0, 1)
+        refs[z] = r
+    first = {}
+    for tile in A.candidates(cin, mode, cout, G):
+        y = S.conv3_fwd(x, wk, b, mode, cout, tile=tile)
+        got = {z: (y[0, :, z] if mode == 0 else y[0, z].permute(2, 0, 1)).double().cpu() for z in planes}
+        del y
+        for z in planes:
+            tol = (1e-4 if mode == 0 else 1.5e-2) * refs[z].abs().max()
+            assert (got[z] - refs[z]).abs().max() < tol, (tile, z)
+        fam = tile[0]
+        if fam in first:
+            assert all(torch.equal(got[z], first[fam][z]) for z in planes), tile
+        else:
+            first[fam] = got
 
 
 def test_conv_into_field_at_256(dev):
