@@ -351,34 +351,29 @@ __device__ __forceinline__ void tile_items_body(uint32_t item, const uint32_t* _
   }
 }
 
-// One axis of a visit's CIC corners relative to the tile: the base corner's
-// tile offset a (wrapped into [0, G)) and the +1 corner's c1.  A corner
-// outside the tile gets weight 0 and the other corner's address, so every
-// corner address of a contributing visit lies in the tile and the 8 LDS
-// atomics need no trash cells.  lo: address of the base corner, d: step to
-// the +1 corner (0 when one of them is outside; 1 - G when the tile spans the
-// axis and the +1 corner wraps to 0).
-struct TdAxis {
-  int lo, d;
-  bool in0, in1;
-};
-__device__ __forceinline__ TdAxis td_axis(int a, int T, int G) {
-  const int c1 = (a + 1) & (G - 1);
-  TdAxis r;
-  r.in0 = a < T;
-  r.in1 = c1 < T;
-  r.lo = r.in0 ? a : c1;
-  r.d = r.in1 ? c1 - r.lo : 0;
-  return r;
-}
-
-// adds a run's 8 corner sums: base address (bytes) plus the corner steps
-__device__ __forceinline__ void td_flush(double* acc, const double (&r)[8], int base, int dx, int dy, int dz) {
-  char* b = reinterpret_cast<char*>(acc) + base;
+// adds a lane's run sums (base cell at tile offsets in key) into the LDS tile.
+// Branch-free: a corner outside the tile, and every corner when key < 0 (no
+// run), is redirected to the lane's own trash cell past the tile (never
+// read), so every lane issues the same 8 atomics.  Byte offsets: an axis
+// term outside the tile is TD_OFF, which lifts the sum past every trash
+// cell, and min(sum, trash) then picks the trash cell.
+constexpr int TD_TRASH = 64;  // one trash cell per lane of a wave
+constexpr int TD_OFF = 1 << 20;
+__device__ __forceinline__ void td_flush(double* acc, const double* a, int key, int G, int lane) {
+  const int ax = key & 1023, ay = (key >> 10) & 1023, az = (key >> 20) & 1023;
+  const int x1 = (ax + 1) & (G - 1), y1 = (ay + 1) & (G - 1), z1 = (az + 1) & (G - 1);
+  const int cx[2] = {ax < TD_X ? ax * 8 : TD_OFF, x1 < TD_X ? x1 * 8 : TD_OFF};
+  const int cy[2] = {ay < TD_Y ? ay * (TD_X * 8) : TD_OFF, y1 < TD_Y ? y1 * (TD_X * 8) : TD_OFF};
+  const bool ok = key >= 0;
+  const int cz[2] = {ok && az < TD_Z ? az * (TD_X * TD_Y * 8) : TD_OFF,
+                     ok && z1 < TD_Z ? z1 * (TD_X * TD_Y * 8) : TD_OFF};
+  const int trash = (TD_CELLS + lane) * 8;
+  char* base = reinterpret_cast<char*>(acc);
 #pragma unroll
-  for (int o = 0; o < 8; ++o)
-    atomicAdd(reinterpret_cast<double*>(b + ((o & 1) ? dx : 0) + (((o >> 1) & 1) ? dy : 0) + ((o >> 2) ? dz : 0)),
-              r[o]);
+  for (int o = 0; o < 8; ++o) {
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+    atomicAdd(reinterpret_cast<double*>(base + min(cz[oz] + cy[oy] + cx[ox], trash)), a[o]);
+  }
 }
 
 // perm (may be NULL): particle index of sorted slot i; otherwise slot i is particle i.
@@ -392,7 +387,7 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
     unsigned long long* __restrict__ fix, float* __restrict__ rho) {
   // unit counts of the fixed-point contributions: integers below 2^53 in FP64,
   // so the LDS float atomics and register sums are exact in any order
-  __shared__ double acc[TD_CELLS];
+  __shared__ double acc[TD_CELLS + TD_TRASH];
   __shared__ uint32_t seg_lo[TD_MAXSEG];
   __shared__ uint32_t seg_pre[TD_MAXSEG + 1];
   __shared__ uint16_t blk_seg[TD_NBLK];
@@ -469,14 +464,13 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
 #pragma unroll
           for (int u = 0; u < TD_K; ++u) pm[u] = mass[sl[u]];
         }
-        // the current run: base cell key, corner sums, LDS base address and steps
-        int rkey = -1, rb = 0, rdx = 0, rdy = 0, rdz = 0;
+        int rkey = -1;
         double run[8];
 #pragma unroll
         for (int o = 0; o < 8; ++o) run[o] = 0;
-        // a visit outside the grown tile's contributing cells (or past the
-        // lane's range) keeps the current run and adds zero units; the
-        // weights and units are only formed for contributing visits
+        // straight-line per visit: a visit outside the grown tile (or past
+        // the lane's range) keeps the current run and adds zero units, so
+        // only the flush on a change of base cell is a branch
 #pragma unroll
         for (int u = 0; u < TD_K; ++u) {
           int i0[3];
@@ -488,40 +482,33 @@ __global__ __launch_bounds__(TD_THREADS) void k_dep_tiles(
             i0[d] = (int)f;
             wv[d] = __fsub_rn(uu, f);
           }
-          // base cell as offsets from the tile origin (wrapped into [0, G))
+          // base cell as offsets from the tile origin (wrapped into [0, G));
+          // a corner is inside when its offset is below the tile size
           const int ax = (i0[0] - t.x0) & (G - 1), ay = (i0[1] - t.y0) & (G - 1), az = (i0[2] - t.z0) & (G - 1);
-          const TdAxis X = td_axis(ax, TD_X, G), Y = td_axis(ay, TD_Y, G), Z = td_axis(az, TD_Z, G);
-          const bool in = vok[u] & (X.in0 | X.in1) & (Y.in0 | Y.in1) & (Z.in0 | Z.in1);
+          const bool in = vok[u] & ((ax < TD_X) | (ax == G - 1)) & ((ay < TD_Y) | (ay == G - 1)) &
+                          ((az < TD_Z) | (az == G - 1));
           const int key = in ? (az << 20) | (ay << 10) | ax : rkey;
-          // a non-contributing visit has key == rkey: no flush, run unchanged
-          const bool same = key == rkey;
-          if (!same && rkey >= 0) td_flush(acc, run, rb, rdx, rdy, rdz);
-          if (in) {
-            // units rint(m * scale * w * 2^E), the power-of-two scaling applied to m * scale
-            const float valE = mass ? ldexpf(__fmul_rn(pm[u], scale), E) : valE0;
-            const float w0[3] = {X.in0 ? __fsub_rn(1.0f, wv[0]) : 0.0f, Y.in0 ? __fsub_rn(1.0f, wv[1]) : 0.0f,
-                                 Z.in0 ? __fsub_rn(1.0f, wv[2]) : 0.0f};
-            const float w1[3] = {X.in1 ? wv[0] : 0.0f, Y.in1 ? wv[1] : 0.0f, Z.in1 ? wv[2] : 0.0f};
-            // run * 1 + q or run * 0 + q: exact on integers below 2^53
-            const double keep = same ? 1.0 : 0.0;
+          // units rint(m * scale * w * 2^E), the power-of-two scaling applied to m * scale
+          const float valE = in ? (mass ? ldexpf(__fmul_rn(pm[u], scale), E) : valE0) : 0.0f;
+          double q[8];
 #pragma unroll
-            for (int o = 0; o < 8; ++o) {
-              const float wx = (o & 1) ? w1[0] : w0[0];
-              const float wy = ((o >> 1) & 1) ? w1[1] : w0[1];
-              const float wz = (o >> 2) ? w1[2] : w0[2];
-              const double q = (double)rintf(__fmul_rn(valE, __fmul_rn(__fmul_rn(wx, wy), wz)));
-              run[o] = fma(run[o], keep, q);
-            }
+          for (int o = 0; o < 8; ++o) {
+            const float wx = (o & 1) ? wv[0] : __fsub_rn(1.0f, wv[0]);
+            const float wy = ((o >> 1) & 1) ? wv[1] : __fsub_rn(1.0f, wv[1]);
+            const float wz = (o >> 2) ? wv[2] : __fsub_rn(1.0f, wv[2]);
+            q[o] = (double)rintf(__fmul_rn(valE, __fmul_rn(__fmul_rn(wx, wy), wz)));
           }
-          if (!same) {
-            rb = ((Z.lo * TD_Y + Y.lo) * TD_X + X.lo) * 8;
-            rdx = X.d * 8;
-            rdy = Y.d * (TD_X * 8);
-            rdz = Z.d * (TD_X * TD_Y * 8);
+          if (key != rkey) {
+            if (rkey >= 0) td_flush(acc, run, rkey, G, lane);
+#pragma unroll
+            for (int o = 0; o < 8; ++o) run[o] = q[o];
+            rkey = key;
+          } else {
+#pragma unroll
+            for (int o = 0; o < 8; ++o) run[o] += q[o];
           }
-          rkey = key;
         }
-        if (rkey >= 0) td_flush(acc, run, rb, rdx, rdy, rdz);
+        if (rkey >= 0) td_flush(acc, run, rkey, G, lane);
       }
     }
     __syncthreads();
