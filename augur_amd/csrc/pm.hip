@@ -211,7 +211,8 @@ constexpr uint32_t TD_CHUNK = 16384;  // most particle visits per work item
 #define AUGUR_TD_K 8
 #endif
 constexpr int TD_K = AUGUR_TD_K;      // consecutive visits per lane in a wave batch
-constexpr int TD_BLK = 64;            // visits per entry of the visit -> segment table
+constexpr int TD_BLK = TD_K;          // visits per entry of the visit -> segment table: one
+                                      // entry per lane batch, so no lane walks other lanes' segments
 constexpr int TD_NBLK = TD_CHUNK / TD_BLK;  // table entries per item
 
 struct TileOrigin {
