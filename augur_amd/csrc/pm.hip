@@ -245,9 +245,9 @@ __device__ __forceinline__ int box_runs(const Piece& px, const Piece& py, const 
   return (py.b - py.a) * (pz.b - pz.a);
 }
 
-// Sort-time plan, one wave per tile: the grown region's key runs as sorted-
-// slot starts and an inclusive prefix of their lengths (TD_MAXSEG each; the
-// unused tail has length 0), and the tile's number of work items of at most
+// Sort-time plan, one wave per tile: the grown region's non-empty key runs as
+// sorted-slot starts and an inclusive prefix of their lengths (TD_MAXSEG
+// entries each; the unused tail has length 0), and the tile's number of work items of at most
 // TD_CHUNK visits; items[ntiles] = 0 so an exclusive scan leaves the total there.
 // A batch of grids has scene-major tiles: tile = scene * tps + tile of the
 // scene (tps tiles per scene, keys scene * G^3 + tile_key), and a region
@@ -275,7 +275,7 @@ __device__ __forceinline__ void dep_plan_body(int bid, const uint32_t* __restric
   uint32_t* lo_t = seg_lo + (int64_t)tile * TD_MAXSEG;
   uint32_t* pre_t = seg_pre + (int64_t)tile * (TD_MAXSEG + 1);
   if (lane == 0) pre_t[0] = 0u;
-  uint32_t carry = 0;
+  uint32_t carry = 0, cnt = 0;  // visits and non-empty runs stored so far
   for (int r0 = 0; r0 < TD_MAXSEG; r0 += 64) {
     const int r = r0 + lane;
     uint32_t lo = 0, hi = 0;
@@ -303,16 +303,27 @@ __device__ __forceinline__ void dep_plan_body(int bid, const uint32_t* __restric
       hi = cell_start[k0 + len];
     }
     uint32_t v = hi - lo;
+    // empty runs are left out of the tables (stored runs are compacted), so a
+    // lane's walk over its visits never steps through an empty run
+    const uint64_t nonempty = __ballot(v != 0u);
+    const uint32_t at = cnt + (uint32_t)__popcll(nonempty & ((1ull << lane) - 1ull));
+    const bool keep = v != 0u;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t y = __shfl_up(v, o, 64);
       if (lane >= o) v += y;
     }
-    if (r < TD_MAXSEG) {
-      lo_t[r] = lo;
-      pre_t[r + 1] = v + carry;
+    if (keep) {
+      lo_t[at] = lo;
+      pre_t[at + 1] = v + carry;
     }
     carry += __shfl(v, 63, 64);
+    cnt += (uint32_t)__popcll(nonempty);
+  }
+  // the unused tail: runs of length 0 after the last visit
+  for (uint32_t r = cnt + lane; r < (uint32_t)TD_MAXSEG; r += 64) {
+    lo_t[r] = 0u;
+    pre_t[r + 1] = carry;
   }
   if (lane == 0) items[tile] = carry == 0u ? 1u : (carry + TD_CHUNK - 1) / TD_CHUNK;
 }
