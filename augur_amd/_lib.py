@@ -67,6 +67,8 @@ SIGNATURES = {
     "augur_pm_force_replay": (c_int, [c_int64, c_int64, c_float, P, P, c_size_t, P]),
     "augur_conv3_bf16_fwd_into": (c_int, [P, c_int, P, P, c_int64, c_int64, c_int, c_float, c_int, P, P, c_int, P]),
     "augur_persistent_step": (c_int, [P, P, P, P, c_int64, c_int64, P, c_int64, c_float, c_float, c_int, P]),
+    "augur_persistent_step_tracked": (c_int, [P, P, P, P, c_int64, c_int64, P, c_int64, c_float, c_float, c_int, P,
+                                              P]),
     "augur_conv3_bf16_fwd": (c_int, [P, c_int, P, P, c_int64, c_int64, c_int, c_int, P, P, P, P, P]),
     "augur_conv3_tile_variants": (c_int, [c_int, c_int, c_int]),
     "augur_conv3_bf16_fwd_tile": (c_int, [P, c_int, P, P, c_int64, c_int64, c_int, c_int, P, P, P, P, c_int, c_int,
@@ -97,6 +99,10 @@ SIGNATURES = {
     "augur_cic_deposit_ragged_tiled_workspace_bytes": (c_size_t, [c_int64, c_int64, c_int64]),
     "augur_cic_deposit_ragged_tiled": (c_int, [P, P, c_float, P, c_int64, c_int64, c_int64, c_float, c_float, P, P,
                                                c_size_t, P]),
+    "augur_cic_deposit_ragged_planned_workspace_bytes": (c_size_t, [c_int64, c_int64, c_int64]),
+    "augur_cic_deposit_ragged_planned": (c_int, [P, P, c_float, P, c_int64, c_int64, c_int64, c_float, c_float,
+                                                 c_float, c_int, P, P, c_size_t, P]),
+    "augur_ragged_plan_drift_word": (c_void_p, [P, c_size_t]),
     "augur_field_add": (c_int, [P, P, c_float, c_int64, P, P]),
     "augur_pm_solve_grids_workspace_bytes": (c_size_t, [c_int64, c_int64]),
     "augur_pm_solve_grids": (c_int, [P, P, c_int64, c_int64, c_float, c_float, P, c_size_t, P]),

@@ -30,9 +30,11 @@ from .surrogate import pack_cl
 PHASE_GATHER, PHASE_OPEN, PHASE_CLOSE, PHASE_CLOSE_OPEN = 0, 1, 2, 3
 
 
-def persistent_step(pos, vel, acc, starts, force, G, L, dt, phase):
+def persistent_step(pos, vel, acc, starts, force, G, L, dt, phase, plan=None):
     """One ragged launch (phases as augur_persistent_step) on (total, 3) FP32
-    tensors updated in place; force (B, 3, G, G, G) FP32 or None for OPEN."""
+    tensors updated in place; force (B, 3, G, G, G) FP32 or None for OPEN.
+    With a planned deposit's ``plan`` (ops.RaggedPlan) the launch also
+    records its drifts' largest |v| for that deposit's re-sort decision."""
     dev = _dev(pos)
     B = starts.shape[0] - 1
     for t in (pos, vel, acc, force):
@@ -41,8 +43,12 @@ def persistent_step(pos, vel, acc, starts, force, G, L, dt, phase):
     assert starts.dtype == torch.int64 and starts.is_contiguous()
     if force is not None:
         assert tuple(force.shape) == (B, 3, G, G, G), force.shape
-    _lib.call("augur_persistent_step", _p(pos), _p(vel), _p(acc), _p(starts), B, pos.shape[0], _p(force), int(G),
-              float(L), float(dt), int(phase), _stream(dev))
+    if plan is None:
+        _lib.call("augur_persistent_step", _p(pos), _p(vel), _p(acc), _p(starts), B, pos.shape[0], _p(force), int(G),
+                  float(L), float(dt), int(phase), _stream(dev))
+    else:
+        _lib.call("augur_persistent_step_tracked", _p(pos), _p(vel), _p(acc), _p(starts), B, pos.shape[0], _p(force),
+                  int(G), float(L), float(dt), int(phase), plan.drift_word, _stream(dev))
 
 
 class RaggedStepper:
@@ -59,23 +65,38 @@ class RaggedStepper:
         self.h = self.L / self.G
         self.device = torch.device(device)
         self.sur = surrogate
+        self._plan = None  # the planned deposit's workspace of the last batch size stepped
         if surrogate is not None:
             assert surrogate.mode == "correction" and surrogate.rho_scale is not None, \
                 "the surrogate must be a calibrated correction net"
             assert surrogate.G == self.G and surrogate.L == self.L
 
     # ------------------------------------------------------------ fields
-    def grids(self, pos, starts, mass):
-        """(rho (B, G, G, G), PM force (B, 3, G, G, G)) of every scene."""
+    def grids(self, pos, starts, mass, planned=None):
+        """(rho (B, G, G, G), PM force (B, 3, G, G, G)) of every scene.
+        planned: (plan, drift_dt, force_sort) to deposit through the planned
+        ragged deposit (the stepping loop), else a deposit sorted per call."""
         G = self.G
-        rho = ops.cic_deposit_ragged(pos, starts, G, self.h, 0.0, True, mass=mass)[:, 0]
+        if planned is None:
+            rho = ops.cic_deposit_ragged(pos, starts, G, self.h, 0.0, True, mass=mass)[:, 0]
+        else:
+            plan, drift_dt, force_sort = planned
+            rho = ops.cic_deposit_ragged_planned(pos, starts, G, self.h, plan, drift_dt, force_sort, mass=mass)[:, 0]
         return rho, ops.pm_solve_grids(rho, self.L, self.Gconst)
 
-    def force_field(self, pos, starts, mass, scene_density=None):
+    def plan(self, total, starts):
+        """The planned deposit's workspace for a batch of these sizes (kept
+        and reused while the sizes stay the same)."""
+        key = (starts.shape[0] - 1, self.G, int(total))
+        if self._plan is None or self._plan.key != key:
+            self._plan = ops.RaggedPlan(*key, self.device)
+        return self._plan
+
+    def force_field(self, pos, starts, mass, scene_density=None, planned=None):
         """(B, 3, G, G, G) force field of every scene: plain PM, plus the
         net's correction when a surrogate is attached (scene_density (B,):
         each scene's mean density, its mass / L^3)."""
-        rho, force = self.grids(pos, starts, mass)
+        rho, force = self.grids(pos, starts, mass, planned)
         if self.sur is None:
             return force
         sur = self.sur
@@ -125,16 +146,19 @@ class RaggedStepper:
     def kdk(self, pos, vel, acc, starts, mass, dt, steps, scene_density=None):
         """`steps` KDK steps of the whole batch in place (acc must hold the
         force at the incoming positions; it holds the force at the outgoing
-        ones on return).  No host synchronisation."""
+        ones on return).  No host synchronisation.  The deposit keeps its
+        sort and plans across the steps of the call (sorted on the first
+        step, re-sorted on the device when the drift may pass a cell)."""
         if steps < 1:
             return
         if self.sur is not None and scene_density is None:
             scene_density = self.scene_density(starts, mass, pos.shape[0])
-        persistent_step(pos, vel, acc, starts, None, self.G, self.L, dt, PHASE_OPEN)
+        plan = self.plan(pos.shape[0], starts)
+        persistent_step(pos, vel, acc, starts, None, self.G, self.L, dt, PHASE_OPEN, plan=plan)
         for s in range(steps):
-            f = self.force_field(pos, starts, mass, scene_density)
+            f = self.force_field(pos, starts, mass, scene_density, planned=(plan, dt, s == 0))
             persistent_step(pos, vel, acc, starts, f, self.G, self.L, dt,
-                            PHASE_CLOSE if s == steps - 1 else PHASE_CLOSE_OPEN)
+                            PHASE_CLOSE if s == steps - 1 else PHASE_CLOSE_OPEN, plan=plan)
 
     def capture(self, pos, vel, acc, starts, mass, dt, steps):
         """A HIP graph of kdk(...) on these tensors; ``replay()`` advances them

@@ -41,8 +41,9 @@ __global__ __launch_bounds__(RG_THREADS) void k_step_ragged(float* __restrict__ 
                                                             float* __restrict__ acc, const float* __restrict__ force,
                                                             const int64_t* __restrict__ starts, int B, int64_t total,
                                                             int G, float h, float inv_h, float hdt, float dt, float L,
-                                                            int phase) {
+                                                            int phase, uint32_t* __restrict__ vmax_bits) {
   __shared__ int64_t s_start[RG_MAX_SCENES + 1];
+  float vmax = 0.0f;  // largest |v| component of this thread's drifts
   for (int b = threadIdx.x; b <= B; b += RG_THREADS) s_start[b] = starts[b];
   __syncthreads();
   const int64_t G3 = (int64_t)G * G * G;
@@ -60,6 +61,7 @@ __global__ __launch_bounds__(RG_THREADS) void k_step_ragged(float* __restrict__ 
         const float vv = __fmaf_rn(hdt, acc[3 * i + d], v[d]);
         vel[3 * i + d] = vv;
         pos[3 * i + d] = wrap_box(__fmaf_rn(dt, vv, p[d]), L);
+        vmax = fmaxf(vmax, fabsf(vv));
       }
       continue;
     }
@@ -114,9 +116,16 @@ __global__ __launch_bounds__(RG_THREADS) void k_step_ragged(float* __restrict__ 
       } else {
         vv = __fmaf_rn(hdt, a[d], vv);
         pos[3 * i + d] = wrap_box(__fmaf_rn(dt, vv, p[d]), L);
+        vmax = fmaxf(vmax, fabsf(vv));
       }
       vel[3 * i + d] = vv;
     }
+  }
+  // the drift bound of the tracked caller (a planned ragged deposit): one
+  // atomic max per wave on the float bits (non-negative, so ordered as uints)
+  if (vmax_bits) {
+    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+    if ((threadIdx.x & 63) == 0 && vmax > 0.0f) atomicMax(vmax_bits, __float_as_uint(vmax));
   }
 }
 
@@ -124,8 +133,9 @@ __global__ __launch_bounds__(RG_THREADS) void k_step_ragged(float* __restrict__ 
 
 extern "C" {
 
-int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B, int64_t total,
-                          const float* force, int64_t G, float L, float dt, int phase, void* stream) {
+static int persistent_step(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B, int64_t total,
+                           const float* force, int64_t G, float L, float dt, int phase, uint32_t* vmax_bits,
+                           void* stream) {
   AUGUR_REQUIRE(B >= 1 && B <= RG_MAX_SCENES, AUGUR_EINVAL, "persistent_step: B must be in [1, 1024]");
   AUGUR_REQUIRE(total >= 0 && starts && pos, AUGUR_EINVAL, "persistent_step: bad arguments");
   AUGUR_REQUIRE(is_pow2(G) && G >= 2 && G <= 1024, AUGUR_EUNSUPPORTED,
@@ -147,12 +157,24 @@ int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* sta
   TimedScope ts_("step_ragged", s);
   if (p2)
     hipLaunchKernelGGL((k_step_ragged<true>), dim3(blocks), dim3(RG_THREADS), 0, s, pos, vel, acc, force, starts,
-                       (int)B, total, (int)G, h, inv_h, 0.5f * dt, dt, L, phase);
+                       (int)B, total, (int)G, h, inv_h, 0.5f * dt, dt, L, phase, vmax_bits);
   else
     hipLaunchKernelGGL((k_step_ragged<false>), dim3(blocks), dim3(RG_THREADS), 0, s, pos, vel, acc, force, starts,
-                       (int)B, total, (int)G, h, inv_h, 0.5f * dt, dt, L, phase);
+                       (int)B, total, (int)G, h, inv_h, 0.5f * dt, dt, L, phase, vmax_bits);
   AUGUR_LAUNCHED("step_ragged");
   return AUGUR_OK;
+}
+
+int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B, int64_t total,
+                          const float* force, int64_t G, float L, float dt, int phase, void* stream) {
+  return persistent_step(pos, vel, acc, starts, B, total, force, G, L, dt, phase, nullptr, stream);
+}
+
+int augur_persistent_step_tracked(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B,
+                                  int64_t total, const float* force, int64_t G, float L, float dt, int phase,
+                                  uint32_t* drift_word, void* stream) {
+  AUGUR_REQUIRE(drift_word, AUGUR_EINVAL, "persistent_step_tracked: null drift word");
+  return persistent_step(pos, vel, acc, starts, B, total, force, G, L, dt, phase, drift_word, stream);
 }
 
 }  // extern "C"

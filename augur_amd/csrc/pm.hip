@@ -1207,7 +1207,8 @@ __global__ __launch_bounds__(DIAG_THREADS) void k_diag_finish(const double* __re
 // order is scene-major tile-major and its tiles are scene * tps + tile.
 __global__ void k_ragged_keys(const float* __restrict__ pos, const int64_t* __restrict__ starts, int B, int64_t n,
                               int G, float h, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                              uint32_t* __restrict__ ngaps) {
+                              uint32_t* __restrict__ ngaps, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) *ngaps = 0u;  // for k_cell_starts, later in the same call
   const uint32_t G3 = (uint32_t)G * G * G;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1226,20 +1227,23 @@ __global__ void k_ragged_keys(const float* __restrict__ pos, const int64_t* __re
 
 __global__ __launch_bounds__(256) void k_dep_plans(const uint32_t* __restrict__ cell_start, int G, int ntiles, int tps,
                                                    uint32_t* __restrict__ items, uint32_t* __restrict__ seg_lo,
-                                                   uint32_t* __restrict__ seg_pre) {
+                                                   uint32_t* __restrict__ seg_pre, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   dep_plan_body((int)blockIdx.x, cell_start, G, ntiles, tps, items, seg_lo, seg_pre);
 }
 
 __global__ __launch_bounds__(256) void k_dep_items(const uint32_t* __restrict__ item_start, int ntiles,
                                                    const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
-                                                   uint16_t* __restrict__ blk) {
+                                                   uint16_t* __restrict__ blk, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   tile_items_body<TD_MAXSEG, TD_CHUNK>(blockIdx.x, item_start, ntiles, tseg_pre, info, blk);
 }
 
 // the int64 images of the heavy tiles (more than one item) zeroed before
 // their items add into them (a transient workspace is not kept zero)
 __global__ __launch_bounds__(256) void k_dep_zero_heavy(const uint32_t* __restrict__ item_start,
-                                                        unsigned long long* __restrict__ fix) {
+                                                        unsigned long long* __restrict__ fix, const uint32_t* gate) {
+  if (gate && *gate == 0u) return;
   const int tile = blockIdx.x;
   if (item_start[tile + 1] - item_start[tile] <= 1u) return;
   unsigned long long* img = fix + (int64_t)tile * TD_CELLS;
@@ -1910,52 +1914,45 @@ size_t augur_cic_deposit_ragged_tiled_workspace_bytes(int64_t B, int64_t G, int6
   return c.used + 4096;
 }
 
-int augur_cic_deposit_ragged_tiled(const float* pos, const float* mass, float mass_value, const int64_t* starts,
-                                   int64_t B, int64_t total, int64_t G, float h, float scale, float* out, void* ws,
-                                   size_t ws_bytes, void* stream) {
-  AUGUR_REQUIRE(is_pow2(G) && G >= 32 && G <= 512, AUGUR_EUNSUPPORTED,
-                "cic_deposit_ragged_tiled: G must be a power of two in [32, 512]");
-  AUGUR_REQUIRE(B >= 1 && B <= 65535 && total >= 0 && total < ((int64_t)1 << 31) && h > 0.0f && starts && out,
-                AUGUR_EINVAL, "cic_deposit_ragged_tiled: bad arguments");
-  AUGUR_REQUIRE(B * G * G * G < ((int64_t)1 << 32), AUGUR_EUNSUPPORTED,
-                "cic_deposit_ragged_tiled: B G^3 must be below 2^32");
-  hipStream_t s = as_stream(stream);
-  const int64_t G3 = G * G * G;
-  if (total == 0) return fill_u32(out, B * G3, 0u, s);
-  AUGUR_REQUIRE(pos, AUGUR_EINVAL, "cic_deposit_ragged_tiled: null positions");
-  Carver c(ws, ws_bytes);
-  RaggedDepBufs b;
-  AUGUR_REQUIRE(carve_ragged_dep(c, B, total, G, b), AUGUR_EWORKSPACE,
-                "cic_deposit_ragged_tiled: workspace too small (%zu < %zu)", ws_bytes, c.used);
-  const int64_t n = total, tps = pm_tiles(G), ntiles = B * tps;
+}  // extern "C"
+
+// The ragged tiled deposit after validation and carving: the sort and plans
+// (gated by `gate`, NULL = always), the deposit bound and the tiled deposit
+// of every scene's tiles into out (B grids).
+static int ragged_deposit(const float* pos, const float* mass, float mass_value, const int64_t* starts, int64_t B,
+                          int64_t n, int64_t G, float h, float scale, float* out, RaggedDepBufs& b,
+                          const uint32_t* gate, hipStream_t s) {
+  const int64_t G3 = G * G * G, tps = pm_tiles(G), ntiles = B * tps;
   const dim3 g(blocks_for(n, 256, 1024)), bl(256);
   {
     TimedScope ts_("ragged_sort", s);
-    hipLaunchKernelGGL(k_ragged_keys, g, bl, 0, s, pos, starts, (int)B, n, (int)G, h, b.keys, b.vals, b.ngaps);
+    hipLaunchKernelGGL(k_ragged_keys, g, bl, 0, s, pos, starts, (int)B, n, (int)G, h, b.keys, b.vals, b.ngaps, gate);
     AUGUR_LAUNCHED("ragged_keys");
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) < B * G3) ++bits;
     uint32_t *ks, *perm;
-    int rc = radix_sort_pairs(b.keys, b.vals, n, bits, b.rws, radix_workspace_bytes(n), &ks, &perm, s, nullptr);
+    int rc = radix_sort_pairs(b.keys, b.vals, n, bits, b.rws, radix_workspace_bytes(n), &ks, &perm, s, gate);
     if (rc) return rc;
     hipLaunchKernelGGL(k_cell_starts, dim3(blocks_for(n + 1, 256, 4096)), bl, 0, s, (const uint32_t*)ks, n,
-                       (uint32_t)(B * G3), b.cell_start, b.gaps, b.ngaps, (const uint32_t*)nullptr);
+                       (uint32_t)(B * G3), b.cell_start, b.gaps, b.ngaps, gate);
     AUGUR_LAUNCHED("cell_starts");
     hipLaunchKernelGGL(k_cell_fill, dim3(1024), bl, 0, s, (const uint32_t*)b.gaps, (const uint32_t*)b.ngaps,
-                       b.cell_start, (const uint32_t*)nullptr);
+                       b.cell_start, gate);
     AUGUR_LAUNCHED("cell_fill");
     hipLaunchKernelGGL(k_dep_plans, dim3((unsigned)((ntiles + 1 + 3) / 4)), dim3(256), 0, s,
-                       (const uint32_t*)b.cell_start, (int)G, (int)ntiles, (int)tps, b.items, b.tseg_lo, b.tseg_pre);
+                       (const uint32_t*)b.cell_start, (int)G, (int)ntiles, (int)tps, b.items, b.tseg_lo, b.tseg_pre,
+                       gate);
     AUGUR_LAUNCHED("dep_plans");
-    rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, nullptr);
+    rc = exclusive_scan_u32(b.items, b.items, ntiles + 1, b.scan_ws, s, gate);
     if (rc) return rc;
     const int64_t max_items = ntiles + (8 * n + TD_CHUNK - 1) / TD_CHUNK;
     hipLaunchKernelGGL(k_dep_items, dim3((unsigned)max_items), dim3(256), 0, s, (const uint32_t*)b.items, (int)ntiles,
-                       (const uint32_t*)b.tseg_pre, b.info, b.iblk);
+                       (const uint32_t*)b.tseg_pre, b.info, b.iblk, gate);
     AUGUR_LAUNCHED("dep_items");
-    hipLaunchKernelGGL(k_dep_zero_heavy, dim3((unsigned)ntiles), dim3(256), 0, s, (const uint32_t*)b.items, b.fix);
+    hipLaunchKernelGGL(k_dep_zero_heavy, dim3((unsigned)ntiles), dim3(256), 0, s, (const uint32_t*)b.items, b.fix,
+                       gate);
     AUGUR_LAUNCHED("dep_zero_heavy");
-    b.vals = perm;  // sorted slot -> particle
+    b.vals = perm;  // sorted slot -> particle (the same buffer on every call: the pass count is fixed)
   }
   {
     TimedScope ts_("dep_bound", s);
@@ -1989,6 +1986,70 @@ int augur_cic_deposit_ragged_tiled(const float* pos, const float* mass, float ma
     AUGUR_LAUNCHED("dep_finish");
   }
   return AUGUR_OK;
+}
+
+static int check_ragged_dep(int64_t B, int64_t total, int64_t G, float h, const int64_t* starts, const float* out,
+                            const char* who) {
+  AUGUR_REQUIRE(is_pow2(G) && G >= 32 && G <= 512, AUGUR_EUNSUPPORTED, "%s: G must be a power of two in [32, 512]",
+                who);
+  AUGUR_REQUIRE(B >= 1 && B <= 65535 && total >= 0 && total < ((int64_t)1 << 31) && h > 0.0f && starts && out,
+                AUGUR_EINVAL, "%s: bad arguments", who);
+  AUGUR_REQUIRE(B * G * G * G < ((int64_t)1 << 32), AUGUR_EUNSUPPORTED, "%s: B G^3 must be below 2^32", who);
+  return AUGUR_OK;
+}
+
+extern "C" {
+
+int augur_cic_deposit_ragged_tiled(const float* pos, const float* mass, float mass_value, const int64_t* starts,
+                                   int64_t B, int64_t total, int64_t G, float h, float scale, float* out, void* ws,
+                                   size_t ws_bytes, void* stream) {
+  int rc = check_ragged_dep(B, total, G, h, starts, out, "cic_deposit_ragged_tiled");
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  if (total == 0) return fill_u32(out, B * G * G * G, 0u, s);
+  AUGUR_REQUIRE(pos, AUGUR_EINVAL, "cic_deposit_ragged_tiled: null positions");
+  Carver c(ws, ws_bytes);
+  RaggedDepBufs b;
+  AUGUR_REQUIRE(carve_ragged_dep(c, B, total, G, b), AUGUR_EWORKSPACE,
+                "cic_deposit_ragged_tiled: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  return ragged_deposit(pos, mass, mass_value, starts, B, total, G, h, scale, out, b, nullptr, s);
+}
+
+size_t augur_cic_deposit_ragged_planned_workspace_bytes(int64_t B, int64_t G, int64_t total) {
+  return augur_cic_deposit_ragged_tiled_workspace_bytes(B, G, total);
+}
+
+int augur_cic_deposit_ragged_planned(const float* pos, const float* mass, float mass_value, const int64_t* starts,
+                                     int64_t B, int64_t total, int64_t G, float h, float scale, float drift_dt,
+                                     int force_sort, float* out, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_ragged_dep(B, total, G, h, starts, out, "cic_deposit_ragged_planned");
+  if (rc) return rc;
+  AUGUR_REQUIRE(drift_dt >= 0.0f, AUGUR_EINVAL, "cic_deposit_ragged_planned: drift_dt must be >= 0");
+  hipStream_t s = as_stream(stream);
+  if (total == 0) return fill_u32(out, B * G * G * G, 0u, s);
+  AUGUR_REQUIRE(pos, AUGUR_EINVAL, "cic_deposit_ragged_planned: null positions");
+  Carver c(ws, ws_bytes);
+  RaggedDepBufs b;
+  AUGUR_REQUIRE(carve_ragged_dep(c, B, total, G, b), AUGUR_EWORKSPACE,
+                "cic_deposit_ragged_planned: workspace too small (%zu < %zu)", ws_bytes, c.used);
+  if (force_sort) {
+    hipLaunchKernelGGL(k_pm_init, dim3(1), dim3(64), 0, s, b.st);
+    AUGUR_LAUNCHED("pm_init");
+  }
+  // the drift since the last sort (max |v| of the tracked drifts, dt each) decides the re-sort
+  hipLaunchKernelGGL(k_pm_decide, dim3(1), dim3(64), 0, s, b.st, drift_dt / h, force_sort);
+  AUGUR_LAUNCHED("pm_decide");
+  rc = ragged_deposit(pos, mass, mass_value, starts, B, total, G, h, scale, out, b, &b.st->need_sort, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st);
+  AUGUR_LAUNCHED("pm_sorted");
+  return AUGUR_OK;
+}
+
+uint32_t* augur_ragged_plan_drift_word(void* ws, size_t ws_bytes) {
+  Carver c(ws, ws_bytes);
+  PmDev* st = c.take<PmDev>(1);
+  return c.ok() ? &st->vmax_bits : nullptr;
 }
 
 int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, void* ws, size_t ws_bytes,

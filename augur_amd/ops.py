@@ -169,6 +169,43 @@ def cic_deposit_ragged(pos, starts, G, h, off, periodic, mass=None, vel=None, sc
     return out
 
 
+class RaggedPlan:
+    """The persistent workspace of a planned ragged deposit
+    (augur_cic_deposit_ragged_planned): the batch's sort and tile plans kept
+    between calls and redone on the device only when the drift recorded by
+    augur_persistent_step_tracked (through ``drift_word``) may have moved a
+    particle a cell since the last sort."""
+
+    def __init__(self, B, G, total, device):
+        self.key = (int(B), int(G), int(total))
+        self.ws = _ws(_lib.size("augur_cic_deposit_ragged_planned_workspace_bytes", B, G, total), device)
+        self.drift_word = _lib.get().augur_ragged_plan_drift_word(_p(self.ws), self.ws.numel())
+        assert self.drift_word, "planned ragged deposit: workspace too small"
+
+
+def cic_deposit_ragged_planned(pos, starts, G, h, plan, drift_dt, force_sort, mass=None, mass_value=1.0,
+                               scale=None):
+    """As cic_deposit_ragged on the tiled deposit (3D periodic, off 0, mass
+    only; the same bits), with the sort and plans kept in `plan` (a
+    RaggedPlan of this batch's sizes): force_sort re-sorts unconditionally,
+    otherwise the device re-sorts when the tracked drift (drift_dt times the
+    largest |v| of the drifts since the previous call) would pass a cell."""
+    pos = _c(pos, F32)
+    dev = _dev(pos)
+    total = pos.shape[0]
+    starts = _c(starts, torch.int64)
+    B = starts.shape[0] - 1
+    mass = _c(mass, F32)
+    assert plan.key == (B, int(G), total), (plan.key, (B, G, total))
+    if scale is None:
+        scale = 1.0 / h ** 3
+    out = torch.empty((B, 1, G, G, G), dtype=F32, device=dev)
+    _lib.call("augur_cic_deposit_ragged_planned", _p(pos), _p(mass), float(mass_value), _p(starts), B, total, int(G),
+              float(h), float(scale), float(drift_dt), int(bool(force_sort)), _p(out), _p(plan.ws), plan.ws.numel(),
+              _stream(dev))
+    return out
+
+
 def cic_gather(field, pos, h, off, periodic):
     """field (B, C, G..G), pos (B, N, D) -> (B, N, C)."""
     field = _c(field, F32)

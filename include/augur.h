@@ -332,6 +332,17 @@ int augur_pm_force_replay(int64_t n, int64_t G, float L, const uint32_t* gate, v
 int augur_persistent_step(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B, int64_t total,
                           const float* force, int64_t G, float L, float dt, int phase, void* stream);
 
+/* augur_persistent_step that also records the largest |v| component of its
+ * drifts (phases OPEN and CLOSE_OPEN) into drift_word, the drift word of a
+ * planned ragged deposit's workspace (augur_ragged_plan_drift_word), with
+ * device atomics: the bound from which that deposit decides its re-sort.
+ * Same bits as augur_persistent_step.  Part of the ragged batch stepping
+ * that replaces ScatterField._step / rollout (scripts/scatter_field.py:
+ * 677-688, 745), as augur_persistent_step. */
+int augur_persistent_step_tracked(float* pos, float* vel, float* acc, const int64_t* starts, int64_t B,
+                                  int64_t total, const float* force, int64_t G, float L, float dt, int phase,
+                                  uint32_t* drift_word, void* stream);
+
 /* The PM force planes of B density grids in one pass sequence: rho (B, G, G,
  * G) FP32 -> force (B, 3, G, G, G) FP32 (a_x, a_y, a_z planes per grid), the
  * fused FFT passes of augur_pm_step_begin with the batch as a second grid
@@ -355,6 +366,26 @@ size_t augur_cic_deposit_ragged_tiled_workspace_bytes(int64_t B, int64_t G, int6
 int augur_cic_deposit_ragged_tiled(const float* pos, const float* mass, float mass_value, const int64_t* starts,
                                    int64_t B, int64_t total, int64_t G, float h, float scale, float* out, void* ws,
                                    size_t ws_bytes, void* stream);
+
+/* augur_cic_deposit_ragged_tiled with its sort and plans kept in the
+ * caller's workspace between calls (the same size) and redone only when the
+ * particles may have moved a cell: force_sort != 0 sorts unconditionally
+ * (the first call, or after the caller changed the positions); otherwise
+ * the drift since the last sort is accumulated on the device from the drift
+ * word (drift_dt times the largest |v| recorded by
+ * augur_persistent_step_tracked since the previous call) and the batch is
+ * re-sorted when it would pass one cell.  Same bits as the unplanned
+ * deposit; no host synchronisation.  Replaces, as augur_cic_deposit_ragged_
+ * tiled, the deposit of scripts/scatter_field.py:237-250; the drift-gated
+ * re-sort is the stepper's (the reference sorts every call,
+ * scripts/scatter_field.py:331-333). */
+size_t augur_cic_deposit_ragged_planned_workspace_bytes(int64_t B, int64_t G, int64_t total);
+int augur_cic_deposit_ragged_planned(const float* pos, const float* mass, float mass_value, const int64_t* starts,
+                                     int64_t B, int64_t total, int64_t G, float h, float scale, float drift_dt,
+                                     int force_sort, float* out, void* ws, size_t ws_bytes, void* stream);
+/* The drift word of a planned ragged deposit's workspace (a device pointer
+ * for augur_persistent_step_tracked), NULL if the workspace is too small. */
+uint32_t* augur_ragged_plan_drift_word(void* ws, size_t ws_bytes);
 
 size_t augur_pm_solve_grids_workspace_bytes(int64_t B, int64_t G);
 int augur_pm_solve_grids(const float* rho, float* force, int64_t B, int64_t G, float L, float Gconst, void* ws,

@@ -18,6 +18,15 @@ def test_exports_every_header_symbol(lib):
     assert lib.augur_abi_version() == 1
 
 
+def test_planned_deposit_drift_word_is_in_its_workspace(lib):
+    n = _lib.size("augur_cic_deposit_ragged_planned_workspace_bytes", 4, 64, 10000)
+    assert n >= _lib.size("augur_cic_deposit_ragged_tiled_workspace_bytes", 4, 64, 10000)
+    base = 1 << 20
+    w = lib.augur_ragged_plan_drift_word(base, n)
+    assert w is not None and base <= w < base + 4096
+    assert lib.augur_ragged_plan_drift_word(base, 8) is None
+
+
 def test_workspace_queries_are_host_only(lib):
     assert _lib.size("augur_sort_workspace_bytes", 1 << 20, 1 << 18) >= 8 << 20
     assert _lib.size("augur_cic_deposit_workspace_bytes", 2, 3, 64, 4) >= 2 * 4 * 64 ** 3 * 8
@@ -38,6 +47,11 @@ def test_workspace_queries_are_host_only(lib):
     ("augur_poisson_green_grad", (None, 1, 1024, 1.0, 1.0, 1.0, None, None, None), "power-of-two"),
     ("augur_direct_sum_f64", (None, None, None, 10, 3, None, 4, 0.1, None, None, 0, None), "exactly one"),
     ("augur_persistent_step", (1, 1, 1, 1, 0, 10, 1, 32, 1.0, 0.1, 0, None), "B must be"),
+    ("augur_persistent_step_tracked", (1, 1, 1, 1, 2, 10, 1, 32, 1.0, 0.1, 3, None, None), "null drift word"),
+    ("augur_cic_deposit_ragged_planned", (1, None, 1.0, 1, 2, 10, 48, 0.1, 1.0, 0.1, 1, 1, None, 0, None),
+     "power of two"),
+    ("augur_cic_deposit_ragged_planned", (1, None, 1.0, 1, 2, 10, 32, 0.1, 1.0, -1.0, 1, 1, None, 0, None),
+     "drift_dt"),
     ("augur_persistent_step", (1, 1, 1, 1, 2, 10, 1, 48, 1.0, 0.1, 0, None), "power of two"),
     ("augur_persistent_step", (1, 1, 1, 1, 2, 10, 1, 32, 1.0, 0.1, 7, None), "unknown phase"),
     ("augur_persistent_step", (1, None, 1, 1, 2, 10, 1, 32, 1.0, 0.1, 3, None), "vel is required"),

@@ -201,6 +201,38 @@ def test_ragged_kdk_matches_oracle_per_scene(dev):
         lo += n
 
 
+def test_planned_deposit_resorts_on_drift_and_keeps_the_bits(dev):
+    """The stepping loop's planned ragged deposit sorts on the first step of
+    a call and re-sorts on the device only when the tracked drift may pass a
+    cell: fast particles (about 0.4 cells per step) force re-sorts inside a
+    12-step call, and the call gives the bits of 12 one-step calls (every
+    deposit freshly sorted), with empty and one-particle scenes."""
+    from augur_amd import batch as RB
+    G, dt, steps = 64, 2e-3, 12
+    sizes = [6000, 0, 2500, 1]
+    pos, vel, acc, starts = _ragged_state(sizes, 41, dev)
+    vel = vel * 60.0                                         # |v| ~ 3: 0.4 cells per step at G = 64
+    total = pos.shape[0]
+    mass = torch.from_numpy(np.random.default_rng(42).uniform(0.5, 1.5, total).astype(np.float32)).to(dev)
+    mass /= total
+    st = RB.RaggedStepper(G, device=dev)
+    acc = st.accel(pos, starts, mass)
+    P, V, A = pos.clone(), vel.clone(), acc.clone()
+    st.kdk(pos, vel, acc, starts, mass, dt, steps)
+    for _ in range(steps):
+        st.kdk(P, V, A, starts, mass, dt, 1)
+    assert torch.equal(pos, P) and torch.equal(vel, V) and torch.equal(acc, A)
+    # the long call re-sorted on the way: the drift accumulated since its last
+    # sort (0 if the last step re-sorted) is below a cell, where 12 steps
+    # without one would be ~5 cells
+    plan = st.plan(total, starts)
+    off = plan.drift_word - plan.ws.data_ptr() - 4        # PmDev: cum_cells just before the drift word
+    st.kdk(pos, vel, acc, starts, mass, dt, steps)
+    torch.cuda.synchronize(dev)
+    cum = float(plan.ws[off: off + 4].view(torch.float32)[0])
+    assert 0.0 <= cum < 1.0, cum
+
+
 @pytest.mark.parametrize("with_net", [False, True])
 def test_ragged_kdk_graph_replays_match_eager(dev, with_net):
     """A 2-step call of the batched stepper (plain PM, or PM with the
