@@ -579,7 +579,11 @@ __global__ void k_zero_u64(unsigned long long* p, int64_t n) {
 // columns per row and a row pitch RP rounded up to 16 complex (128 bytes), so
 // a column tile's row segment is one aligned cache line.
 __host__ __device__ constexpr int64_t pm_pitch(int64_t G) { return (G / 2 + 1 + 15) / 16 * 16; }
-// Column tiles take NL adjacent kx columns (line-fastest thread mapping).
+// Column tiles take NL lines (line-fastest thread mapping): the first N / NQ
+// tiles hold the Nyquist column kx = N/2 of NQ <= NL adjacent rows (one cache
+// line per point, so they start first and overlap the rest; lines NQ .. NL-1
+// stay empty), and N * nfull more hold NL adjacent kx < N/2 columns of one
+// row each.
 template <int N>
 struct PmCol {
   static constexpr int T = N >= 512 ? 1024 : FFT_THREADS;
@@ -587,7 +591,38 @@ struct PmCol {
   static constexpr int LS = FftStride<N>::value;
   static constexpr int nxc = N / 2 + 1;
   static constexpr int RP = (int)pm_pitch(N);
-  static constexpr int nblk = (nxc + NL - 1) / NL;
+  static constexpr int nfull = N / 2 / NL;
+  // Nyquist rows per tile, z pass and y pass.  Measured at N = 256 (column
+  // pass times): 16 rows put all the y pass's one-line-per-point traffic on
+  // 16 workgroups and ran 8 us slower than one tile per row; 2 rows (y) and 8
+  // (z) ran 2.4 and 3.2 us faster than one tile per row.
+  static constexpr int NQZ = NL < 8 ? NL : 8, NQY = 2;
+  static constexpr int blocks(int nq) { return N / nq + N * nfull; }  // workgroups per grid
+  static_assert((N / 2) % NL == 0 && NQZ <= NL && NQY <= NL && N % NQZ == 0 && N % NQY == 0,
+                "column tiles must divide the rows");
+};
+
+// Where a column tile's lines sit: line c, point k at base + k * es + c * ls,
+// with its kx (and, for the row-indexed pass, its row) per line.
+template <int N, bool NYQ, int NQ>
+struct PmColTile {
+  using C = PmCol<N>;
+  int r0, c0;  // first row (ky or z) and first kx
+  int64_t ls;  // line stride: one row apart (row_stride) for the Nyquist tiles
+  __device__ PmColTile(int b, int64_t row_stride) {
+    if (NYQ) {
+      r0 = b * NQ;
+      c0 = N / 2;
+      ls = row_stride;
+    } else {
+      r0 = (b - N / NQ) / C::nfull;
+      c0 = ((b - N / NQ) % C::nfull) * C::NL;
+      ls = 1;
+    }
+  }
+  __device__ bool used(int c) const { return !NYQ || NQ == C::NL || c < NQ; }
+  __device__ int kx(int c) const { return NYQ ? N / 2 : c0 + c; }
+  __device__ int row(int c) const { return NYQ ? r0 + c : r0; }
 };
 
 // z columns at fixed ky: forward z FFT, Green's function g = -4 pi G / k^2 / G^3,
@@ -596,27 +631,20 @@ struct PmCol {
 // is formed in place, the B transform runs in place too, and each thread
 // keeps the P values its first B pass reads, to put P back for the A
 // transform (four workgroups per CU instead of two).
-template <int N>
-__global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ spec, float2* __restrict__ outB,
-                                                         float kf, float green) {
+template <int N, bool NYQ>
+__device__ __forceinline__ void pm_zpass_tile(float2* __restrict__ spec, float2* __restrict__ outB, float kf,
+                                              float green, float2* X, const float2* tw) {
   using C = PmCol<N>;
   constexpr int R0 = FftPlan<N>::R0, Q = N / R0;
-  spec += blockIdx.y * (int64_t)N * N * C::RP;  // grid blockIdx.y of a batch
-  outB += blockIdx.y * (int64_t)N * N * C::RP;
-  __shared__ float2 X[C::NL * C::LS];
-  __shared__ float2 tw[N];
-  make_twiddles<N>(tw);
-  __syncthreads();
-  const int ky = blockIdx.x / C::nblk;
-  const int c0 = (blockIdx.x % C::nblk) * C::NL;
-  const int64_t base = (int64_t)ky * C::RP + c0, es = (int64_t)N * C::RP;
-  const int ny = ky <= N / 2 ? ky : ky - N;
+  const PmColTile<N, NYQ, C::NQZ> tl(blockIdx.x, C::RP);
+  const int64_t base = (int64_t)tl.r0 * C::RP + tl.c0, es = (int64_t)N * C::RP;
   auto gsrc = [&](int c, int k) -> float2 {
-    return (c0 + c < C::nxc) ? spec[base + k * es + c] : make_float2(0.f, 0.f);
+    return tl.used(c) ? spec[base + k * es + c * tl.ls] : make_float2(0.f, 0.f);
   };
   // forward transform, scaled by the Green's function, into X (in place)
   auto green_dst = [&](int c, int kz, float2 v) {
-    const int kx = c0 + c;
+    const int kx = tl.kx(c), ky = tl.row(c);
+    const int ny = ky <= N / 2 ? ky : ky - N;
     const int nz = kz <= N / 2 ? kz : kz - N;
     const float fx = kf * kx, fy = kf * ny, fz = kf * nz;
     const float k2 = fx * fx + fy * fy + fz * fz;
@@ -635,7 +663,7 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ s
     return make_float2(az * p.y, -az * p.x);
   };
   auto bdst = [&](int c, int k, float2 v) {
-    if (c0 + c < C::nxc) outB[base + k * es + c] = v;
+    if (tl.used(c)) outB[base + k * es + c * tl.ls] = v;
   };
   fft_lines<N, true, true, true, false>(X, tw, C::NL, bsrc, bdst);
   // P back into X: the first pass of a line-fastest transform gives thread t
@@ -649,34 +677,42 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ s
   __syncthreads();
   auto asrc = [&](int c, int k) { return X[c * C::LS + fft_pad(k)]; };
   auto adst = [&](int c, int k, float2 v) {
-    if (c0 + c < C::nxc) spec[base + k * es + c] = v;
+    if (tl.used(c)) spec[base + k * es + c * tl.ls] = v;
   };
   fft_lines<N, true, true, true, false>(X, tw, C::NL, asrc, adst);
+}
+
+template <int N>
+__global__ __launch_bounds__(PmCol<N>::T) void k_pm_zpass(float2* __restrict__ spec, float2* __restrict__ outB,
+                                                         float kf, float green) {
+  using C = PmCol<N>;
+  spec += blockIdx.y * (int64_t)N * N * C::RP;  // grid blockIdx.y of a batch
+  outB += blockIdx.y * (int64_t)N * N * C::RP;
+  __shared__ float2 X[C::NL * C::LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  __syncthreads();
+  if (blockIdx.x < (unsigned)(N / C::NQZ))
+    pm_zpass_tile<N, true>(spec, outB, kf, green, X, tw);
+  else
+    pm_zpass_tile<N, false>(spec, outB, kf, green, X, tw);
 }
 
 // y columns at fixed z: reads A (s0) and B (s2); writes
 //   s1 = IFFTy(-i ky A)      (Nyquist ky zeroed)
 //   s0 = IFFTy(A)            (in place; the x pass applies -i kx)
 //   s2 = IFFTy(B)            (in place)
-template <int N>
-__global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0, float2* __restrict__ s1,
-                                                        float2* __restrict__ s2, float kf) {
+template <int N, bool NYQ>
+__device__ __forceinline__ void pm_yinv_tile(float2* __restrict__ s0, float2* __restrict__ s1,
+                                             float2* __restrict__ s2, float kf, float2* X, const float2* tw) {
   using C = PmCol<N>;
-  s0 += blockIdx.y * (int64_t)N * N * C::RP;  // grid blockIdx.y of a batch
-  s1 += blockIdx.y * (int64_t)N * N * C::RP;
-  s2 += blockIdx.y * (int64_t)N * N * C::RP;
-  __shared__ float2 X[C::NL * C::LS];
-  __shared__ float2 tw[N];
-  make_twiddles<N>(tw);
-  __syncthreads();
-  const int z = blockIdx.x / C::nblk;
-  const int c0 = (blockIdx.x % C::nblk) * C::NL;
-  const int64_t base = (int64_t)z * N * C::RP + c0, es = C::RP;
+  const PmColTile<N, NYQ, C::NQY> tl(blockIdx.x, (int64_t)N * C::RP);
+  const int64_t base = (int64_t)tl.r0 * N * C::RP + tl.c0, es = C::RP;
   auto ld = [&](const float2* p, int c, int k) -> float2 {
-    return (c0 + c < C::nxc) ? p[base + k * es + c] : make_float2(0.f, 0.f);
+    return tl.used(c) ? p[base + k * es + c * tl.ls] : make_float2(0.f, 0.f);
   };
   auto st = [&](float2* p, int c, int k, float2 v) {
-    if (c0 + c < C::nxc) p[base + k * es + c] = v;
+    if (tl.used(c)) p[base + k * es + c * tl.ls] = v;
   };
   // s0 is read once: the first pass of the -i ky transform keeps the values
   // it reads, and the first pass of the plain transform (same thread, same
@@ -696,6 +732,23 @@ __global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0
                            [&](int c, int k, float2 v) { st(s0, c, k, v); });
   fft_lines<N, true, true>(X, tw, C::NL, [&](int c, int k) { return ld(s2, c, k); },
                            [&](int c, int k, float2 v) { st(s2, c, k, v); });
+}
+
+template <int N>
+__global__ __launch_bounds__(PmCol<N>::T) void k_pm_yinv(float2* __restrict__ s0, float2* __restrict__ s1,
+                                                        float2* __restrict__ s2, float kf) {
+  using C = PmCol<N>;
+  s0 += blockIdx.y * (int64_t)N * N * C::RP;  // grid blockIdx.y of a batch
+  s1 += blockIdx.y * (int64_t)N * N * C::RP;
+  s2 += blockIdx.y * (int64_t)N * N * C::RP;
+  __shared__ float2 X[C::NL * C::LS];
+  __shared__ float2 tw[N];
+  make_twiddles<N>(tw);
+  __syncthreads();
+  if (blockIdx.x < (unsigned)(N / C::NQY))
+    pm_yinv_tile<N, true>(s0, s1, s2, kf, X, tw);
+  else
+    pm_yinv_tile<N, false>(s0, s1, s2, kf, X, tw);
 }
 
 // x rows: three half spectra (A~, -i ky A~, B~) -> real rows; the first gets
@@ -1494,13 +1547,13 @@ static int pm_solve(int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s, 
   const float green = (float)(-4.0 * PI * (double)Gconst / ((double)G * G * G));
   {
     TimedScope ts_("pm_zpass", s);
-    PM_SWITCH_G(G, k_pm_zpass, dim3((unsigned)(G * PmCol<GG>::nblk), (unsigned)nb), dim3(PmCol<GG>::T), s, b.s0, b.s2,
+    PM_SWITCH_G(G, k_pm_zpass, dim3((unsigned)PmCol<GG>::blocks(PmCol<GG>::NQZ), (unsigned)nb), dim3(PmCol<GG>::T), s, b.s0, b.s2,
                 kf, green);
     AUGUR_LAUNCHED("pm_zpass");
   }
   {
     TimedScope ts_("pm_yinv", s);
-    PM_SWITCH_G(G, k_pm_yinv, dim3((unsigned)(G * PmCol<GG>::nblk), (unsigned)nb), dim3(PmCol<GG>::T), s, b.s0, b.s1,
+    PM_SWITCH_G(G, k_pm_yinv, dim3((unsigned)PmCol<GG>::blocks(PmCol<GG>::NQY), (unsigned)nb), dim3(PmCol<GG>::T), s, b.s0, b.s1,
                 b.s2, kf);
     AUGUR_LAUNCHED("pm_yinv");
   }
