@@ -201,6 +201,24 @@ int exclusive_scan2_u32(const uint32_t* in0, uint32_t* out0, int64_t n0, const u
   return rc ? rc : exclusive_scan_u32(in1, out1, n1, ws, s, gate);
 }
 
+// First level only, when the block sums are short enough for the consumer to
+// add their prefix itself (at most SC_FUSE of them): *bsums / *nbs name them,
+// and out[i] + (exclusive prefix of bsums)[i / SC_TILE] is the scan.  Else
+// (*bsums = nullptr) the scan is complete.
+static int exclusive_scan_u32_split(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s,
+                                    const uint32_t* gate, const uint32_t** bsums, int* nbs) {
+  *bsums = nullptr;
+  *nbs = 0;
+  const int64_t nb = (n + SC_TILE - 1) / SC_TILE;
+  if (n <= SC_SMALL || nb > SC_FUSE) return exclusive_scan_u32(in, out, n, ws, s, gate);
+  uint32_t* sums = (uint32_t*)ws;
+  hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SC_THREADS), 0, s, in, out, n, sums, gate);
+  AUGUR_LAUNCHED("scan_block");
+  *bsums = sums;
+  *nbs = (int)nb;
+  return AUGUR_OK;
+}
+
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* ws, hipStream_t s, const uint32_t* gate) {
   if (n <= 0) return AUGUR_OK;
   int64_t nb = (n + SC_TILE - 1) / SC_TILE;
@@ -256,13 +274,34 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, in
   hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// offs: the scanned histograms, or their first scan level with the block sums
+// bsums (nbs <= SC_FUSE of them) whose prefix each workgroup adds itself
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                                                            uint32_t* vout, int64_t n, int shift, const uint32_t* offs,
-                                                           int nblocks, const uint32_t* gate) {
+                                                           const uint32_t* bsums, int nbs, int nblocks,
+                                                           const uint32_t* gate) {
+  static_assert(RS_THREADS == SC_THREADS && SC_FUSE == 4 * SC_THREADS, "block-sum prefix layout");
   if (gate && *gate == 0u) return;
   __shared__ uint32_t cnt[RS_THREADS / 64][256];
+  __shared__ uint32_t bpre[SC_FUSE];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < (RS_THREADS / 64) * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+  if (bsums) {  // exclusive prefix of the block sums: four per thread
+    uint32_t v[4], loc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = 4 * threadIdx.x + j;
+      v[j] = i < nbs ? bsums[i] : 0u;
+      loc += v[j];
+    }
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan_u32(loc, &tot);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bpre[4 * threadIdx.x + j] = pre;
+      pre += v[j];
+    }
+  }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * RS_WAVE_RUN;
   constexpr int NB = RS_WAVE_RUN / 64;
@@ -295,7 +334,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, 
   __syncthreads();
   {
     const int d = threadIdx.x;  // 256 threads, one digit each
-    uint32_t s = offs[(int64_t)d * nblocks + blockIdx.x];
+    const int64_t at = (int64_t)d * nblocks + blockIdx.x;
+    uint32_t s = offs[at] + (bsums ? bpre[at / SC_TILE] : 0u);
     for (int q = 0; q < RS_THREADS / 64; ++q) {
       uint32_t c = cnt[q][d];
       cnt[q][d] = s;
@@ -345,10 +385,12 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* 
   for (int shift = 0; shift < bits; shift += 8) {
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ka, n, shift, hist, (int)nb, gate);
     AUGUR_LAUNCHED("rs_hist");
-    int rc = exclusive_scan_u32(hist, hist, nb * 256, sws, s, gate);
+    const uint32_t* bsums;
+    int nbs;
+    int rc = exclusive_scan_u32_split(hist, hist, nb * 256, sws, s, gate, &bsums, &nbs);
     if (rc) return rc;
     hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n, shift, hist,
-                       (int)nb, gate);
+                       bsums, nbs, (int)nb, gate);
     AUGUR_LAUNCHED("rs_scatter");
     uint32_t* t = ka;
     ka = kb;
