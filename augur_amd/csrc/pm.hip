@@ -1431,7 +1431,7 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   int rc = radix_sort_pairs(b.keys, b.vals, n, 3 * ilog2(G), b.rws, radix_workspace_bytes(n), &ks, &perm, s, gate);
   if (rc) return rc;
   // cell_start from the sorted keys
-  hipLaunchKernelGGL(k_cell_starts, dim3(blocks_for(n + 1, 256, 4096)), bl, 0, s, (const uint32_t*)ks, n,
+  hipLaunchKernelGGL(k_cell_starts, dim3(blocks_for(n + 1, 256, 2048)), bl, 0, s, (const uint32_t*)ks, n,
                      (uint32_t)G3, b.cell_start, b.gaps, b.ngaps, gate);
   AUGUR_LAUNCHED("cell_starts");
   hipLaunchKernelGGL(k_cell_fill, dim3(1024), bl, 0, s, (const uint32_t*)b.gaps, (const uint32_t*)b.ngaps,
@@ -1986,7 +1986,7 @@ static int ragged_deposit(const float* pos, const float* mass, float mass_value,
     uint32_t *ks, *perm;
     int rc = radix_sort_pairs(b.keys, b.vals, n, bits, b.rws, radix_workspace_bytes(n), &ks, &perm, s, gate);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_cell_starts, dim3(blocks_for(n + 1, 256, 4096)), bl, 0, s, (const uint32_t*)ks, n,
+    hipLaunchKernelGGL(k_cell_starts, dim3(blocks_for(n + 1, 256, 2048)), bl, 0, s, (const uint32_t*)ks, n,
                        (uint32_t)(B * G3), b.cell_start, b.gaps, b.ngaps, gate);
     AUGUR_LAUNCHED("cell_starts");
     hipLaunchKernelGGL(k_cell_fill, dim3(1024), bl, 0, s, (const uint32_t*)b.gaps, (const uint32_t*)b.ngaps,
