@@ -1037,15 +1037,17 @@ __global__ __launch_bounds__(256) void k_plans(const uint32_t* __restrict__ cell
 template <int MAXSEG, uint32_t CHUNK>
 __global__ __launch_bounds__(256) void k_plan_items(const uint32_t* __restrict__ item_start, int ntiles,
                                                     const uint32_t* __restrict__ tseg_pre, DepItem* __restrict__ info,
-                                                    uint16_t* __restrict__ blk, int item_blocks,
+                                                    uint16_t* __restrict__ blk, int max_items, int item_blocks,
                                                     const uint32_t* __restrict__ cell_start,
                                                     const uint32_t* __restrict__ gitem_start, int npieces,
                                                     DepItem* __restrict__ ginfo, const uint32_t* gate) {
   if (gate && *gate == 0u) return;
-  if ((int)blockIdx.x < item_blocks)
-    tile_items_body<MAXSEG, CHUNK>(blockIdx.x, item_start, ntiles, tseg_pre, info, blk);
-  else
+  if ((int)blockIdx.x < item_blocks) {  // the tiles' items from a capped grid (a short launch when gated off)
+    for (int item = blockIdx.x; item < max_items; item += item_blocks)
+      tile_items_body<MAXSEG, CHUNK>((uint32_t)item, item_start, ntiles, tseg_pre, info, blk);
+  } else {
     gather_items_body((int)blockIdx.x - item_blocks, cell_start, gitem_start, npieces, ginfo);
+  }
 }
 
 // Raw buffer views of an item's particle arrays: a lane past the item's end
@@ -1448,11 +1450,12 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
   AUGUR_LAUNCHED("plans");
   rc = exclusive_scan2_u32(b.items, b.items, ntiles + 1, b.gitems, b.gitems, npc + 1, b.scan_ws, s, gate);
   if (rc) return rc;
-  const int item_blocks = (int)pm_max_items(n, G);
+  const int max_items = (int)pm_max_items(n, G);
+  const int item_blocks = max_items < 2048 ? max_items : 2048;
   hipLaunchKernelGGL((k_plan_items<TD_MAXSEG, TD_CHUNK>), dim3((unsigned)(item_blocks + blocks_for(npc, 256))),
                      dim3(256), 0, s, (const uint32_t*)b.items, (int)ntiles, (const uint32_t*)b.tseg_pre, b.info,
-                     b.iblk, item_blocks, (const uint32_t*)b.cell_start, (const uint32_t*)b.gitems, (int)npc,
-                     b.ginfo, gate);
+                     b.iblk, max_items, item_blocks, (const uint32_t*)b.cell_start, (const uint32_t*)b.gitems,
+                     (int)npc, b.ginfo, gate);
   AUGUR_LAUNCHED("plan_items");
   if (!physical) {
     hipLaunchKernelGGL(k_copy<uint32_t>, g, bl, 0, s, perm, b.perm, n, gate);
