@@ -64,12 +64,15 @@ __global__ void k_pm_decide(PmDev* st, float dt_over_h, int force_sort) {
   st->need_sort = (force_sort || st->cum_cells + DRIFT_MARGIN > (float)DMAX) ? 1u : 0u;
 }
 
-__global__ void k_pm_sorted(PmDev* st) {
+// also, with uniform masses (uniform != 0), the deposit bound |mass_value *
+// scale| that k_pm_set_maxbits would write (one launch less per step)
+__global__ void k_pm_sorted(PmDev* st, int uniform, float mass_value, float scale) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (st->need_sort) {
     st->cum_cells = 0.0f;
     st->need_sort = 0u;
   }
+  if (uniform) st->maxbits = __float_as_uint(fabsf(__fmul_rn(mass_value, scale)));
 }
 
 __global__ void k_pm_set_maxbits(PmDev* st, const float* mass, float mass_value, float scale, int64_t n) {
@@ -1488,12 +1491,19 @@ static int pm_sort(float* pos, float* vel, float* acc, float* mass, int32_t* ids
     default: set_error("pm: unsupported G %d", (int)(G)); return AUGUR_EUNSUPPORTED;                  \
   }
 
-// deposit of sorted slots (optional perm) -> b.rho
-static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
-                      int64_t G, float L, PmBufs& b, hipStream_t s) {
+// the deposit's density scale 1 / h^3
+static float pm_dep_scale(int64_t G, float L) {
   const float h = (float)((double)L / (double)G);
-  const float scale = (float)(1.0 / ((double)h * (double)h * (double)h));
-  {
+  return (float)(1.0 / ((double)h * (double)h * (double)h));
+}
+
+// deposit of sorted slots (optional perm) -> b.rho; bound_ready: the deposit
+// bound is already in b.st (k_pm_sorted wrote it for uniform masses)
+static int pm_deposit(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
+                      int64_t G, float L, PmBufs& b, hipStream_t s, bool bound_ready = false) {
+  const float h = (float)((double)L / (double)G);
+  const float scale = pm_dep_scale(G, L);
+  if (!bound_ready) {
     TimedScope ts_("dep_bound", s);
     if (mass) {
       hipLaunchKernelGGL(k_pm_set_maxbits, dim3(1), dim3(64), 0, s, b.st, (const float*)nullptr, 0.0f, scale, n);
@@ -1571,8 +1581,8 @@ static int pm_solve(int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s, 
 
 // deposit -> FFT Poisson + spectral gradient -> force planes
 static int pm_force_grid(const float* pos, const uint32_t* perm, const float* mass, float mass_value, int64_t n,
-                         int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s) {
-  const int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s);
+                         int64_t G, float L, float Gconst, PmBufs& b, hipStream_t s, bool bound_ready = false) {
+  const int rc = pm_deposit(pos, perm, mass, mass_value, n, G, L, b, s, bound_ready);
   if (rc) return rc;
   return pm_solve(G, L, Gconst, b, s);
 }
@@ -1857,9 +1867,9 @@ int augur_pm_step_begin(float* pos, float* vel, float* acc, float* mass, int32_t
   // opens; acc is not (the last step rewrites all of it in the final order)
   rc = pm_sort(pos, vel, nullptr, mass, ids, n, G, L, b, &b.st->need_sort, true, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st);
+  hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st, mass ? 0 : 1, mass_value, pm_dep_scale(G, L));
   AUGUR_LAUNCHED("pm_sorted");
-  return pm_force_grid(pos, nullptr, mass, mass_value, n, G, L, Gconst, b, s);
+  return pm_force_grid(pos, nullptr, mass, mass_value, n, G, L, Gconst, b, s, mass == nullptr);
 }
 
 int augur_pm_step_end(float* pos, float* vel, float* acc, int64_t n, int64_t G, float L, float dt, int last, void* ws,
@@ -2097,7 +2107,7 @@ int augur_cic_deposit_ragged_planned(const float* pos, const float* mass, float 
   AUGUR_LAUNCHED("pm_decide");
   rc = ragged_deposit(pos, mass, mass_value, starts, B, total, G, h, scale, out, b, &b.st->need_sort, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st);
+  hipLaunchKernelGGL(k_pm_sorted, dim3(1), dim3(64), 0, s, b.st, 0, 0.0f, 0.0f);
   AUGUR_LAUNCHED("pm_sorted");
   return AUGUR_OK;
 }
