@@ -111,13 +111,58 @@ struct ColCfg {
   static constexpr int LS = FftStride<N>::value;
 };
 
+// Column tiles of NL adjacent columns of one outer row.  For a half spectrum
+// whose last column (the Nyquist column) would be a tile of its own, that
+// column instead goes to the first ceil(n_outer / COL_NQ) workgroups, COL_NQ
+// rows each (lines past COL_NQ empty), and the rest hold NL full columns.
+constexpr int COL_NQ = 2;
+
+template <int N, bool INV, bool NYQ>
+__device__ __forceinline__ void fft_cols_tile(float2* __restrict__ data, const ColMap& m, float2* lds,
+                                              const float2* tw, int64_t b) {
+  using C = ColCfg<N>;
+  if (NYQ) {  // Nyquist tile: line c is the last column of row b * COL_NQ + c
+    const int64_t o0 = b * COL_NQ;
+    const int col = m.ncols - 1;
+    auto base_of = [&](int c) -> int64_t {
+      const int64_t o = o0 + c;
+      return (o / m.q) * m.s1 + (o % m.q) * m.s2 + col;
+    };
+    auto used = [&](int c) { return c < COL_NQ && o0 + c < m.n_outer; };
+    auto src = [&](int c, int k) -> float2 {
+      return used(c) ? data[base_of(c) + k * m.es] : make_float2(0.f, 0.f);
+    };
+    auto dst = [&](int c, int k, float2 v) {
+      if (used(c)) data[base_of(c) + k * m.es] = v;
+    };
+    fft_lines<N, INV, true>(lds, tw, C::NL, src, dst);
+  } else {
+    const int nblk = NYQ ? 0 : (m.ncols - 1) / C::NL;  // full tiles per row (Nyquist column elsewhere)
+    const int64_t o = b / nblk;
+    const int c0 = (int)(b - o * nblk) * C::NL;
+    const int64_t base = (o / m.q) * m.s1 + (o % m.q) * m.s2 + c0;
+    auto src = [&](int c, int k) -> float2 { return data[base + k * m.es + c]; };
+    auto dst = [&](int c, int k, float2 v) { data[base + k * m.es + c] = v; };
+    fft_lines<N, INV, true>(lds, tw, C::NL, src, dst);
+  }
+}
+
+// nnyq > 0: Nyquist tiles first (nnyq of them), then full tiles; nnyq == 0:
+// every row in ceil(ncols / NL) tiles, the last one partly empty
 template <int N, bool INV>
-__global__ __launch_bounds__(ColCfg<N>::T) void k_fft_cols(float2* __restrict__ data, ColMap m) {
+__global__ __launch_bounds__(ColCfg<N>::T) void k_fft_cols(float2* __restrict__ data, ColMap m, int64_t nnyq) {
   using C = ColCfg<N>;
   __shared__ float2 lds[C::NL * C::LS];
   __shared__ float2 tw[N];
   make_twiddles<N>(tw);
   __syncthreads();
+  if (nnyq > 0) {
+    if ((int64_t)blockIdx.x < nnyq)
+      fft_cols_tile<N, INV, true>(data, m, lds, tw, blockIdx.x);
+    else
+      fft_cols_tile<N, INV, false>(data, m, lds, tw, (int64_t)blockIdx.x - nnyq);
+    return;
+  }
   const int nblk = (m.ncols + C::NL - 1) / C::NL;
   const int64_t o = blockIdx.x / nblk;
   const int c0 = (int)(blockIdx.x - o * nblk) * C::NL;
@@ -137,9 +182,13 @@ int launch_cols(int n, float2* data, const ColMap& m, hipStream_t s) {
   switch (n) {
 #define CASE(NN)                                                                                  \
   case NN: {                                                                                      \
-    const int64_t blocks = m.n_outer * ((m.ncols + ColCfg<NN>::NL - 1) / ColCfg<NN>::NL);         \
+    constexpr int NL = ColCfg<NN>::NL;                                                            \
+    const bool nyq = m.ncols > NL && (m.ncols - 1) % NL == 0;                                     \
+    const int64_t nnyq = nyq ? (m.n_outer + COL_NQ - 1) / COL_NQ : 0;                             \
+    const int64_t blocks = nyq ? nnyq + m.n_outer * ((m.ncols - 1) / NL)                          \
+                               : m.n_outer * ((m.ncols + NL - 1) / NL);                           \
     AUGUR_REQUIRE(blocks < ((int64_t)1 << 31), AUGUR_EINVAL, "fft: too many lines");              \
-    hipLaunchKernelGGL((k_fft_cols<NN, INV>), dim3((unsigned)blocks), dim3(ColCfg<NN>::T), 0, s, data, m); \
+    hipLaunchKernelGGL((k_fft_cols<NN, INV>), dim3((unsigned)blocks), dim3(ColCfg<NN>::T), 0, s, data, m, nnyq); \
     break;                                                                                        \
   }
     CASE(4) CASE(8) CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512)

@@ -151,8 +151,11 @@ def test_gather_matches_oracle(dev, D, periodic, G, C):
 
 
 # -------------------------------------------------------------------- a4 FFT
+# (16, 256) x 3 and (32, 64, 128) x 2: the Nyquist column's own tiles, with an odd row count (the
+# last such tile holds one row) and on both column axes of a batch
 @pytest.mark.parametrize("s,ins,B", [((8, 16), None, 3), ((64, 64), (40, 33), 2), ((256, 256), (128, 128), 1),
-                                     ((8, 8, 8), None, 2), ((16, 32, 64), (16, 20, 64), 1), ((64, 64, 64), None, 1)])
+                                     ((8, 8, 8), None, 2), ((16, 32, 64), (16, 20, 64), 1), ((64, 64, 64), None, 1),
+                                     ((16, 256), None, 3), ((32, 64, 128), None, 2)])
 def test_rfftn_matches_numpy(dev, s, ins, B):
     from augur_amd import ops
     rng = np.random.default_rng(sum(s))
