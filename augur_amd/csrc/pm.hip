@@ -18,6 +18,8 @@
 // D (cells) is the cumulative max drift since the last sort; when it would
 // exceed 1 the gated re-sort kernels run (they are always launched and exit
 // at once when not needed, so the step has no host sync and is capturable).
+#include <algorithm>
+
 #include "fft_core.hpp"
 
 using namespace augur;
@@ -1938,9 +1940,21 @@ int augur_pm_solve_grids(const float* rho, float* force, int64_t B, int64_t G, f
   b.s1 = c.take<float2>(nc);
   b.s2 = c.take<float2>(nc);
   AUGUR_REQUIRE(c.ok(), AUGUR_EWORKSPACE, "pm_solve_grids: workspace too small (%zu < %zu)", ws_bytes, c.used);
-  b.rho = const_cast<float*>(rho);  // read only by the solve
-  b.force = force;
-  return pm_solve(G, L, Gconst, b, as_stream(stream), (int)B);
+  // the grids in groups whose three spectra (at most 128 MB, or one grid)
+  // stay in the 256 MB Infinity Cache between the five passes, every group
+  // on the first group's spectra; each grid's arithmetic is unchanged (at 64
+  // x 128^3: 2.66 ms in one group, 1.98 ms in groups of 4, the same bits)
+  const int64_t G3 = G * G * G;
+  const int64_t per_grid = 3 * G * G * pm_pitch(G) * (int64_t)sizeof(float2);
+  const int64_t grp = std::max<int64_t>(1, std::min<int64_t>(B, ((int64_t)128 << 20) / per_grid));
+  for (int64_t g0 = 0; g0 < B; g0 += grp) {
+    PmBufs bg = b;
+    bg.rho = const_cast<float*>(rho) + g0 * G3;  // read only by the solve
+    bg.force = force + g0 * 3 * G3;
+    rc = pm_solve(G, L, Gconst, bg, as_stream(stream), (int)std::min(grp, B - g0));
+    if (rc) return rc;
+  }
+  return AUGUR_OK;
 }
 
 // The workspace of the ragged tiled deposit, carved in this order.
